@@ -1,0 +1,188 @@
+/*
+ * ncpamodes.h -- C ABI of the MI355X effective-sound-speed normal-mode library.
+ *
+ * Every entry point returns an int status (NCPA_OK == 0, negative on error; the message
+ * is available from ncpa_ctx_error()).  No C++ exception crosses this boundary.  Host
+ * pointers are owned by the caller; device buffers passed in ("dev_" arguments) are
+ * caller-allocated device memory; buffers the library allocates live behind handles.
+ * Work is ordered on the context's stream; one context per host thread.
+ *
+ * Reference interfaces replaced (ncpaprop, chetzer-ncpa/ncpaprop-release):
+ *   EigenEngine::doESSCalculation        src/libmodes/EigenEngine.h:13-14, EigenEngine.cpp:10-172
+ *   ESSModeSolver::setParams/getModalTrace  src/libmodes/ESSModeSolver.cpp:30-240, 518-645
+ *   ESSModeSolver::doSelect              src/libmodes/ESSModeSolver.cpp:647-665
+ *   ModeSolver::doPerturb                src/libmodes/ModeSolver.cpp:263-284
+ *   ModeSolver::getTLoss1D / getTLoss1DNx2  src/libmodes/ModeSolver.cpp:292-381, 387-489
+ *   ModeSolver::getTLoss2D               src/libmodes/ModeSolver.cpp:492-565
+ *   ModalBroadbandPropagator::compute_modal_sum  src/libbroadband/ModalBroadbandPropagator.cpp:245-277
+ *   BroadbandPropagator::fft_pulse_prop  src/libbroadband/BroadbandPropagator.cpp:573-651
+ */
+#ifndef NCPAMODES_H
+#define NCPAMODES_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NCPA_OK 0
+#define NCPA_ERR_ARG (-1)
+#define NCPA_ERR_HIP (-2)
+#define NCPA_ERR_ALLOC (-3)
+#define NCPA_ERR_MAX_MODES (-4)
+#define NCPA_ERR_STATE (-5)
+
+#define NCPA_MAX_MODES 4000   /* src/libmodes/ESSModeSolver.cpp:17 */
+
+typedef struct ncpa_ctx ncpa_ctx_t;
+typedef struct ncpa_modes ncpa_modes_t;
+
+/* ---------------------------------------------------------------------------------- */
+/* context                                                                             */
+/* ---------------------------------------------------------------------------------- */
+
+/* Create a context on HIP device `device`.  `hip_stream` may be NULL (the library then
+ * creates its own stream, default priority) or an existing hipStream_t to share. */
+int ncpa_ctx_create(int device, void *hip_stream, ncpa_ctx_t **out);
+void ncpa_ctx_destroy(ncpa_ctx_t *ctx);
+const char *ncpa_ctx_error(const ncpa_ctx_t *ctx);
+int ncpa_ctx_synchronize(ncpa_ctx_t *ctx);
+/* Returns the hipStream_t the context launches on. */
+void *ncpa_ctx_stream(ncpa_ctx_t *ctx);
+
+/* Per-kernel timing with HIP events recorded on the context stream around each launch of
+ * the named kernel family ("modes", "tl1d", "setup", "interval", "rows", "field2d",
+ * "bb_transfer", "fft").  ncpa_ctx_kernel_time synchronises and returns the summed
+ * duration (ms) and the number of timed launches since the last reset. */
+int ncpa_ctx_set_timing(ncpa_ctx_t *ctx, int enable);
+int ncpa_ctx_kernel_time(ncpa_ctx_t *ctx, const char *kernel, double *total_ms, int *launches);
+int ncpa_ctx_reset_timing(ncpa_ctx_t *ctx);
+
+/* Device memory owned by the caller (allocated on the context's device).  Copies are
+ * ordered on the context stream; download and upload return once the host buffer may be
+ * reused. */
+int ncpa_dev_alloc(ncpa_ctx_t *ctx, size_t bytes, void **dev_ptr);
+int ncpa_dev_free(ncpa_ctx_t *ctx, void *dev_ptr);
+int ncpa_dev_upload(ncpa_ctx_t *ctx, void *dev_dst, const void *host_src, size_t bytes);
+int ncpa_dev_download(ncpa_ctx_t *ctx, void *host_dst, const void *dev_src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel ("copy"); bytes must be a multiple of 32.
+ * Used to calibrate hardware byte counters against a kernel of known traffic. */
+int ncpa_dev_copy(ncpa_ctx_t *ctx, void *dev_dst, const void *dev_src, size_t bytes);
+
+/* ---------------------------------------------------------------------------------- */
+/* atmosphere profiles (node arrays in SI units, as ESSModeSolver::setParams leaves them)*/
+/* ---------------------------------------------------------------------------------- */
+
+/* Upload one profile.  z [m] ascending; c0, u, v [m/s]; T [K]; P [Pa]; rho [kg/m3].
+ * Wind speed/direction and the node-level quantities of Atmosphere1D.cpp:484-610 are
+ * formed on the device.  Returns a profile id (>= 0) in *profile_id. */
+int ncpa_profile_upload(ncpa_ctx_t *ctx, int n_nodes, const double *z, const double *c0,
+                        const double *u, const double *v, const double *T, const double *P,
+                        const double *rho, int *profile_id);
+/* Optional user attenuation (--use_attn_file): alpha [Np/m] at the profile nodes. */
+int ncpa_profile_set_attenuation(ncpa_ctx_t *ctx, int profile_id, const double *alpha_nodes);
+int ncpa_profile_clear(ncpa_ctx_t *ctx);
+
+typedef struct ncpa_profile_info {
+    int n_nodes;
+    int has_z0;        /* scalar Z0 present */
+    double z0;         /* ground height [m] */
+    double z_bottom;   /* first altitude [m] */
+    double z_top;      /* last altitude [m] */
+} ncpa_profile_info_t;
+
+/* Read an Atmosphere1D text profile (Atmosphere1D(file, headerfile),
+ * src/atmosphere/Atmosphere1D.cpp:44-60,85-285; units src/common/units.cpp:131-178), convert
+ * to SI, choose c0 as ESSModeSolver.cpp:159-183 does, and upload it.  header_path may be
+ * NULL or "" (the header is then read from `path`). */
+int ncpa_profile_read(ncpa_ctx_t *ctx, const char *path, const char *header_path, int *profile_id,
+                      ncpa_profile_info_t *info);
+/* Host copies of a profile's SI node arrays (each n_nodes long; any pointer may be NULL):
+ * z, c0, wind speed, wind direction [deg CW from N], T, P, rho. */
+int ncpa_profile_nodes(ncpa_ctx_t *ctx, int profile_id, double *z, double *c0, double *ws,
+                       double *wd_deg, double *T, double *P, double *rho);
+/* Lamb-wave ground admittance -rho'(z)/rho(z)/2 from the Steffen spline of rho
+ * (ESSModeSolver.cpp:347-349). */
+int ncpa_profile_admittance(ncpa_ctx_t *ctx, int profile_id, double z_min, double *admittance);
+
+/* ---------------------------------------------------------------------------------- */
+/* batched ESS solve: grid -> modal trace -> eigenvalues -> selection -> perturbation   */
+/* ---------------------------------------------------------------------------------- */
+
+/* Vertical grid shared by a batch (ESSModeSolver.cpp:144-151): rows Hgt[i] = z_min + i dz. */
+typedef struct ncpa_grid {
+    int Nz;            /* --Nz_grid */
+    double z_min;      /* [m], after the Z0 adjustment */
+    double dz;         /* (maxheight - z_min)/(Nz - 1), maxheight already clamped */
+    double maxheight;  /* [m] */
+} ncpa_grid_t;
+
+/* The grid ESSModeSolver::setParams builds for a profile: maxheight clamped to the profile
+ * top - 1e-6 m (:144-148), z_min raised to the profile's Z0 (:212-220). */
+int ncpa_grid_make(ncpa_ctx_t *ctx, int profile_id, int Nz, double maxheight_km,
+                   double zground_km, ncpa_grid_t *grid);
+/* Steffen samples of rho, P, T on the grid rows (host arrays of Nz, may be NULL). */
+int ncpa_profile_sample(ncpa_ctx_t *ctx, int profile_id, const ncpa_grid_t *grid, double *rho,
+                        double *P, double *T, double *unused);
+
+/* One (profile, frequency, azimuth) problem. */
+typedef struct ncpa_problem {
+    int profile;          /* id from ncpa_profile_upload */
+    int flags;            /* NCPA_F_* */
+    double freq;          /* [Hz] */
+    double azimuth;       /* [deg CW from N] */
+    double alpha_freq;    /* frequency at which Sutherland-Bass attenuation is evaluated */
+    double zs;            /* source height AGL [m] */
+    double zr;            /* receiver height AGL [m] */
+    double admittance;    /* ground admittance (0 for rigid ground) */
+    double c_min, c_max;  /* phase-speed filter [m/s] (NCPA_F_WVNUM_FILTER) */
+} ncpa_problem_t;
+
+#define NCPA_F_WKB          1  /* WKB k_max cut-off (ground-to-ground 1-D runs) */
+#define NCPA_F_WVNUM_FILTER 2  /* k_min = w/c_max, k_max = w/c_min */
+
+/* Solve a batch of problems; results stay on the device behind *out.
+ * Replaces, per problem, getModalTrace + doESSCalculation + doSelect + doPerturb for the
+ * mode data the 1-D TL and dispersion outputs need: k_pert and psi(zs) psi(zr). */
+int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems,
+                   int n_problems, ncpa_modes_t **out);
+void ncpa_modes_free(ncpa_modes_t *modes);
+
+/* Mode counts per problem (host array of n_problems) and total. */
+int ncpa_modes_counts(ncpa_modes_t *modes, int *counts, int *total);
+/* Per-problem interval [k_min, k_max] actually used (host arrays, may be NULL). */
+int ncpa_modes_interval(ncpa_modes_t *modes, double *k_min, double *k_max);
+/* Download modes (host arrays of length total, problem-major, ascending k within a
+ * problem): k_pert real/imag, psi(zs)*psi(zr), psi(zs), psi(zr) (sign convention:
+ * psi(zs) >= 0); any pointer may be NULL. */
+int ncpa_modes_download(ncpa_modes_t *modes, double *k_re, double *k_im, double *psi_sr,
+                        double *psi_s, double *psi_r);
+/* Sweeps each eigenvalue needed (host int[total]); negative when its bracket collapsed
+ * to working precision before Newton converged. */
+int ncpa_modes_iterations(ncpa_modes_t *modes, int *iters);
+/* Device views (valid until ncpa_modes_free): offsets[n_problems+1] (int), and per mode
+ * k2, absorption, psi_s^2, psi_r^2 (double). */
+int ncpa_modes_device(ncpa_modes_t *modes, const int **dev_offsets, const double **dev_k2,
+                      const double **dev_absorption, const double **dev_psi_sr);
+
+/* 1-D transmission loss for every problem of `modes` at ranges r_i = (i+1) dr,
+ * i < n_r (ModeSolver.cpp:325-356).  Outputs are device buffers of n_problems*n_r
+ * doubles: Re p, Im p and the incoherent sum, scaled by 1000 and 4 pi as the reference
+ * writes them.  lossless != 0 uses Re(k_pert). */
+int ncpa_tl1d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, double dr, int n_r, int lossless,
+              double *dev_p_re, double *dev_p_im, double *dev_incoh);
+
+/* Drop-in for EigenEngine::doESSCalculation (EigenEngine.h:13-14): host diag[Nz]
+ * (the (omega/c_eff)^2 trace with the ground term), all eigenpairs with k^2 in
+ * [k_min^2, k_max^2]; k2[nconv] ascending; v row-major v[j*ldv + mode] = x_j/sqrt(dz)
+ * (sum psi^2 dz = 1).  tol is accepted for signature compatibility (the solver converges
+ * to working precision).  Returns NCPA_ERR_MAX_MODES if more than max_modes modes. */
+int ncpa_ess_eigen(ncpa_ctx_t *ctx, const double *diag, int Nz, double dz, double tol,
+                   const double *k_min, const double *k_max, int max_modes, int *nconv,
+                   double *k2, double *v, int ldv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NCPAMODES_H */

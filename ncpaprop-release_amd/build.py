@@ -1,0 +1,60 @@
+"""Build the ncpamodes HIP library (gfx950) in-tree.
+
+    python ncpaprop-release_amd/build.py            # libncpamodes.so next to this file
+
+Every HIP source is compiled with hipcc for gfx950 only; packed-FP32 instruction
+selection is disabled (see DESIGN.md, MI355X notes).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libncpamodes.so")
+SOURCES = ["csrc/ctx.cpp", "csrc/ess.hip", "host/atmosphere.cpp"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def sources():
+    return [os.path.join(HERE, s) for s in SOURCES]
+
+
+def build(verbose=False, jobs=8):
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs, objs = [], []
+    for src in sources():
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        objs.append(obj)
+        cmd = [HIPCC, "-x", "hip", "-c", src, "-o", obj, "-I", CSRC,
+               "-I", os.path.join(ROOT, "include")] + FLAGS
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        text = out.decode(errors="replace")
+        if p.returncode != 0:
+            sys.stderr.write(text)
+            raise RuntimeError("hipcc failed: " + " ".join(cmd))
+        if verbose and text.strip():
+            sys.stderr.write(text)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,110 @@
+"""ctypes binding of libncpamodes.so (the C ABI declared in include/ncpamodes.h).
+
+The library is loaded from the package directory.  There is no fallback: if the HIP
+library is missing or fails to load, every entry point raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libncpamodes.so")
+
+NCPA_OK = 0
+NCPA_ERR_ARG = -1
+NCPA_ERR_HIP = -2
+NCPA_ERR_ALLOC = -3
+NCPA_ERR_MAX_MODES = -4
+NCPA_ERR_STATE = -5
+NCPA_MAX_MODES = 4000
+
+NCPA_F_WKB = 1
+NCPA_F_WVNUM_FILTER = 2
+
+# Every symbol include/ncpamodes.h declares (checked by tests/test_abi.py).
+EXPORTS = [
+    "ncpa_ctx_create", "ncpa_ctx_destroy", "ncpa_ctx_error", "ncpa_ctx_synchronize",
+    "ncpa_ctx_stream", "ncpa_ctx_set_timing", "ncpa_ctx_kernel_time", "ncpa_ctx_reset_timing",
+    "ncpa_dev_alloc", "ncpa_dev_free", "ncpa_dev_upload", "ncpa_dev_download", "ncpa_dev_copy",
+    "ncpa_profile_upload", "ncpa_profile_set_attenuation", "ncpa_profile_clear",
+    "ncpa_profile_read", "ncpa_profile_nodes", "ncpa_profile_admittance",
+    "ncpa_grid_make", "ncpa_profile_sample",
+    "ncpa_ess_solve", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
+    "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_tl1d", "ncpa_ess_eigen",
+]
+
+
+class Grid(C.Structure):
+    _fields_ = [("Nz", C.c_int), ("z_min", C.c_double), ("dz", C.c_double),
+                ("maxheight", C.c_double)]
+
+
+class Problem(C.Structure):
+    _fields_ = [("profile", C.c_int), ("flags", C.c_int), ("freq", C.c_double),
+                ("azimuth", C.c_double), ("alpha_freq", C.c_double), ("zs", C.c_double),
+                ("zr", C.c_double), ("admittance", C.c_double), ("c_min", C.c_double),
+                ("c_max", C.c_double)]
+
+
+class ProfileInfo(C.Structure):
+    _fields_ = [("n_nodes", C.c_int), ("has_z0", C.c_int), ("z0", C.c_double),
+                ("z_bottom", C.c_double), ("z_top", C.c_double)]
+
+
+_lib = None
+
+
+class NcpaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("ncpamodes error %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Load libncpamodes.so (raises OSError if it is missing: there is no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError("libncpamodes.so not built (run __graft_entry__.build()): " + LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    dp, ip, vp = P(C.c_double), P(C.c_int), C.c_void_p
+    sig = {
+        "ncpa_ctx_create": (C.c_int, [C.c_int, vp, P(vp)]),
+        "ncpa_ctx_destroy": (None, [vp]),
+        "ncpa_ctx_error": (C.c_char_p, [vp]),
+        "ncpa_ctx_synchronize": (C.c_int, [vp]),
+        "ncpa_ctx_stream": (vp, [vp]),
+        "ncpa_ctx_set_timing": (C.c_int, [vp, C.c_int]),
+        "ncpa_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
+        "ncpa_ctx_reset_timing": (C.c_int, [vp]),
+        "ncpa_dev_alloc": (C.c_int, [vp, C.c_size_t, P(vp)]),
+        "ncpa_dev_free": (C.c_int, [vp, vp]),
+        "ncpa_dev_upload": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "ncpa_dev_download": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "ncpa_dev_copy": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "ncpa_profile_upload": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp, dp, ip]),
+        "ncpa_profile_set_attenuation": (C.c_int, [vp, C.c_int, dp]),
+        "ncpa_profile_clear": (C.c_int, [vp]),
+        "ncpa_profile_read": (C.c_int, [vp, C.c_char_p, C.c_char_p, ip, P(ProfileInfo)]),
+        "ncpa_profile_nodes": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "ncpa_profile_admittance": (C.c_int, [vp, C.c_int, C.c_double, dp]),
+        "ncpa_grid_make": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, P(Grid)]),
+        "ncpa_profile_sample": (C.c_int, [vp, C.c_int, P(Grid), dp, dp, dp, dp]),
+        "ncpa_ess_solve": (C.c_int, [vp, P(Grid), P(Problem), C.c_int, P(vp)]),
+        "ncpa_modes_free": (None, [vp]),
+        "ncpa_modes_counts": (C.c_int, [vp, ip, ip]),
+        "ncpa_modes_interval": (C.c_int, [vp, dp, dp]),
+        "ncpa_modes_download": (C.c_int, [vp, dp, dp, dp, dp, dp]),
+        "ncpa_modes_iterations": (C.c_int, [vp, ip]),
+        "ncpa_modes_device": (C.c_int, [vp, P(vp), P(vp), P(vp), P(vp)]),
+        "ncpa_tl1d": (C.c_int, [vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp]),
+        "ncpa_ess_eigen": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int,
+                                     ip, dp, dp, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib

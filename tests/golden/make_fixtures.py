@@ -1,0 +1,47 @@
+"""Extract the reference-held golden data used by the oracle tests into tests/golden/.
+
+Run in a container that has the reference checkout (read-only) at /root/reference:
+
+    python tests/golden/make_fixtures.py
+
+Copies (data files only, unchanged or line subsets):
+  * samples/NCPA_canonical_profile_zuvwtdp.dat                         -> canonical_profile.dat
+  * docs/.../modess_gp_files/code_outputs/tloss_1d.nm, tloss_1d.lossless.nm
+        (Modess, canonical profile, 0.1 Hz, az 90, --write_2d_tloss; pre-2.0.1 scaling)
+  * samples/disp_256freqs_0p5hz_NCPA_canonic.dat, lines for 6 frequencies -> disp256_subset.dat
+  * docs/.../modbb_gp_files/code_outputs/myDispersionFile.dat, 4 lines -> disp250_subset.dat
+  * docs/.../modbb_gp_files/code_outputs/source_spectrum.dat            -> source_spectrum.dat
+"""
+import os
+import shutil
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODESS = os.path.join(REF, "docs/ncpaprop_run_manual_TeX/figs/modess_gp_files/code_outputs")
+MODBB = os.path.join(REF, "docs/ncpaprop_run_manual_TeX/figs/modbb_gp_files/code_outputs")
+
+
+def subset(src, dst, line_numbers):
+    with open(src) as fh:
+        lines = fh.readlines()
+    with open(dst, "w") as fh:
+        for n in line_numbers:
+            fh.write(lines[n])
+
+
+def main():
+    shutil.copyfile(os.path.join(REF, "samples/NCPA_canonical_profile_zuvwtdp.dat"),
+                    os.path.join(HERE, "canonical_profile.dat"))
+    for f in ("tloss_1d.nm", "tloss_1d.lossless.nm"):
+        shutil.copyfile(os.path.join(MODESS, f), os.path.join(HERE, f))
+    # 0-based line indices: f = (i+1)/512 Hz
+    subset(os.path.join(REF, "samples/disp_256freqs_0p5hz_NCPA_canonic.dat"),
+           os.path.join(HERE, "disp256_subset.dat"), [0, 4, 25, 64, 127, 255])
+    subset(os.path.join(MODBB, "myDispersionFile.dat"), os.path.join(HERE, "disp250_subset.dat"),
+           [0, 4, 62, 249])
+    shutil.copyfile(os.path.join(MODBB, "source_spectrum.dat"),
+                    os.path.join(HERE, "source_spectrum.dat"))
+
+
+if __name__ == "__main__":
+    main()

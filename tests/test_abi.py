@@ -1,0 +1,35 @@
+"""The C-ABI library loads and exports every entry point include/ncpamodes.h declares."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "ncpamodes.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ncpa_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_binding_list():
+    from ncpamodes import _lib
+    assert declared() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    from ncpamodes import _lib
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in declared() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_no_device_context_fails_loudly(monkeypatch):
+    # without a GPU the context must fail with an error code, never fall back to the CPU
+    import ncpamodes
+    monkeypatch.setenv("HIP_VISIBLE_DEVICES", "")
+    try:
+        ctx = ncpamodes.Context(0)
+    except ncpamodes.NcpaError:
+        return
+    ctx.close()   # a real device is present (GPU box): nothing to check here
