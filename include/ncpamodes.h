@@ -166,12 +166,31 @@ int ncpa_modes_iterations(ncpa_modes_t *modes, int *iters);
 int ncpa_modes_device(ncpa_modes_t *modes, const int **dev_offsets, const double **dev_k2,
                       const double **dev_absorption, const double **dev_psi_sr);
 
+/* Eigenvectors of every mode at grid rows row0 + k*stride (k < nrows), normalised so that
+ * sum psi^2 dz = 1 with psi(zs) >= 0: dev_psi[k * total + mode] (device, nrows*total
+ * doubles).  Used for 2-D fields and mode files (ModeSolver.cpp:513-540, 618-641). */
+int ncpa_modes_rows(ncpa_ctx_t *ctx, ncpa_modes_t *modes, int row0, int stride, int nrows,
+                    double *dev_psi);
+
 /* 1-D transmission loss for every problem of `modes` at ranges r_i = (i+1) dr,
  * i < n_r (ModeSolver.cpp:325-356).  Outputs are device buffers of n_problems*n_r
  * doubles: Re p, Im p and the incoherent sum, scaled by 1000 and 4 pi as the reference
  * writes them.  lossless != 0 uses Re(k_pert). */
 int ncpa_tl1d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, double dr, int n_r, int lossless,
               double *dev_p_re, double *dev_p_im, double *dev_incoh);
+
+/* Contraction precisions of the 2-D field */
+#define NCPA_PREC_FP64   0   /* FP64 MFMA, complex128 output */
+#define NCPA_PREC_FP32   1   /* FP32 MFMA, complex64 output */
+#define NCPA_PREC_BF16X3 2   /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulate, complex64 */
+
+/* 2-D field of every problem (ModeSolver::getTLoss2D, ModeSolver.cpp:524-544) at the rows of
+ * dev_psi_rows (from ncpa_modes_rows, nrows x total) and ranges r_i = (i+1) dr, i < n_r:
+ * dev_out[p][k][2i + {0,1}] = Re, Im of p(r_i, z_k), scaled by 1000 and 4 pi as written by
+ * the reference; element type double (FP64) or float (FP32, BF16X3).  lossless != 0 uses
+ * Re(k_pert).  The phase k r is formed in FP64 and reduced mod 2 pi in every precision. */
+int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, const double *dev_psi_rows, int nrows,
+                 double dr, int n_r, int lossless, int precision, void *dev_out);
 
 /* Drop-in for EigenEngine::doESSCalculation (EigenEngine.h:13-14): host diag[Nz]
  * (the (omega/c_eff)^2 trace with the ground term), all eigenpairs with k^2 in

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -313,6 +314,12 @@ __global__ void __launch_bounds__(256) k_interval(const ProbDev *__restrict__ pr
 // ------------------------------------------------------------------------------------
 // eigenvalues + sensitivities: one 64-lane block per (problem, 64 consecutive indices)
 // ------------------------------------------------------------------------------------
+// Sturm counts at two shifts (lane 0: lo, lane 1: hi) for one matrix
+__global__ void k_counts2(const double *__restrict__ a, int n, double b2, double lo, double hi,
+                          int *__restrict__ out) {
+    if (threadIdx.x < 2) out[threadIdx.x] = sturm_count(as_const(a), n, b2, threadIdx.x ? hi : lo);
+}
+
 struct ModeProb {
     const double *a;
     const double *w;
@@ -412,6 +419,183 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     psir_out[o] = (usr < 0.0) ? -pr : pr;
     psisr_out[o] = usr * P.inv_dz;
     iters_out[o] = my_iters;
+}
+
+
+// ------------------------------------------------------------------------------------
+// eigenvectors at a strided set of rows (plus the source row)
+// ------------------------------------------------------------------------------------
+// Slots 0..nrec-1 are rows row0 + k*stride, slot nrec the source row.  For each mode (lane)
+// k_rows records theta_{i-1} (forward continuant from the ground) and phi_{i+1} (backward
+// continuant from the top) at every recorded row i, with their power-of-two exponents, and
+// q_0 = d det/ds at the eigenvalue.  k_rows_combine then forms
+//     u_i^2 = -theta_{i-1} phi_{i+1} / q_0                 (adjugate diagonal)
+// picks the row of largest |u| as reference r and writes
+//     u_i = [-(-b)^d theta_{lo-1} phi_{hi+1} / q_0] / u_r,  (lo, hi) = sorted(i, r)
+// so every factor comes from the direction in which its recurrence is stable.  Signs are
+// fixed so that u(source row) >= 0, the convention psi(zs) >= 0 of the mode products.
+struct RowsArgs {
+    int row0, stride, nrec;
+    size_t ld;      // scratch / output leading dimension (total modes)
+};
+
+__global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
+                                             const double *__restrict__ k2, RowsArgs ra,
+                                             double *__restrict__ TH, int *__restrict__ THE,
+                                             double *__restrict__ PH, int *__restrict__ PHE,
+                                             double *__restrict__ Q0, int *__restrict__ QEX) {
+    const ModeWork wk = work[blockIdx.x];
+    const ModeProb P = mp[wk.prob];
+    const int j = wk.j0 + (int)threadIdx.x;
+    const bool active = j < P.n_hi;
+    const size_t o = (size_t)P.out_off + (size_t)(active ? (j - P.n_lo) : 0);
+    const double s = active ? k2[o] : 0.5 * (P.lo + P.hi);
+    const int n = P.n;
+    const double b2 = P.b2;
+    const cdptr a = as_const(P.a);
+    const int last_rec = ra.row0 + (ra.nrec - 1) * ra.stride;
+    // forward: theta_k for k = 0 .. n-2, stored at row k+1
+    {
+        double t1 = 1.0, t2 = 0.0;
+        int ex = 0;
+        int slot = 0, next = ra.row0;
+        if (next == 0) {
+            if (active) { TH[o] = 1.0; THE[o] = 0; }
+            slot = 1;
+            next = ra.nrec > 1 ? ra.row0 + ra.stride : INT_MAX;
+        }
+        if (P.srow == 0 && active) { TH[(size_t)ra.nrec * ra.ld + o] = 1.0; THE[(size_t)ra.nrec * ra.ld + o] = 0; }
+        sweep_up(
+            a, n - 1,
+            [&](double ak, int k) {
+                const double t0 = fma(ak - s, t1, -b2 * t2);
+                t2 = t1;
+                t1 = t0;
+                if (k + 1 == next) {
+                    if (active) {
+                        TH[(size_t)slot * ra.ld + o] = t1;
+                        THE[(size_t)slot * ra.ld + o] = ex;
+                    }
+                    ++slot;
+                    next = slot < ra.nrec ? next + ra.stride : INT_MAX;
+                }
+                if (k + 1 == P.srow && active) {
+                    TH[(size_t)ra.nrec * ra.ld + o] = t1;
+                    THE[(size_t)ra.nrec * ra.ld + o] = ex;
+                }
+            },
+            [&]() {
+                const int kk = pow2_exponent(t1);
+                t1 = ldexp(t1, -kk);
+                t2 = ldexp(t2, -kk);
+                ex += kk;
+            });
+    }
+    // backward: phi_i for i = n-1 .. 0, stored at row i-1; q for the normalisation
+    {
+        double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
+        int ex = 0;
+        int slot = ra.nrec - 1, next = last_rec;
+        if (next == n - 1) {
+            if (active) { PH[(size_t)slot * ra.ld + o] = 1.0; PHE[(size_t)slot * ra.ld + o] = 0; }
+            --slot;
+            next = slot >= 0 ? next - ra.stride : INT_MIN;
+        }
+        if (P.srow == n - 1 && active) { PH[(size_t)ra.nrec * ra.ld + o] = 1.0; PHE[(size_t)ra.nrec * ra.ld + o] = 0; }
+        sweep_down(
+            a, n,
+            [&](double ai, int i) {
+                const double am = ai - s;
+                const double p0 = fma(am, p1, -b2 * p2);
+                const double q0 = fma(am, q1, fma(-b2, q2, -p1));
+                p2 = p1;
+                p1 = p0;
+                q2 = q1;
+                q1 = q0;
+                if (i - 1 == next) {
+                    if (active) {
+                        PH[(size_t)slot * ra.ld + o] = p1;
+                        PHE[(size_t)slot * ra.ld + o] = ex;
+                    }
+                    --slot;
+                    next = slot >= 0 ? next - ra.stride : INT_MIN;
+                }
+                if (i - 1 == P.srow && active) {
+                    PH[(size_t)ra.nrec * ra.ld + o] = p1;
+                    PHE[(size_t)ra.nrec * ra.ld + o] = ex;
+                }
+            },
+            [&](int) {
+                const int k = pow2_exponent(p1);
+                p1 = ldexp(p1, -k);
+                p2 = ldexp(p2, -k);
+                q1 = ldexp(q1, -k);
+                q2 = ldexp(q2, -k);
+                ex += k;
+            });
+        if (active) {
+            Q0[o] = q1;
+            QEX[o] = ex;
+        }
+    }
+}
+
+__device__ __forceinline__ double log2abs_s(double v, int e) { return log2(fabs(v)) + (double)e; }
+
+// one thread per mode; psi[k * ld + o] = u_k / sqrt(dz)
+__global__ void __launch_bounds__(256) k_rows_combine(const double *__restrict__ TH, const int *__restrict__ THE,
+                                                      const double *__restrict__ PH, const int *__restrict__ PHE,
+                                                      const double *__restrict__ Q0, const int *__restrict__ QEX,
+                                                      const int *__restrict__ mode_prob,
+                                                      const ModeProb *__restrict__ mp, RowsArgs ra,
+                                                      size_t n_modes, double *__restrict__ psi,
+                                                      double *__restrict__ psi_src) {
+    const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (o >= n_modes) return;
+    const ModeProb P = mp[mode_prob[o]];
+    const double q0 = Q0[o];
+    const double lq = log2abs_s(q0, QEX[o]);
+    const double sq = copysign(1.0, q0);
+    // reference row: largest u^2 among the recorded rows
+    int r = -1;
+    double l2r = -INFINITY;
+    for (int k = 0; k < ra.nrec; ++k) {
+        const double t = TH[(size_t)k * ra.ld + o], p = PH[(size_t)k * ra.ld + o];
+        if (t == 0.0 || p == 0.0) continue;
+        if (-copysign(1.0, t) * copysign(1.0, p) * sq <= 0.0) continue;
+        const double l2 = log2abs_s(t, THE[(size_t)k * ra.ld + o]) + log2abs_s(p, PHE[(size_t)k * ra.ld + o]) - lq;
+        if (l2 > l2r) { l2r = l2; r = k; }
+    }
+    const double inv_sqrt_dz = sqrt(P.inv_dz);
+    if (r < 0) {
+        for (int k = 0; k < ra.nrec; ++k) psi[(size_t)k * ra.ld + o] = 0.0;
+        psi_src[o] = 0.0;
+        return;
+    }
+    const int row_r = ra.row0 + r * ra.stride;
+    const double tr = TH[(size_t)r * ra.ld + o], pr = PH[(size_t)r * ra.ld + o];
+    const int ter = THE[(size_t)r * ra.ld + o], per = PHE[(size_t)r * ra.ld + o];
+    auto u_at = [&](double t, int te, double p, int pe, int row) -> double {
+        // u_row u_r / u_r with the adjugate product for the sorted pair
+        double tv, pv;
+        int tev, pev, d;
+        if (row <= row_r) { tv = t; tev = te; pv = pr; pev = per; d = row_r - row; }
+        else { tv = tr; tev = ter; pv = p; pev = pe; d = row - row_r; }
+        if (tv == 0.0 || pv == 0.0) return 0.0;
+        const double sg = ((d & 1) ? 1.0 : -1.0) * copysign(1.0, tv) * copysign(1.0, pv) * sq;
+        const double l2 = log2abs_s(tv, tev) + log2abs_s(pv, pev) + d * P.log2b - lq - 0.5 * l2r;
+        return sg * exp2(l2);
+    };
+    const size_t so = (size_t)ra.nrec * ra.ld + o;
+    const double us = u_at(TH[so], THE[so], PH[so], PHE[so], P.srow);
+    const double flip = (us < 0.0) ? -inv_sqrt_dz : inv_sqrt_dz;
+    // psi(zs) from the reference row: accurate also for modes trapped aloft, where the
+    // square root of the adjugate diagonal at the ground would sit at its noise floor
+    psi_src[o] = fabs(us) * inv_sqrt_dz;
+    for (int k = 0; k < ra.nrec; ++k) {
+        const size_t ko = (size_t)k * ra.ld + o;
+        psi[ko] = flip * u_at(TH[ko], THE[ko], PH[ko], PHE[ko], ra.row0 + k * ra.stride);
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -524,10 +708,16 @@ struct ncpa_modes {
     ncpa_grid_t grid{};
     std::vector<int> counts, offsets;
     std::vector<IntervalOut> iv;
+    std::vector<ModeProb> mp;                  // host copy of the per-problem records
     int *d_offsets = nullptr;
     double *d_k2 = nullptr, *d_abs = nullptr, *d_psis = nullptr, *d_psir = nullptr, *d_psisr = nullptr;
     int *d_iters = nullptr;
     double *d_A = nullptr, *d_W = nullptr, *d_CE = nullptr;   // per-problem grid samples
+    double *d_psis_rows = nullptr;             // psi(zs) from the last eigenvector-rows pass
+    ModeProb *d_mp = nullptr;
+    ModeWork *d_work = nullptr;
+    int *d_mode_prob = nullptr;
+    size_t n_work = 0;
 };
 
 namespace {
@@ -541,16 +731,117 @@ int check_grid(ncpa_ctx *ctx, const ncpa_grid_t *g) {
 void release_modes(ncpa_modes *m) {
     if (!m || !m->ctx) return;
     DevicePool &pl = m->ctx->pool;
-    pl.put(m->d_offsets);
-    pl.put(m->d_k2);
-    pl.put(m->d_abs);
-    pl.put(m->d_psis);
-    pl.put(m->d_psir);
-    pl.put(m->d_psisr);
-    pl.put(m->d_iters);
-    pl.put(m->d_A);
-    pl.put(m->d_W);
-    pl.put(m->d_CE);
+    void *bufs[] = {m->d_offsets, m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters,
+                    m->d_A, m->d_W, m->d_CE, m->d_mp, m->d_work, m->d_mode_prob, m->d_psis_rows};
+    for (void *b : bufs) pl.put(b);
+}
+
+// Work list and launch of k_modes for intervals already in m->iv and matrices in d_A/d_W.
+int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, const std::vector<int> &rrow) {
+    const int P = m->P, Nz = m->grid.Nz;
+    const double dz = m->grid.dz;
+    m->counts.assign(P, 0);
+    m->offsets.assign(P + 1, 0);
+    m->mp.resize(P);
+    std::vector<ModeWork> work;
+    std::vector<int> mode_prob;
+    const double h2 = dz * dz;
+    const double b = 1.0 / h2;
+    int total = 0;
+    for (int p = 0; p < P; ++p) {
+        const IntervalOut &v = m->iv[p];
+        const int nm = std::max(0, v.n_hi - v.n_lo);
+        if (nm > NCPA_MAX_MODES)
+            return set_error(ctx, NCPA_ERR_MAX_MODES,
+                             std::to_string(nm) + " modes exceed MAX_MODES=" + std::to_string(NCPA_MAX_MODES));
+        m->counts[p] = nm;
+        m->offsets[p] = total;
+        ModeProb &q = m->mp[p];
+        q.a = m->d_A + size_t(p) * Nz;
+        q.w = m->d_W + size_t(p) * Nz;
+        q.b2 = b * b;
+        q.log2b = std::log2(b);
+        q.lo = v.lo;
+        q.hi = v.hi;
+        // ||T|| <= 4/h^2 + max diag: the absolute floor of FP64 eigenvalues of T
+        const double normT = 4.0 / h2 + std::max(std::fabs(v.hi), std::fabs(v.lo));
+        q.tol = 8.0 * DBL_EPSILON * normT;
+        q.inv_dz = 1.0 / dz;
+        q.n = Nz;
+        q.srow = srow[p];
+        q.rrow = rrow[p];
+        q.n_lo = v.n_lo;
+        q.n_hi = v.n_lo + nm;
+        q.out_off = total;
+        for (int j0 = v.n_lo; j0 < v.n_lo + nm; j0 += 64) work.push_back({p, j0});
+        mode_prob.insert(mode_prob.end(), nm, p);
+        total += nm;
+    }
+    m->offsets[P] = total;
+    m->total = total;
+    m->n_work = work.size();
+    const size_t tot_alloc = std::max(total, 1);
+    m->d_k2 = pool_alloc<double>(ctx, tot_alloc);
+    m->d_abs = pool_alloc<double>(ctx, tot_alloc);
+    m->d_psis = pool_alloc<double>(ctx, tot_alloc);
+    m->d_psir = pool_alloc<double>(ctx, tot_alloc);
+    m->d_psisr = pool_alloc<double>(ctx, tot_alloc);
+    m->d_iters = pool_alloc<int>(ctx, tot_alloc);
+    m->d_offsets = pool_alloc<int>(ctx, P + 1);
+    m->d_mp = pool_alloc<ModeProb>(ctx, P);
+    m->d_work = pool_alloc<ModeWork>(ctx, std::max<size_t>(work.size(), 1));
+    m->d_mode_prob = pool_alloc<int>(ctx, tot_alloc);
+    if (!m->d_k2 || !m->d_abs || !m->d_psis || !m->d_psir || !m->d_psisr || !m->d_iters || !m->d_offsets ||
+        !m->d_mp || !m->d_work || !m->d_mode_prob)
+        return set_error(ctx, NCPA_ERR_ALLOC, "mode buffers: device allocation failed");
+    NCPA_HIP(ctx, hipMemcpyAsync(m->d_offsets, m->offsets.data(), sizeof(int) * (P + 1), hipMemcpyHostToDevice, ctx->stream));
+    NCPA_HIP(ctx, hipMemcpyAsync(m->d_mp, m->mp.data(), sizeof(ModeProb) * P, hipMemcpyHostToDevice, ctx->stream));
+    if (!work.empty()) {
+        NCPA_HIP(ctx, hipMemcpyAsync(m->d_work, work.data(), sizeof(ModeWork) * work.size(), hipMemcpyHostToDevice, ctx->stream));
+        NCPA_HIP(ctx, hipMemcpyAsync(m->d_mode_prob, mode_prob.data(), sizeof(int) * total, hipMemcpyHostToDevice, ctx->stream));
+        TimedLaunch tl(ctx, "modes");
+        hipLaunchKernelGGL(k_modes, dim3((unsigned)work.size()), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
+                           m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(ctx, e, "k_modes launch");
+    return NCPA_OK;
+}
+
+// eigenvectors of every mode at rows row0 + k*stride, k < nrec -> dev_psi[k * total + o]
+int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, double *dev_psi) {
+    if (m->total == 0) return NCPA_OK;
+    const size_t ld = m->total;
+    const size_t slots = (size_t)nrec + 1;
+    double *TH = pool_alloc<double>(ctx, slots * ld), *PH = pool_alloc<double>(ctx, slots * ld);
+    int *THE = pool_alloc<int>(ctx, slots * ld), *PHE = pool_alloc<int>(ctx, slots * ld);
+    double *Q0 = pool_alloc<double>(ctx, ld);
+    int *QEX = pool_alloc<int>(ctx, ld);
+    auto cleanup = [&]() {
+        ctx->pool.put(TH); ctx->pool.put(PH); ctx->pool.put(THE); ctx->pool.put(PHE);
+        ctx->pool.put(Q0); ctx->pool.put(QEX);
+    };
+    if (!TH || !PH || !THE || !PHE || !Q0 || !QEX) {
+        cleanup();
+        return set_error(ctx, NCPA_ERR_ALLOC, "eigenvector rows: scratch allocation failed");
+    }
+    RowsArgs ra{row0, stride, nrec, ld};
+    if (!m->d_psis_rows) m->d_psis_rows = pool_alloc<double>(ctx, ld);
+    if (!m->d_psis_rows) {
+        cleanup();
+        return set_error(ctx, NCPA_ERR_ALLOC, "eigenvector rows: allocation failed");
+    }
+    {
+        TimedLaunch tl(ctx, "rows");
+        hipLaunchKernelGGL(k_rows, dim3((unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
+                           m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
+        hipLaunchKernelGGL(k_rows_combine, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, ctx->stream, TH, THE,
+                           PH, PHE, Q0, QEX, m->d_mode_prob, m->d_mp, ra, ld, dev_psi, m->d_psis_rows);
+    }
+    hipError_t e = hipGetLastError();
+    cleanup();
+    if (e != hipSuccess) return hip_error(ctx, e, "k_rows launch");
+    return NCPA_OK;
 }
 
 }  // namespace
@@ -568,6 +859,7 @@ int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_
     const double dz = grid->dz;
     const int P = n_problems;
     std::vector<ProbDev> hp(P);
+    std::vector<int> srow(P), rrow(P);
     int max_nodes = 0;
     for (int p = 0; p < P; ++p) {
         const ncpa_problem_t &q = problems[p];
@@ -594,6 +886,8 @@ int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_
         d.rrow = (int)std::ceil(q.zr / dz);
         if (d.srow < 0 || d.srow >= Nz || d.rrow < 0 || d.rrow >= Nz)
             return set_error(ctx, NCPA_ERR_ARG, "ncpa_ess_solve: source/receiver outside the grid");
+        srow[p] = d.srow;
+        rrow[p] = d.rrow;
         max_nodes = std::max(max_nodes, pd.n);
     }
     ncpa_modes *m = new ncpa_modes();
@@ -636,78 +930,14 @@ int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_
     e = hipMemcpyAsync(m->iv.data(), d_iv, sizeof(IntervalOut) * P, hipMemcpyDeviceToHost, ctx->stream);
     if (e != hipSuccess) return fail(hip_error(ctx, e, "copy intervals"));
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(hip_error(ctx, e, "interval sync"));
+    rc = launch_modes(ctx, m, srow, rrow);
     ctx->pool.put(d_alpha);
-    d_alpha = nullptr;
     ctx->pool.put(d_iv);
-    d_iv = nullptr;
-
-    // work list: 64 consecutive eigen indices per block, blocks never span problems
-    m->counts.resize(P);
-    m->offsets.resize(P + 1);
-    std::vector<ModeProb> mp(P);
-    std::vector<ModeWork> work;
-    const double h2 = dz * dz;
-    const double b = 1.0 / h2;
-    int total = 0;
-    for (int p = 0; p < P; ++p) {
-        const IntervalOut &v = m->iv[p];
-        int nm = std::max(0, v.n_hi - v.n_lo);
-        if (nm > NCPA_MAX_MODES)
-            return fail(set_error(ctx, NCPA_ERR_MAX_MODES,
-                                  "ncpa_ess_solve: " + std::to_string(nm) + " modes exceed MAX_MODES=4000"));
-        m->counts[p] = nm;
-        m->offsets[p] = total;
-        ModeProb &q = mp[p];
-        q.a = m->d_A + size_t(p) * Nz;
-        q.w = m->d_W + size_t(p) * Nz;
-        q.b2 = b * b;
-        q.log2b = std::log2(b);
-        q.lo = v.lo;
-        q.hi = v.hi;
-        // ||T||_inf bound: max|a| <= 2/h^2 + max diag, plus two off-diagonals
-        const double normT = 4.0 / h2 + std::max(std::fabs(v.hi), std::fabs(v.lo));
-        q.tol = 8.0 * DBL_EPSILON * normT;
-        q.inv_dz = 1.0 / dz;
-        q.n = Nz;
-        q.srow = hp[p].srow;
-        q.rrow = hp[p].rrow;
-        q.n_lo = v.n_lo;
-        q.n_hi = v.n_lo + nm;
-        q.out_off = total;
-        for (int j0 = v.n_lo; j0 < v.n_lo + nm; j0 += 64) work.push_back({p, j0});
-        total += nm;
-    }
-    m->offsets[P] = total;
-    m->total = total;
-    const size_t tot_alloc = std::max(total, 1);
-    m->d_k2 = pool_alloc<double>(ctx, tot_alloc);
-    m->d_abs = pool_alloc<double>(ctx, tot_alloc);
-    m->d_psis = pool_alloc<double>(ctx, tot_alloc);
-    m->d_psir = pool_alloc<double>(ctx, tot_alloc);
-    m->d_psisr = pool_alloc<double>(ctx, tot_alloc);
-    m->d_iters = pool_alloc<int>(ctx, tot_alloc);
-    m->d_offsets = pool_alloc<int>(ctx, P + 1);
-    ModeProb *d_mp = pool_alloc<ModeProb>(ctx, P);
-    ModeWork *d_work = pool_alloc<ModeWork>(ctx, std::max<size_t>(work.size(), 1));
-    if (!m->d_k2 || !m->d_abs || !m->d_psis || !m->d_psir || !m->d_psisr || !m->d_iters || !m->d_offsets || !d_mp || !d_work) {
-        ctx->pool.put(d_mp);
-        ctx->pool.put(d_work);
-        return fail(set_error(ctx, NCPA_ERR_ALLOC, "ncpa_ess_solve: device allocation failed"));
-    }
-    (void)hipMemcpyAsync(m->d_offsets, m->offsets.data(), sizeof(int) * (P + 1), hipMemcpyHostToDevice, ctx->stream);
-    (void)hipMemcpyAsync(d_mp, mp.data(), sizeof(ModeProb) * P, hipMemcpyHostToDevice, ctx->stream);
-    if (!work.empty()) {
-        (void)hipMemcpyAsync(d_work, work.data(), sizeof(ModeWork) * work.size(), hipMemcpyHostToDevice, ctx->stream);
-        TimedLaunch tl(ctx, "modes");
-        hipLaunchKernelGGL(k_modes, dim3((unsigned)work.size()), dim3(64), 0, ctx->stream, d_mp, d_work,
-                           m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters);
-    }
-    e = hipGetLastError();
-    ctx->pool.put(d_mp);
-    ctx->pool.put(d_work);
     ctx->pool.put(d_probs);
+    d_alpha = nullptr;
+    d_iv = nullptr;
     d_probs = nullptr;
-    if (e != hipSuccess) return fail(hip_error(ctx, e, "k_modes launch"));
+    if (rc) return fail(rc);
     *out = m;
     return NCPA_OK;
 }
@@ -776,6 +1006,28 @@ int ncpa_modes_device(ncpa_modes_t *m, const int **off, const double **k2, const
     return NCPA_OK;
 }
 
+}  // extern "C"
+
+// internal accessor for the other translation units of the library
+int ncpa_modes_internal(ncpa_modes_t *m, int *P, int *total, const int **h_offsets, const int **h_counts,
+                        const double **d_psis) {
+    *P = m->P;
+    *total = m->total;
+    *h_offsets = m->offsets.data();
+    *h_counts = m->counts.data();
+    *d_psis = m->d_psis_rows;
+    return NCPA_OK;
+}
+
+extern "C" {
+
+int ncpa_modes_rows(ncpa_ctx_t *ctx, ncpa_modes_t *m, int row0, int stride, int nrows, double *dev_psi) {
+    if (!ctx || !m || !dev_psi || nrows <= 0 || stride <= 0 || row0 < 0 ||
+        row0 + (long long)(nrows - 1) * stride > m->grid.Nz - 1)
+        return set_error(ctx, NCPA_ERR_ARG, "ncpa_modes_rows: rows must lie in [0, Nz)");
+    return launch_rows(ctx, m, row0, stride, nrows, dev_psi);
+}
+
 int ncpa_tl1d(ncpa_ctx_t *ctx, ncpa_modes_t *m, double dr, int n_r, int lossless, double *p_re,
               double *p_im, double *incoh) {
     if (!ctx || !m || n_r <= 0 || !(dr > 0.0) || !p_re || !p_im || !incoh)
@@ -818,9 +1070,70 @@ int ncpa_dev_copy(ncpa_ctx_t *ctx, void *dst, const void *src, size_t bytes) {
     return NCPA_OK;
 }
 
-int ncpa_ess_eigen(ncpa_ctx_t *ctx, const double *, int, double, double, const double *,
-                   const double *, int, int *, double *, double *, int) {
-    return set_error(ctx, NCPA_ERR_STATE, "ncpa_ess_eigen: eigenvector path not built yet");
+int ncpa_ess_eigen(ncpa_ctx_t *ctx, const double *diag, int Nz, double dz, double tol, const double *k_min,
+                   const double *k_max, int max_modes, int *nconv, double *k2, double *v, int ldv) {
+    (void)tol;
+    if (!ctx || !diag || Nz < 3 || !(dz > 0.0) || !k_min || !k_max || !nconv || !k2 || !v || max_modes <= 0 ||
+        ldv < 1)
+        return set_error(ctx, NCPA_ERR_ARG, "ncpa_ess_eigen: bad arguments");
+    NCPA_HIP(ctx, hipSetDevice(ctx->device));
+    *nconv = 0;
+    // the operator of EigenEngine::doESSCalculation (EigenEngine.cpp:57-79)
+    const double h2 = dz * dz;
+    std::vector<double> a(Nz);
+    for (int i = 0; i < Nz - 1; ++i) a[i] = -2.0 / h2 + diag[i];
+    a[Nz - 1] = -2.0 / h2 + diag[Nz - 2];
+    ncpa_modes *m = new ncpa_modes();
+    m->ctx = ctx;
+    m->P = 1;
+    m->grid.Nz = Nz;
+    m->grid.dz = dz;
+    m->grid.z_min = 0.0;
+    m->grid.maxheight = dz * (Nz - 1);
+    m->d_A = pool_alloc<double>(ctx, Nz);
+    m->d_W = pool_alloc<double>(ctx, Nz);
+    int *d_cnt = pool_alloc<int>(ctx, 2);
+    double *d_psi = nullptr;
+    auto done = [&](int code) {
+        ctx->pool.put(d_cnt);
+        ctx->pool.put(d_psi);
+        release_modes(m);
+        delete m;
+        return code;
+    };
+    if (!m->d_A || !m->d_W || !d_cnt) return done(set_error(ctx, NCPA_ERR_ALLOC, "ncpa_ess_eigen: allocation failed"));
+    hipError_t e;
+    if ((e = hipMemcpyAsync(m->d_A, a.data(), sizeof(double) * Nz, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(m->d_W, 0, sizeof(double) * Nz, ctx->stream)) != hipSuccess)
+        return done(hip_error(ctx, e, "ncpa_ess_eigen upload"));
+    const double lo = (*k_min) * (*k_min), hi = (*k_max) * (*k_max);   // EPSSetInterval(k_min^2, k_max^2)
+    hipLaunchKernelGGL(k_counts2, dim3(1), dim3(64), 0, ctx->stream, m->d_A, Nz, (1.0 / h2) * (1.0 / h2), lo, hi, d_cnt);
+    int cnt[2] = {0, 0};
+    if ((e = hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return done(hip_error(ctx, e, "ncpa_ess_eigen counts"));
+    const int nm = cnt[1] - cnt[0];
+    if (nm > max_modes || nm > NCPA_MAX_MODES)
+        return done(set_error(ctx, NCPA_ERR_MAX_MODES, "ncpa_ess_eigen: " + std::to_string(nm) + " eigenpairs exceed max_modes"));
+    if (ldv < nm) return done(set_error(ctx, NCPA_ERR_ARG, "ncpa_ess_eigen: ldv smaller than the number of modes"));
+    m->iv.resize(1);
+    m->iv[0] = IntervalOut{*k_min, *k_max, lo, hi, cnt[0], cnt[1], 0, 0};
+    int rc = launch_modes(ctx, m, std::vector<int>{0}, std::vector<int>{0});
+    if (rc) return done(rc);
+    if (nm > 0) {
+        d_psi = pool_alloc<double>(ctx, (size_t)Nz * nm);
+        if (!d_psi) return done(set_error(ctx, NCPA_ERR_ALLOC, "ncpa_ess_eigen: allocation failed"));
+        rc = launch_rows(ctx, m, 0, 1, Nz, d_psi);
+        if (rc) return done(rc);
+        std::vector<double> psi((size_t)Nz * nm);
+        if ((e = hipMemcpyAsync(k2, m->d_k2, sizeof(double) * nm, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(psi.data(), d_psi, sizeof(double) * psi.size(), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+            return done(hip_error(ctx, e, "ncpa_ess_eigen download"));
+        for (int j = 0; j < Nz; ++j) std::memcpy(v + (size_t)j * ldv, psi.data() + (size_t)j * nm, sizeof(double) * nm);
+    }
+    *nconv = nm;
+    return done(NCPA_OK);
 }
 
 }  // extern "C"

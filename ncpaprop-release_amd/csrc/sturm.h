@@ -122,6 +122,30 @@ __device__ __forceinline__ void sweep_down2(cdptr a, cdptr w, int n, Step step, 
     }
 }
 
+// Drive `step(a_i, i)` over rows i = 0 .. m-1 (ascending), `chunk_end()` after every
+// 16 rows; same scalar double-buffered streaming as sweep_down.
+template <class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_up(cdptr a, int m, Step step, ChunkEnd chunk_end) {
+    const int nfull = m / NCPA_CHUNK;
+    if (nfull > 0) {
+        double buf[NCPA_CHUNK];
+#pragma unroll
+        for (int u = 0; u < NCPA_CHUNK; ++u) buf[u] = a[u];
+        for (int c = 0; c < nfull; ++c) {
+            double nxt[NCPA_CHUNK];
+            const int cn = (c + 1 < nfull) ? c + 1 : c;
+#pragma unroll
+            for (int u = 0; u < NCPA_CHUNK; ++u) nxt[u] = a[cn * NCPA_CHUNK + u];
+#pragma unroll
+            for (int u = 0; u < NCPA_CHUNK; ++u) step(buf[u], c * NCPA_CHUNK + u);
+            chunk_end();
+#pragma unroll
+            for (int u = 0; u < NCPA_CHUNK; ++u) buf[u] = nxt[u];
+        }
+    }
+    for (int i = nfull * NCPA_CHUNK; i < m; ++i) step(a[i], i);
+}
+
 // count of eigenvalues below s
 __device__ __forceinline__ int sturm_count(cdptr a, int n, double b2, double s) {
     double p1 = 1.0, p2 = 0.0;

@@ -101,6 +101,21 @@ class Context:
         return Modes(self, h, len(problems), grid)
 
 
+def ess_eigen(ctx, diag, dz, k_min, k_max, max_modes=4000):
+    """Drop-in of EigenEngine::doESSCalculation: (k2 [nconv], psi [Nz, nconv])."""
+    diag = np.ascontiguousarray(diag, dtype=np.float64)
+    Nz = diag.size
+    kmin = C.c_double(k_min)
+    kmax = C.c_double(k_max)
+    nconv = C.c_int()
+    k2 = np.zeros(max_modes)
+    v = np.zeros((Nz, max_modes))
+    ctx.check(ctx.lib.ncpa_ess_eigen(ctx.h, _dp(diag), Nz, dz, 1e-8, C.byref(kmin), C.byref(kmax),
+                                     max_modes, C.byref(nconv), _dp(k2), _dp(v), max_modes))
+    n = nconv.value
+    return k2[:n], v[:, :n]
+
+
 class DeviceArray:
     """A flat device buffer allocated through the C ABI (no torch: see DESIGN.md)."""
 
@@ -192,6 +207,41 @@ class Modes:
         k, psr, ps, pr = self.download()
         s = slice(self.offsets[p], self.offsets[p + 1])
         return k[s], psr[s], ps[s], pr[s]
+
+    def rows(self, row0, stride, nrows):
+        """Eigenvectors at rows row0 + k*stride: array [nrows, total] (psi, sum psi^2 dz = 1)."""
+        buf = self.ctx.empty(nrows * max(self.total, 1))
+        self.ctx.check(self.ctx.lib.ncpa_modes_rows(self.ctx.h, self.h, row0, stride, nrows,
+                                                    C.c_void_p(buf.ptr)))
+        out = buf.numpy()[:nrows * self.total].reshape(nrows, self.total)
+        buf.free()
+        return out
+
+    def rows_device(self, row0, stride, nrows):
+        """Eigenvector rows left on the device (DeviceArray of nrows * total doubles)."""
+        buf = self.ctx.empty(nrows * max(self.total, 1))
+        self.ctx.check(self.ctx.lib.ncpa_modes_rows(self.ctx.h, self.h, row0, stride, nrows,
+                                                    C.c_void_p(buf.ptr)))
+        return buf
+
+    def field2d_device(self, psi_rows, nrows, dr, n_r, precision=0, lossless=False, out=None):
+        """2-D field into a device buffer [P, nrows, 2 n_r] (float64 for FP64 else float32)."""
+        dt = np.float64 if precision == 0 else np.float32
+        if out is None:
+            out = self.ctx.empty(self.P * nrows * 2 * n_r, dt)
+        self.ctx.check(self.ctx.lib.ncpa_field2d(self.ctx.h, self.h, C.c_void_p(psi_rows.ptr), nrows, dr,
+                                                 n_r, 1 if lossless else 0, precision,
+                                                 C.c_void_p(out.ptr)))
+        return out
+
+    def field2d(self, row0, stride, nrows, dr, n_r, precision=0, lossless=False):
+        """Host copy of the complex field [P, nrows, n_r]."""
+        rows = self.rows_device(row0, stride, nrows)
+        out = self.field2d_device(rows, nrows, dr, n_r, precision, lossless)
+        a = out.numpy().reshape(self.P, nrows, n_r, 2)
+        out.free()
+        rows.free()
+        return a[..., 0] + 1j * a[..., 1]
 
     def tl1d(self, dr, n_r, lossless=False):
         """1-D TL of every problem: (p complex [P, n_r], incoherent [P, n_r]) on the host."""

@@ -17,6 +17,10 @@ NCPA_ERR_MAX_MODES = -4
 NCPA_ERR_STATE = -5
 NCPA_MAX_MODES = 4000
 
+NCPA_PREC_FP64 = 0
+NCPA_PREC_FP32 = 1
+NCPA_PREC_BF16X3 = 2
+
 NCPA_F_WKB = 1
 NCPA_F_WVNUM_FILTER = 2
 
@@ -29,7 +33,7 @@ EXPORTS = [
     "ncpa_profile_read", "ncpa_profile_nodes", "ncpa_profile_admittance",
     "ncpa_grid_make", "ncpa_profile_sample",
     "ncpa_ess_solve", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
-    "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_tl1d", "ncpa_ess_eigen",
+    "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d", "ncpa_ess_eigen",
 ]
 
 
@@ -97,8 +101,10 @@ def load():
         "ncpa_modes_interval": (C.c_int, [vp, dp, dp]),
         "ncpa_modes_download": (C.c_int, [vp, dp, dp, dp, dp, dp]),
         "ncpa_modes_iterations": (C.c_int, [vp, ip]),
+        "ncpa_modes_rows": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, vp]),
         "ncpa_modes_device": (C.c_int, [vp, P(vp), P(vp), P(vp), P(vp)]),
         "ncpa_tl1d": (C.c_int, [vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp]),
+        "ncpa_field2d": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, vp]),
         "ncpa_ess_eigen": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int,
                                      ip, dp, dp, C.c_int]),
     }
