@@ -182,7 +182,8 @@ int ncpa_tl1d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, double dr, int n_r, int loss
 /* Contraction precisions of the 2-D field */
 #define NCPA_PREC_FP64   0   /* FP64 MFMA, complex128 output */
 #define NCPA_PREC_FP32   1   /* FP32 MFMA, complex64 output */
-#define NCPA_PREC_BF16X3 2   /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulate, complex64 */
+#define NCPA_PREC_BF16X3 2   /* split-BF16, 2 parts, 3 products (hi*hi + hi*lo + lo*hi), FP32 accumulate */
+#define NCPA_PREC_BF16X6 3   /* split-BF16, 3 parts, 6 products (FP32-class accuracy), FP32 accumulate */
 
 /* 2-D field of every problem (ModeSolver::getTLoss2D, ModeSolver.cpp:524-544) at the rows of
  * dev_psi_rows (from ncpa_modes_rows, nrows x total) and ranges r_i = (i+1) dr, i < n_r:

@@ -56,7 +56,7 @@ __device__ __forceinline__ void f2_csqrt(double x, double y, double &re, double 
     }
 }
 
-enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
+enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3 };
 
 // B rows for every mode: 1-D grid of total * ceil(nr/256) blocks
 template <int PREC>
@@ -253,6 +253,210 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(const GemmProb *__restrict_
             }
 }
 
+
+// ------------------------------------------------------------------------------------
+// split-BF16 contraction, tuned path
+// ------------------------------------------------------------------------------------
+// Operands are split v = v0 + v1 (+ v2) into bf16 parts (S = 2 or 3).  The products kept are
+// those with part-index sum < S: S = 2 -> v0w0 + v0w1 + v1w0 (3 MFMAs, ~16-bit products),
+// S = 3 -> adds v0w2 + v1w1 + v2w0 (6 MFMAs, ~24-bit, FP32-class).  Accumulation is FP32.
+//
+// Layouts (per problem p, written by k_asplit / k_esplit, zero padded):
+//   Ahat[s] : [Mp][Kp]  rows x modes, modes contiguous    (Mp = M rounded up to 128)
+//   Bhat[s] : [Np][Kp]  output columns x modes            (Np = 2 n_r rounded up to 128)
+// so both MFMA operands read 8 consecutive modes as one 16-byte vector.  Kp = K rounded up
+// to 32.  A block computes a 128 x 128 tile of C with 4 waves of 64 x 64 (4 x 4 MFMA
+// 16x16x32 tiles); tiles of the next k-step are fetched into registers while the current
+// k-step runs from LDS.
+struct SplitProb {
+    size_t a_off, b_off, c_off;   // element offsets of the problem's Ahat, Bhat blocks and C block
+    int K, Kp;
+};
+
+constexpr int TBM = 128, TBN = 128, TKS = 32, TLD = TKS + 8;   // LDS row: 40 bf16 = 80 B
+
+__device__ __forceinline__ __bf16 bf_round(double v) { return (__bf16)(float)v; }
+
+// A split: one thread per (row, k) of the padded block of problem p (grid.y = p)
+template <int S>
+__global__ void k_asplit(const double *__restrict__ psi, size_t ldpsi, const SplitProb *__restrict__ sp,
+                         const int *__restrict__ col_off, int M, int Mp, size_t split_stride,
+                         __bf16 *__restrict__ Ahat) {
+    const int p = blockIdx.y;
+    const SplitProb P = sp[p];
+    const size_t n = (size_t)Mp * P.Kp;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int row = (int)(e / P.Kp), k = (int)(e % P.Kp);
+        double v = (row < M && k < P.K) ? psi[(size_t)row * ldpsi + col_off[p] + k] : 0.0;
+        for (int s = 0; s < S; ++s) {
+            const __bf16 h = bf_round(v);
+            Ahat[s * split_stride + P.a_off + e] = h;
+            v -= (double)(float)h;
+        }
+    }
+}
+
+// E split: block = 32 modes x 64 ranges of one problem; phasors formed in FP64, split to
+// bf16 parts and written transposed ([column][mode]) through LDS.
+struct ETile {
+    int p, k0;
+};
+
+template <int S>
+__global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles, const SplitProb *__restrict__ sp,
+                                                const int *__restrict__ mode_off, const double *__restrict__ k2,
+                                                const double *__restrict__ absn, const double *__restrict__ psis,
+                                                double dr, int nr, int lossless, double ev_re, double ev_im,
+                                                size_t split_stride, __bf16 *__restrict__ Bhat) {
+    __shared__ __attribute__((aligned(16))) __bf16 T[S][128][TKS + 8];
+    const ETile t = tiles[blockIdx.y];
+    const SplitProb P = sp[t.p];
+    const int i0 = blockIdx.x * 64;
+    const int mloc = threadIdx.x & 31;       // mode within the tile
+    const int k = t.k0 + mloc;               // mode within the problem
+    double kr = 0, ki = 0, cr = 0, ci = 0;
+    const bool live = k < P.K;
+    if (live) {
+        const size_t o = (size_t)mode_off[t.p] + k;
+        f2_csqrt(k2[o], absn[o], kr, ki);
+        if (lossless) {
+            ki = 0.0;
+            cr = psis[o] / sqrt(kr);
+        } else {
+            double sr, si;
+            f2_csqrt(kr, ki, sr, si);
+            const double den = sr * sr + si * si;
+            cr = psis[o] * sr / den;
+            ci = -psis[o] * si / den;
+        }
+    }
+    for (int q = 0; q < 8; ++q) {
+        const int il = (threadIdx.x >> 5) + 8 * q;   // range within the tile, 0..63
+        const int i = i0 + il;
+        double vr = 0.0, vi = 0.0;
+        if (live && i < nr) {
+            const double r = (i + 1) * dr;
+            double sn, cs;
+            sincos(f2_reduce(kr * r), &sn, &cs);
+            const double amp = exp(-ki * r) * (1000.0 / sqrt(r));
+            const double er = cs * amp, ei = sn * amp;
+            const double tr = cr * er - ci * ei, ti = cr * ei + ci * er;
+            vr = tr * ev_re - ti * ev_im;
+            vi = tr * ev_im + ti * ev_re;
+        }
+        for (int s = 0; s < S; ++s) {
+            const __bf16 hr = bf_round(vr), hi = bf_round(vi);
+            T[s][2 * il][mloc] = hr;
+            T[s][2 * il + 1][mloc] = hi;
+            vr -= (double)(float)hr;
+            vi -= (double)(float)hi;
+        }
+    }
+    __syncthreads();
+    // write 128 columns x 32 modes per split as 16-byte vectors (4 per column)
+    for (int s = 0; s < S; ++s)
+        for (int v = threadIdx.x; v < 128 * 4; v += 256) {
+            const int col = v >> 2, part = v & 3;
+            const size_t n = 2 * (size_t)i0 + col;
+            const uint4 val = *(const uint4 *)&T[s][col][part * 8];
+            *(uint4 *)&Bhat[s * split_stride + P.b_off + n * P.Kp + t.k0 + part * 8] = val;
+        }
+}
+
+template <int S>
+__global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restrict__ sp, const __bf16 *__restrict__ Ahat,
+                                                       const __bf16 *__restrict__ Bhat, size_t split_stride_a,
+                                                       size_t split_stride_b, float *__restrict__ C, size_t ldc,
+                                                       int M, int N, int n_tiles_n) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[S][TBM][TLD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[S][TBN][TLD];
+    const SplitProb P = sp[blockIdx.z];
+    // Blocks are dealt to the 8 XCDs round-robin (id % 8).  All M tiles of one N tile are
+    // kept on one XCD so its Bhat tile is fetched into one L2 only.
+    const int n_tiles_n8 = (n_tiles_n + 7) & ~7;
+    const int tiles_m = gridDim.x / n_tiles_n8;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int tm = slot % tiles_m;
+    const int tn = (slot / tiles_m) * 8 + xcd;
+    if (tn >= n_tiles_n) return;
+    const int m0 = tm * TBM, n0 = tn * TBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
+    const __bf16 *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
+    const int Kp = P.Kp;
+    // each thread moves 2 x 16 B of A and 2 x 16 B of B per split and k-step:
+    // tile rows 128 x 4 vectors = 512 vectors per operand and split
+    uint4 ra[S][2], rb[S][2];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int v = tid + 256 * h;
+                const int row = v >> 2, part = v & 3;
+                ra[s][h] = *(const uint4 *)&Ab[s * split_stride_a + (size_t)(m0 + row) * Kp + k0 + part * 8];
+                rb[s][h] = *(const uint4 *)&Bb[s * split_stride_b + (size_t)(n0 + row) * Kp + k0 + part * 8];
+            }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int v = tid + 256 * h;
+                const int row = v >> 2, part = v & 3;
+                *(uint4 *)&As[s][row][part * 8] = ra[s][h];
+                *(uint4 *)&Bs[s][row][part * 8] = rb[s][h];
+            }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < Kp; k0 += TKS) {
+        __syncthreads();
+        stash();
+        __syncthreads();
+        if (k0 + TKS < Kp) fetch(k0 + TKS);
+        const int kq = 8 * (lane >> 4);
+        bf8 af[S][4], bfr[S][4];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                af[s][a] = *(const bf8 *)&As[s][wm + a * 16 + (lane & 15)][kq];
+                bfr[s][a] = *(const bf8 *)&Bs[s][wn + a * 16 + (lane & 15)][kq];
+            }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                // smallest products first
+                if constexpr (S == 3) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2][a], bfr[0][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][a], bfr[1][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[S - 1][b], acc[a][b], 0, 0, 0);
+                }
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][a], bfr[0][b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[1][b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[0][b], acc[a][b], 0, 0, 0);
+            }
+    }
+    float *Cp = C + P.c_off;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = m0 + wm + a * 16 + 4 * (lane >> 4) + v;
+                const int c = n0 + wn + b * 16 + (lane & 15);
+                if (r < M && c < N) Cp[(size_t)r * ldc + c] = acc[a][b][v];
+            }
+}
+
 }  // namespace ncpa
 
 using namespace ncpa;
@@ -263,12 +467,76 @@ extern "C" int ncpa_modes_device(ncpa_modes_t *m, const int **off, const double 
 int ncpa_modes_internal(ncpa_modes_t *m, int *P, int *total, const int **h_offsets, const int **h_counts,
                         const double **d_psis);
 
+namespace {
+
+template <int S>
+int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *hcnt, const int *d_off,
+                  const double *d_k2, const double *d_abs, const double *d_psis, const double *psi, int M,
+                  double dr, int n_r, int lossless, float *out) {
+    const int N = 2 * n_r;
+    const int Mp = (M + TBM - 1) / TBM * TBM, Np = (N + TBN - 1) / TBN * TBN;
+    std::vector<SplitProb> sp(P);
+    std::vector<ETile> tiles;
+    size_t sa = 0, sb = 0;
+    for (int p = 0; p < P; ++p) {
+        const int K = hcnt[p], Kp = (K + TKS - 1) / TKS * TKS;
+        sp[p] = {sa, sb, (size_t)p * M * N, K, Kp};
+        sa += (size_t)Mp * Kp;
+        sb += (size_t)Np * Kp;
+        for (int k0 = 0; k0 < Kp; k0 += TKS) tiles.push_back({p, k0});
+    }
+    SplitProb *d_sp = pool_alloc<SplitProb>(ctx, P);
+    ETile *d_tiles = pool_alloc<ETile>(ctx, std::max<size_t>(tiles.size(), 1));
+    __bf16 *Ahat = pool_alloc<__bf16>(ctx, std::max<size_t>(S * sa, 8));
+    __bf16 *Bhat = pool_alloc<__bf16>(ctx, std::max<size_t>(S * sb, 8));
+    auto cleanup = [&]() {
+        ctx->pool.put(d_sp);
+        ctx->pool.put(d_tiles);
+        ctx->pool.put(Ahat);
+        ctx->pool.put(Bhat);
+    };
+    if (!d_sp || !d_tiles || !Ahat || !Bhat) {
+        cleanup();
+        return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_field2d: allocation failed");
+    }
+    hipError_t e = hipMemcpyAsync(d_sp, sp.data(), sizeof(SplitProb) * P, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && !tiles.empty())
+        e = hipMemcpyAsync(d_tiles, tiles.data(), sizeof(ETile) * tiles.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return hip_error(ctx, e, "ncpa_field2d copy");
+    }
+    const std::complex<double> I(0.0, 1.0);
+    const std::complex<double> ev = 4 * PI * I * std::exp(-I * PI * 0.25) / std::sqrt(8.0 * PI);
+    if (!tiles.empty()) {
+        TimedLaunch tl(ctx, "emat");
+        hipLaunchKernelGGL(k_asplit<S>, dim3(64, P), dim3(256), 0, ctx->stream, psi, (size_t)total, d_sp, d_off, M, Mp,
+                           sa, Ahat);
+        hipLaunchKernelGGL(k_esplit<S>, dim3((n_r + 63) / 64, (unsigned)tiles.size()), dim3(256), 0, ctx->stream,
+                           d_tiles, d_sp, d_off, d_k2, d_abs, d_psis, dr, n_r, lossless, ev.real(), ev.imag(), sb,
+                           Bhat);
+    }
+    {
+        TimedLaunch tl(ctx, "field2d");
+        const int tn = Np / TBN, tm = Mp / TBM;
+        dim3 g((unsigned)(tm * ((tn + 7) & ~7)), 1, (unsigned)P);
+        hipLaunchKernelGGL(k_gemm_split<S>, g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb, out, (size_t)N, M,
+                           N, tn);
+    }
+    e = hipGetLastError();
+    cleanup();
+    if (e != hipSuccess) return hip_error(ctx, e, "ncpa_field2d split launch");
+    return NCPA_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, int nrows, double dr, int n_r,
                  int lossless, int precision, void *dev_out) {
     if (!ctx || !m || !dev_psi_rows || nrows <= 0 || n_r <= 0 || !(dr > 0.0) || !dev_out ||
-        precision < 0 || precision > 2)
+        precision < 0 || precision > 3)
         return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d: bad arguments");
     int P = 0, total = 0;
     const int *hoff = nullptr, *hcnt = nullptr;
@@ -277,6 +545,15 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, i
     ncpa_modes_device(m, nullptr, &d_k2, &d_abs, nullptr);
     if (total > 0 && !d_psis)
         return set_error(ctx, NCPA_ERR_STATE, "ncpa_field2d: call ncpa_modes_rows first (it supplies psi(zs))");
+    if (precision >= PREC_BF16X3) {
+        const int *d_off = nullptr;
+        ncpa_modes_device(m, &d_off, nullptr, nullptr, nullptr);
+        if (precision == PREC_BF16X3)
+            return field2d_split<2>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, nrows, dr,
+                                    n_r, lossless, (float *)dev_out);
+        return field2d_split<3>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, nrows, dr, n_r,
+                                lossless, (float *)dev_out);
+    }
     const size_t ldb = 2 * (size_t)n_r, ldc = ldb, lda = (size_t)std::max(total, 1);
     std::vector<GemmProb> gp(P);
     for (int p = 0; p < P; ++p) gp[p] = {(size_t)hoff[p], (size_t)hoff[p], (size_t)p * nrows * ldc, hcnt[p]};

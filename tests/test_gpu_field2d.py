@@ -22,7 +22,7 @@ def field_case(gpu_ctx, synth_profile):
     return m, stride, nrows, nr, dr, ref.T   # ref [rows, ranges]
 
 
-@pytest.mark.parametrize("prec,tol_db", [(0, 1e-2), (1, 0.1), (2, 0.1)])
+@pytest.mark.parametrize("prec,tol_db", [(0, 1e-2), (1, 0.1), (2, 0.1), (3, 0.1)])
 def test_field2d(field_case, prec, tol_db):
     m, stride, nrows, nr, dr, ref = field_case
     print("case field2d precision=%d" % prec, flush=True)
