@@ -162,7 +162,7 @@ int ncpa_modes_download(ncpa_modes_t *modes, double *k_re, double *k_im, double 
  * to working precision before Newton converged. */
 int ncpa_modes_iterations(ncpa_modes_t *modes, int *iters);
 /* Device views (valid until ncpa_modes_free): offsets[n_problems+1] (int), and per mode
- * k2, absorption, psi_s^2, psi_r^2 (double). */
+ * k^2, absorption A (k_pert = sqrt(k^2 + i A)) and psi(zs) psi(zr) (double). */
 int ncpa_modes_device(ncpa_modes_t *modes, const int **dev_offsets, const double **dev_k2,
                       const double **dev_absorption, const double **dev_psi_sr);
 
@@ -201,6 +201,55 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, const double *dev_psi_row
 int ncpa_ess_eigen(ncpa_ctx_t *ctx, const double *diag, int Nz, double dz, double tol,
                    const double *k_min, const double *k_max, int max_modes, int *nconv,
                    double *k2, double *v, int ldv);
+
+/* ---- broadband pulse synthesis (ModBB --propagation) ------------------------------ */
+
+/* Batched complex FFT of `batch` contiguous transforms of n points (interleaved re/im
+ * doubles, device).  sign -1 = FFTW_FORWARD, +1 = FFTW_BACKWARD, both unnormalised
+ * (the plans of BroadbandPropagator.cpp:342-345, 365-368, 453-455, 639-641). */
+int ncpa_fft_batch(ncpa_ctx_t *ctx, const double *dev_in, double *dev_out, int n, int batch, int sign);
+
+/* k_pert = sqrt(k^2 + i A) (ModeSolver.cpp:277-282) of n device-resident modes, e.g. the
+ * k2 / absorption views of ncpa_modes_device, into device arrays dev_kre, dev_kim. */
+int ncpa_kpert(ncpa_ctx_t *ctx, const double *dev_k2, const double *dev_absorption, int n,
+               double *dev_kre, double *dev_kim);
+
+/* Transfer functions of n_sets dispersion sets (e.g. azimuths) at n_ranges ranges:
+ * ModalBroadbandPropagator::compute_modal_sum (ModalBroadbandPropagator.cpp:245-277), i.e.
+ * H = i e^{-i pi/4}/sqrt(8 pi r) sqrt(rho_ratio) sum_j psi_S psi_R e^{i k r}/sqrt(k) with the
+ * half-Hann taper over the top floor(0.1 n_freq) bins.  Modes of (set s, bin f) are the
+ * device entries [mode_begin[s*n_freq+f], + mode_count[...]) of dev_kre, dev_kim,
+ * dev_psi_sr (host index arrays; count 0 for the DC bin).  rho_ratio[s] = rho_r/rho_s.
+ * zero_attn != 0 takes Im k = 0.  dev_H: complex (interleaved) [n_sets][n_ranges][n_freq]. */
+int ncpa_bb_transfer(ncpa_ctx_t *ctx, int n_sets, int n_freq, const int *mode_begin,
+                     const int *mode_count, const double *dev_kre, const double *dev_kim,
+                     const double *dev_psi_sr, int zero_attn, const double *ranges_m,
+                     int n_ranges, const double *rho_ratio, double *dev_H);
+
+/* Propagated waveforms from transfer functions: BroadbandPropagator::fft_pulse_prop
+ * (BroadbandPropagator.cpp:573-651) and the factor 2 of ModalBroadbandPropagator.cpp:223-226.
+ * spectrum: host complex S[n_freq] (interleaved); f_vec: host frequencies with f_vec[0] the
+ * DC bin; t0 = range / max_celerity; df = f_vec[1] - f_vec[0].  dev_wave:
+ * [n_sets][n_ranges][nfft] doubles = 2 Re(df FFTW_FORWARD(e^{-2 pi i f t0} S H)). */
+int ncpa_bb_pulse(ncpa_ctx_t *ctx, int n_sets, int n_ranges, int n_freq, const double *dev_H,
+                  const double *spectrum, const double *f_vec, const double *ranges_m,
+                  double max_celerity, int nfft, double *dev_wave);
+
+/* Source types of BroadbandPropagator::get_source_spectrum (BroadbandPropagator.cpp:244-417) */
+#define NCPA_SRC_IMPULSE  0
+#define NCPA_SRC_PULSE1   1
+#define NCPA_SRC_PULSE2   2
+#define NCPA_SRC_SPECTRUM 3
+#define NCPA_SRC_WAVEFORM 4
+
+/* Source spectrum S[n_freq] at f_vec (f_vec[0] = 0, f_step = f_vec[2] - f_vec[1]) and the
+ * time-domain source pulse[nfft] (complex, before the factor the reference applies when it
+ * writes source_waveform_input.dat); both host arrays, pulse may be NULL.
+ * f_center < 0 selects f_max/5 (pulse1, pulse2).  input: for SPECTRUM, n_input rows of
+ * (f, Re S, Im S); for WAVEFORM, n_input rows of (t, amplitude); else NULL. */
+int ncpa_bb_source(ncpa_ctx_t *ctx, int kind, const double *f_vec, int n_freq, int nfft,
+                   double f_center, const double *input, int n_input, double *spectrum,
+                   double *pulse);
 
 #ifdef __cplusplus
 }

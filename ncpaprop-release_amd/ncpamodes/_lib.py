@@ -25,6 +25,8 @@ NCPA_PREC_BF16X6 = 3
 NCPA_F_WKB = 1
 NCPA_F_WVNUM_FILTER = 2
 
+NCPA_SRC = {"impulse": 0, "pulse1": 1, "pulse2": 2, "spectrum": 3, "waveform": 4}
+
 # Every symbol include/ncpamodes.h declares (checked by tests/test_abi.py).
 EXPORTS = [
     "ncpa_ctx_create", "ncpa_ctx_destroy", "ncpa_ctx_error", "ncpa_ctx_synchronize",
@@ -35,6 +37,7 @@ EXPORTS = [
     "ncpa_grid_make", "ncpa_profile_sample",
     "ncpa_ess_solve", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
     "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d", "ncpa_ess_eigen",
+    "ncpa_fft_batch", "ncpa_kpert", "ncpa_bb_transfer", "ncpa_bb_pulse", "ncpa_bb_source",
 ]
 
 
@@ -108,6 +111,14 @@ def load():
         "ncpa_field2d": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, vp]),
         "ncpa_ess_eigen": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int,
                                      ip, dp, dp, C.c_int]),
+        "ncpa_fft_batch": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int]),
+        "ncpa_kpert": (C.c_int, [vp, vp, vp, C.c_int, vp, vp]),
+        "ncpa_bb_transfer": (C.c_int, [vp, C.c_int, C.c_int, ip, ip, vp, vp, vp, C.c_int, dp, C.c_int,
+                                       dp, vp]),
+        "ncpa_bb_pulse": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, dp, dp, dp, C.c_double, C.c_int,
+                                    vp]),
+        "ncpa_bb_source": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, C.c_double, dp, C.c_int, dp,
+                                     dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -11,9 +11,16 @@ Copies (data files only, unchanged or line subsets):
   * samples/disp_256freqs_0p5hz_NCPA_canonic.dat, lines for 6 frequencies -> disp256_subset.dat
   * docs/.../modbb_gp_files/code_outputs/myDispersionFile.dat, 4 lines -> disp250_subset.dat
   * docs/.../modbb_gp_files/code_outputs/source_spectrum.dat            -> source_spectrum.dat
+  * docs/.../modbb_gp_files/code_outputs/myDispersionFile.dat (all 250 lines, parsed, no
+        arithmetic) -> disp250.npz: f[250], counts[250], q[total, 4] = Re k, Im k, psi_S, psi_R,
+        rho_s, rho_r
+  * docs/.../modbb_gp_files/code_outputs/mywavf.dat rows with 880 <= t <= 960 s
+        (ModBB pulse1 at 270 km, max celerity 320 m/s, NFFT 16384)       -> mywavf_880_960.dat
 """
 import os
 import shutil
+
+import numpy as np
 
 REF = "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -41,6 +48,22 @@ def main():
            [0, 4, 62, 249])
     shutil.copyfile(os.path.join(MODBB, "source_spectrum.dat"),
                     os.path.join(HERE, "source_spectrum.dat"))
+    f, counts, rows, rho = [], [], [], None
+    with open(os.path.join(MODBB, "myDispersionFile.dat")) as fh:
+        for line in fh:
+            v = line.split()
+            n = int(v[1])
+            f.append(float(v[0]))
+            counts.append(n)
+            rho = (float(v[2]), float(v[3]))
+            rows.append(np.array(v[4:4 + 4 * n], dtype=float).reshape(n, 4))
+    np.savez_compressed(os.path.join(HERE, "disp250.npz"), f=np.array(f), counts=np.array(counts),
+                        q=np.concatenate(rows), rho_s=rho[0], rho_r=rho[1])
+    with open(os.path.join(MODBB, "mywavf.dat")) as fh, \
+            open(os.path.join(HERE, "mywavf_880_960.dat"), "w") as out:
+        for line in fh:
+            if 880.0 <= float(line.split()[0]) <= 960.0:
+                out.write(line)
 
 
 if __name__ == "__main__":
