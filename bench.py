@@ -1,19 +1,31 @@
 #!/usr/bin/env python3
-"""Benchmark: modal TL solves/s of the MI355X effective-sound-speed normal-mode path.
+"""Benchmark of the MI355X effective-sound-speed normal-mode path (ncpaprop Modess / ModBB).
 
-One step = one batch of (profile, frequency, azimuth) solves through the HIP library:
-grid setup -> k_min/k_max -> all eigenvalues in [k_min^2, k_max^2] with perturbative
-attenuation and psi(zs) psi(zr) -> 1-D transmission loss at 10000 ranges.  Profiles are
-resident on the device before the timed region; outputs stay on the device.
+Default workload (--workload field, the configuration the headline metric is quoted on):
+one step = a batch of 64 (profile, frequency, azimuth) solves at 1 Hz (8 seeded synthetic
+profiles x 8 azimuths), each through the HIP library: grid -> k_min/k_max -> every mode in
+[k_min^2, k_max^2] with perturbative attenuation -> 1-D TL at 10000 ranges (0.1-1000 km) ->
+eigenvector rows at 1500 depths -> 2-D field 1500 depths x 10000 ranges.  Eigenvalues,
+phases (reduced mod 2 pi) and the 1-D TL are FP64; the field contraction runs in the fastest
+precision whose TL stays within 0.1 dB of the FP64 contraction wherever TL > -140 dB, which
+the run checks itself on two of the solves before timing (the FP64-plain rate is reported
+beside it).  Profiles are resident on the device before the timed region; outputs stay there.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+--workload broadband: one step = a batched ModBB job: dispersion solves at 1024 frequencies
+(f_min = f_step = 2/1024 Hz) x 8 azimuths on the seed-0 profile, then transfer functions at
+16 receiver ranges (200-1000 km) and pulse synthesis (NFFT = max(4096, 4 Nfreq) = 4100, the
+reference's rule) with a Gaussian-windowed 0.5 Hz source given as a spectrum.
 
-Prints one JSON line (see DESIGN.md for the fields).  The CPU baseline is the FP64 oracle
-(scipy LAPACK bisection + inverse iteration, numpy TL sums) timed on a bounded sample of the
-same workload on this host, single-threaded.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload field|broadband] [--no-cpu]
+
+Prints one JSON line.  cpu_baseline: the FP64 oracle (scipy LAPACK bisection + inverse
+iteration, numpy sums; one thread per process) on a bounded sample of the same workload, run
+in parallel worker processes on this host's cores and scaled to the workload's unit.
 """
 import argparse
+import ctypes
 import json
+import math
 import os
 import subprocess
 import sys
@@ -24,81 +36,180 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "ncpaprop-release_amd")
 sys.path.insert(0, PKG)
 
+METRIC = "modal TL solves/sec at ≤0.1 dB vs FP64 plain on GPU; p50/p99 request latency"
 FREQ = 1.0
 AZIMUTHS = [0.0, 45.0, 90.0, 135.0, 180.0, 225.0, 270.0, 315.0]
 SEEDS = list(range(8))
 NZ = 20000
 NR = 10000
 MAXRANGE_M = 1000e3
-FLOPS_PER_STURM_STEP = 3.0     # one row of one sweep: sub, mul/div, sub
-HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+FIELD_ROWS, FIELD_STRIDE = 1500, 13        # 1500 depths, 0 - 146 km AGL
+FLOPS_PER_STURM_STEP = 3.0                 # one row of one sweep
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
+PREC_NAMES = {0: "f64", 1: "f32", 2: "bf16x3", 3: "bf16x6"}
+PREC_PEAK = {0: "f64", 1: "f32", 2: "bf16", 3: "bf16"}
+# candidate field precisions, fastest first (measured on MI355X: split-BF16 3 products <
+# split-BF16 6 products ~ FP32 MFMA < FP64 MFMA)
+CANDIDATES = [2, 3, 1]
+TL_GATE_DB = 0.1
+
+BB_NF = 1024
+BB_FSTEP = 2.0 / 1024
+BB_RANGES_KM = [200.0 + i * (800.0 / 15.0) for i in range(16)]
+BB_MAXCEL = 340.0
 
 
-def write_profiles(d):
+def write_profiles(d, seeds=SEEDS):
     from ncpamodes import synth
-    return [synth.write_profile(os.path.join(d, "P%d.dat" % s), seed=s) for s in SEEDS]
+    return [synth.write_profile(os.path.join(d, "P%d.dat" % s), seed=s) for s in seeds]
 
 
-def cpu_baseline(n_problems=2):
-    """FP64 oracle on a bounded sample of the workload (runs in a 1-thread subprocess)."""
+def bb_source_spectrum(f):
+    """Gaussian-windowed 0.5 Hz source pulse s(t) = sin(2 pi 0.5 (t - 10)) exp(-((t - 10)/2)^2),
+    its spectrum S(f) = int s(t) e^{+2 pi i f t} dt in closed form (the --source spectrum
+    convention of BroadbandPropagator.cpp:313-325)."""
+    import numpy as np
+    f0, tc, sg = 0.5, 10.0, 2.0
+    # sin(2 pi f0 (t - tc)) = (e^{i a} - e^{-i a}) / 2i; Gaussian FT: sg sqrt(pi) e^{-(pi sg nu)^2}
+    g = lambda nu: sg * math.sqrt(math.pi) * np.exp(-(math.pi * sg * nu) ** 2)
+    S = (g(f + f0) - g(f - f0)) / 2j * np.exp(2j * math.pi * f * tc)
+    return S
+
+
+# ------------------------------------------------------------------------------------------
+# CPU baseline (oracle) -- worker processes, one thread each
+# ------------------------------------------------------------------------------------------
+def cpu_worker(spec):
     sys.path.insert(0, ROOT)
+    import numpy as np
     from oracle import ess
-    with tempfile.TemporaryDirectory() as d:
-        paths = write_profiles(d)
+    out = {}
+    if spec["workload"] == "field":
         t = time.perf_counter()
-        for i in range(n_problems):
-            sol = ess.solve(paths[i % len(paths)], freq=FREQ, azimuth=AZIMUTHS[i % len(AZIMUTHS)],
-                            Nz=NZ, turnoff_wkb=True)
-            ess.tl1d(sol.k_pert, sol.psi[sol.i_s], sol.psi[sol.i_r], MAXRANGE_M / NR, NR)
-        dt = time.perf_counter() - t
-    return {"value": n_problems / dt, "seconds": dt, "problems": n_problems}
+        sol = ess.solve(spec["path"], freq=FREQ, azimuth=spec["az"], Nz=NZ, turnoff_wkb=True)
+        t_eig = time.perf_counter() - t
+        t = time.perf_counter()
+        ess.tl1d(sol.k_pert, sol.psi[sol.i_s], sol.psi[sol.i_r], MAXRANGE_M / NR, NR)
+        t_tl = time.perf_counter() - t
+        rows = np.arange(FIELD_ROWS) * FIELD_STRIDE
+        nr_s = spec["field_ranges"]
+        t = time.perf_counter()
+        ess.tl2d(sol.k_pert, sol.psi[rows], sol.psi[sol.i_s], MAXRANGE_M / NR, nr_s)
+        t_f = time.perf_counter() - t
+        out["solve_s"] = t_eig + t_tl + t_f * (NR / nr_s)
+        out["parts"] = [t_eig, t_tl, t_f]
+    else:
+        from oracle import broadband as obb
+        t = time.perf_counter()
+        sol = ess.solve(spec["path"], freq=spec["freq"], azimuth=spec["az"], Nz=NZ, turnoff_wkb=True)
+        kp, ps, pr = sol.k_pert, sol.psi[sol.i_s], sol.psi[sol.i_r]
+        t_eig = time.perf_counter() - t
+        # compute_modal_sum terms of this frequency at the 16 ranges
+        t = time.perf_counter()
+        for r in BB_RANGES_KM:
+            np.sum(ps * pr * np.exp(1j * r * 1e3 * kp) / np.sqrt(kp))
+        t_h = time.perf_counter() - t
+        out["solve_s"] = t_eig + t_h
+        out["parts"] = [t_eig, t_h]
+        if spec.get("fft"):
+            nf = BB_NF + 1
+            f = np.arange(nf) * BB_FSTEP
+            nfft = 4 * nf
+            t = time.perf_counter()
+            H = np.ones(nf, dtype=complex)
+            for _ in range(len(BB_RANGES_KM) * len(AZIMUTHS)):
+                obb.pulse_prop(H, H, f, 1.0, nfft)
+            out["fft_s"] = time.perf_counter() - t
+    print(json.dumps(out), flush=True)
 
 
-def run_cpu_leg():
+def cpu_baseline(workload, n_workers):
+    """Runs n_workers oracle processes at once (one thread each); returns the aggregate rate."""
     env = dict(os.environ)
     for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
         env[k] = "1"
-    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg"], env=env,
-                         capture_output=True, text=True, timeout=600)
-    if out.returncode != 0:
-        sys.stderr.write(out.stderr)
-        return None
-    return json.loads(out.stdout.strip().splitlines()[-1])
+    tmp = tempfile.mkdtemp(prefix="ncpa_cpu_")
+    specs = []
+    if workload == "field":
+        paths = write_profiles(tmp)
+        for i in range(n_workers):
+            specs.append({"workload": "field", "path": paths[i % len(paths)],
+                          "az": AZIMUTHS[(i * 3) % len(AZIMUTHS)], "field_ranges": 100})
+    else:
+        path = write_profiles(tmp, [0])[0]
+        # frequencies spread uniformly over the band so the mean cost matches the band's
+        idx = [int((i + 0.5) * BB_NF / n_workers) + 1 for i in range(n_workers)]
+        for i, fi in enumerate(idx):
+            specs.append({"workload": "broadband", "path": path, "freq": fi * BB_FSTEP,
+                          "az": AZIMUTHS[i % len(AZIMUTHS)], "fft": i == 0})
+    t = time.perf_counter()
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", json.dumps(s)],
+                              env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+             for s in specs]
+    res = []
+    for p in procs:
+        out, err = p.communicate(timeout=900)
+        if p.returncode != 0:
+            sys.stderr.write(err)
+            return None
+        res.append(json.loads(out.strip().splitlines()[-1]))
+    wall = time.perf_counter() - t
+    mean_solve = sum(r["solve_s"] for r in res) / len(res)
+    out = {"workers": n_workers, "wall_s": wall, "mean_solve_s": mean_solve,
+           "rate": n_workers / mean_solve, "parts": [sum(x) / len(res) for x in zip(*[r["parts"] for r in res])]}
+    if workload == "broadband":
+        out["fft_s"] = next(r["fft_s"] for r in res if "fft_s" in r)
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
-    args = ap.parse_args()
-    if args.cpu_leg:
-        print(json.dumps(cpu_baseline()))
-        return 0
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (--gpus %d requested); the path shards "
-              "by independent (profile, frequency, azimuth) solves, see DESIGN.md" % args.gpus,
-              file=sys.stderr)
-        return 2
+def cpu_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+# ------------------------------------------------------------------------------------------
+# GPU legs
+# ------------------------------------------------------------------------------------------
+def field_error_db(ctx, grid, problems, prec):
+    """Max |TL_prec - TL_fp64| [dB] over TL > -140 dB for a few problems (untimed check)."""
     import numpy as np
-    import ncpamodes
-
-    tmp = tempfile.mkdtemp(prefix="ncpa_bench_")
-    paths = write_profiles(tmp)
-    ctx = ncpamodes.Context(0)
-    pids = [ctx.read_profile(p)[0] for p in paths]
-    grid = ctx.grid(pids[0], Nz=NZ)
-    problems = [ncpamodes.problem(pid, FREQ, az) for pid in pids for az in AZIMUTHS]
-    P = len(problems)
-    bufs = [ctx.empty(P * NR) for _ in range(3)]
+    m = ctx.solve(grid, problems)
+    rows = m.rows_device(0, FIELD_STRIDE, FIELD_ROWS)
     dr = MAXRANGE_M / NR
+    o64 = m.field2d_device(rows, FIELD_ROWS, dr, NR, 0)
+    oc = m.field2d_device(rows, FIELD_ROWS, dr, NR, prec)
+    a = o64.numpy().reshape(-1, 2)
+    b = oc.numpy().astype(np.float64).reshape(-1, 2)
+    for x in (o64, oc, rows):
+        x.free()
+    m.free()
+    tl0 = 10.0 * np.log10(np.maximum(a[:, 0] ** 2 + a[:, 1] ** 2, 1e-300))
+    tl1 = 10.0 * np.log10(np.maximum(b[:, 0] ** 2 + b[:, 1] ** 2, 1e-300))
+    mask = tl0 > -140.0
+    d = np.abs(tl1 - tl0)[mask]
+    return float(d.max()), float(np.sqrt(np.mean(d ** 2)))
+
+
+def run_field(ctx, args, prec, problems, grid):
+    """Timed loop of the config-1 step at one field precision; returns timing dict."""
+    import numpy as np
+    P = len(problems)
+    dr = MAXRANGE_M / NR
+    tl = [ctx.empty(P * NR) for _ in range(3)]
+    state = {}
 
     def step():
         m = ctx.solve(grid, problems)
-        m.tl1d_device(dr, NR, False, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr)
+        m.tl1d_device(dr, NR, False, tl[0].ptr, tl[1].ptr, tl[2].ptr)
+        if "rows" not in state:
+            state["rows"] = ctx.empty(FIELD_ROWS * max(m.total, 1))
+            state["out"] = ctx.empty(P * FIELD_ROWS * 2 * NR, np.float64 if prec == 0 else np.float32)
+        ctx.check(ctx.lib.ncpa_modes_rows(ctx.h, m.h, 0, FIELD_STRIDE, FIELD_ROWS,
+                                          ctypes.c_void_p(state["rows"].ptr)))
+        m.field2d_device(state["rows"], FIELD_ROWS, dr, NR, prec, out=state["out"])
         return m
 
     for _ in range(args.warmup):
@@ -115,73 +226,217 @@ def main():
         else:
             m.free()
     ctx.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    modes_ms, modes_n = ctx.kernel_time("modes")
-    tl_ms, _ = ctx.kernel_time("tl1d")
-    iters = np.abs(last.iterations()).astype(np.float64)
-    total_modes = last.total
-    # one final sensitivity sweep per mode in addition to the counted sweeps
-    steps_per_launch = float((iters + 1.0).sum()) * NZ
-    modes_avg_s = modes_ms / max(modes_n, 1) / 1e3
-    achieved_tflops = FLOPS_PER_STURM_STEP * steps_per_launch / modes_avg_s / 1e12
-    last.free()
+    elapsed = time.perf_counter() - t0
+    fams = {}
+    for name in ("setup", "interval", "modes", "tl1d", "rows", "emat", "field2d"):
+        ms, n = ctx.kernel_time(name)
+        fams[name] = (ms, n)
     ctx.set_timing(False)
+    iters = np.abs(last.iterations()).astype(np.float64)
+    res = {"elapsed": elapsed, "fams": fams, "total_modes": int(last.total),
+           "sturm_steps": float((iters + 1.0).sum()) * NZ,
+           "field_flops": 4.0 * FIELD_ROWS * last.total * NR}
+    last.free()
+    for b in tl + [state.get("rows"), state.get("out")]:
+        if b is not None:
+            b.free()
+    return res
 
-    cpu = None if args.no_cpu else run_cpu_leg()
-    value = P * args.steps / elapsed
+
+def field_main(args, cpu):
+    import numpy as np
+    import ncpamodes
+    tmp = tempfile.mkdtemp(prefix="ncpa_bench_")
+    paths = write_profiles(tmp)
+    ctx = ncpamodes.Context(0)
+    pids = [ctx.read_profile(p)[0] for p in paths]
+    grid = ctx.grid(pids[0], Nz=NZ)
+    problems = [ncpamodes.problem(pid, FREQ, az) for pid in pids for az in AZIMUTHS]
+    P = len(problems)
+    # precision gate on two solves (different profiles and azimuths), before any timing
+    check = [problems[2], problems[8 * 5 + 6]]
+    errs, chosen = {}, 0
+    for prec in CANDIDATES:
+        errs[prec] = field_error_db(ctx, grid, check, prec)
+        if errs[prec][0] <= TL_GATE_DB:
+            chosen = prec
+            break
+    res = run_field(ctx, args, chosen, problems, grid)
+    plain = run_field(ctx, args, 0, problems, grid) if chosen != 0 else res
+    ctx.close()
+
+    value = P * args.steps / res["elapsed"]
+    fams = res["fams"]
+    per_step = {k: v[0] / args.steps for k, v in fams.items()}
+    # dominant kernel family of the measured step
+    dom = max(("modes", "field2d"), key=lambda k: fams[k][0])
+    if dom == "field2d":
+        avg_s = fams["field2d"][0] / max(fams["field2d"][1], 1) / 1e3
+        achieved = res["field_flops"] / avg_s / 1e12
+        peak = PEAK_TFLOPS[PREC_PEAK[chosen]]
+        roof = {"kernel": "k_gemm_split<%d> (2-D field contraction, %s)" % (3 if chosen == 3 else 2, PREC_NAMES[chosen])
+                if chosen >= 2 else "k_gemm_x4 (2-D field contraction, %s)" % PREC_NAMES[chosen],
+                "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None,
+                "algorithmic": "4 x depths x modes x ranges real flops per problem (real x complex)"}
+    else:
+        avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
+        achieved = FLOPS_PER_STURM_STEP * res["sturm_steps"] / avg_s / 1e12
+        roof = {"kernel": "k_modes", "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f64"],
+                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"], "traffic": None,
+                "algorithmic": "3 FP64 flops per Sturm step (row x sweep); FP64 VALU, MI355X FP64 peak"}
+    modes_avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
     line = {
-        "metric": "modal TL solves/sec at <=0.1 dB vs FP64 plain on GPU; p50/p99 request latency",
+        "metric": METRIC,
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": res["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+%s" % PREC_NAMES[chosen] if chosen else "f64",
         "data": "synthetic",
         "config": {
-            "workload": "ESS normal modes FP64 at 1 Hz, Nz=20000 over 0-150 km, all modes in "
-                        "[k_min,k_max] + perturbative attenuation + 1-D TL at 10000 ranges "
-                        "(0.1-1000 km); batch of 64 solves = 8 seeded profiles x 8 azimuths",
+            "workload": "ESS normal modes at 1 Hz, Nz=20000 over 0-150 km: all modes in [k_min,k_max] "
+                        "(FP64 Sturm bisection/Newton) + perturbative attenuation + 1-D TL at 10000 "
+                        "ranges (0.1-1000 km) + 2-D field 1500 depths x 10000 ranges; batch of 64 "
+                        "solves = 8 seeded profiles x 8 azimuths",
             "global_batch": P,
             "freq_hz": FREQ,
             "Nz": NZ,
             "n_ranges": NR,
-            "modes_per_step": int(total_modes),
+            "field_depths": FIELD_ROWS,
+            "modes_per_step": res["total_modes"],
+            "field_precision": PREC_NAMES[chosen],
             "parallelism": "single GPU",
         },
-        "stage_ms_per_step": {
-            "modes": modes_ms / args.steps,
-            "tl1d": tl_ms / args.steps,
-        },
-        "roofline": {
-            "kernel": "k_modes",
-            "bound": "mfma",
-            "note": "FP64 VALU recurrence; peak is MI355X FP64 (vector = matrix) 78.6 TFLOP/s",
-            "achieved": achieved_tflops,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": None,
-        },
+        "tl_check_db": {PREC_NAMES[p]: {"max": e[0], "rms": e[1]} for p, e in errs.items()},
+        "fp64_plain": {"value": P * args.steps / plain["elapsed"], "unit": "solves/s",
+                       "ms_per_step": plain["elapsed"] / args.steps * 1e3},
+        "stage_ms_per_step": per_step,
+        "roofline": roof,
+        "k_modes": {"avg_ms": modes_avg_s * 1e3,
+                    "tflops": FLOPS_PER_STURM_STEP * res["sturm_steps"] / modes_avg_s / 1e12},
         "cpu_baseline": None if cpu is None else {
-            "value": cpu["value"],
-            "unit": "solves/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": "%d of the benchmark's 1 Hz solves (eigen + perturbation + 1-D TL at "
-                      "10000 ranges), FP64 oracle: scipy LAPACK stebz/stein + numpy, 1 thread, "
-                      "%.1f s" % (cpu["problems"], cpu["seconds"]),
+            "value": cpu["rate"], "unit": "solves/s", "cores": cpu["workers"], "kind": "port",
+            "sample": "%d concurrent 1-thread processes, each one of the benchmark's 1 Hz solves "
+                      "(FP64 oracle: scipy LAPACK stebz/stein eigenpairs, numpy 1-D TL at 10000 ranges, "
+                      "2-D field on 1500 depths x 100 ranges scaled x100 to 10000 ranges); "
+                      "mean %.1f s per solve, %.1f s wall" % (cpu["workers"], cpu["mean_solve_s"],
+                                                               cpu["wall_s"]),
         },
     }
-    print(json.dumps(line), flush=True)
-    for b in bufs:
-        b.free()
+    return line
+
+
+def broadband_main(args, cpu):
+    import numpy as np
+    import ncpamodes
+    from ncpamodes import broadband as bb
+    tmp = tempfile.mkdtemp(prefix="ncpa_bench_")
+    path = write_profiles(tmp, [0])[0]
+    ctx = ncpamodes.Context(0)
+    pid = ctx.read_profile(path)[0]
+    grid = ctx.grid(pid, Nz=NZ)
+    f = np.arange(BB_NF + 1) * BB_FSTEP
+    problems = [ncpamodes.problem(pid, f[i], az) for az in AZIMUTHS for i in range(1, BB_NF + 1)]
+    ranges = np.array(BB_RANGES_KM) * 1e3
+    nfft = bb.default_nfft(4096, f.size)
+    S, _ = bb.source(ctx, "spectrum", f, nfft, -1.0, np.stack([f, bb_source_spectrum(f).real,
+                                                                bb_source_spectrum(f).imag], axis=1))
+    nset = len(AZIMUTHS)
+
+    def step():
+        m = ctx.solve(grid, problems)
+        W, kb = bb.synthesize_from_modes(ctx, m, nset, f, ranges, np.ones(nset), S, BB_MAXCEL, nfft)
+        return m, W, kb
+
+    for _ in range(args.warmup):
+        m, W, kb = step()
+        for b in (W,) + kb:
+            b.free()
+        m.free()
+    ctx.synchronize()
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    total_modes = 0
+    sturm = 0.0
+    for k in range(args.steps):
+        m, W, kb = step()
+        if k == args.steps - 1:
+            ctx.synchronize()
+            total_modes = m.total
+            sturm = float((np.abs(m.iterations()).astype(np.float64) + 1.0).sum()) * NZ
+        for b in (W,) + kb:
+            b.free()
+        m.free()
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    fams = {n: ctx.kernel_time(n) for n in ("setup", "interval", "modes", "bb_transfer", "fft")}
+    ctx.set_timing(False)
     ctx.close()
+    P = len(problems)
+    modes_avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
+    achieved = FLOPS_PER_STURM_STEP * sturm / modes_avg_s / 1e12
+    value = P * args.steps / elapsed
+    line = {
+        "metric": METRIC + " (broadband job)",
+        "value": value,
+        "unit": "dispersion solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "ModBB job: dispersion at 1024 frequencies (2/1024 - 2 Hz) x 8 azimuths, "
+                               "Nz=20000, + transfer functions at 16 ranges (200-1000 km) + pulse "
+                               "synthesis NFFT %d, Gaussian-windowed 0.5 Hz source spectrum" % nfft,
+                   "global_batch": P, "modes_per_step": int(total_modes), "nfft": nfft,
+                   "parallelism": "single GPU"},
+        "stage_ms_per_step": {k: v[0] / args.steps for k, v in fams.items()},
+        "roofline": {"kernel": "k_modes", "bound": "mfma", "achieved": achieved,
+                     "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"],
+                     "traffic": None,
+                     "algorithmic": "3 FP64 flops per Sturm step (row x sweep); FP64 VALU"},
+        "cpu_baseline": None,
+    }
+    if cpu is not None:
+        per_job_s = cpu["mean_solve_s"] * P / cpu["workers"] + cpu["fft_s"]
+        line["cpu_baseline"] = {
+            "value": P / per_job_s, "unit": "dispersion solves/s", "cores": cpu["workers"], "kind": "port",
+            "sample": "%d concurrent 1-thread processes, each one dispersion solve at a frequency spread "
+                      "over the band (FP64 oracle eigenpairs + compute_modal_sum at 16 ranges), plus the "
+                      "128 NFFT-%d pulse FFTs once; scaled to the %d-solve job" % (cpu["workers"], nfft, P)}
+        line["speedup_vs_cpu_all_cores"] = value / line["cpu_baseline"]["value"]
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["field", "broadband"], default="field")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-worker", help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(json.loads(args.cpu_worker))
+        return 0
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus %d requested); independent "
+              "(profile, frequency, azimuth) solves would shard across GPUs with no communication"
+              % args.gpus, file=sys.stderr)
+        return 2
+    cpu = None if args.no_cpu else cpu_baseline(args.workload, cpu_cores())
+    if args.workload == "field":
+        line = field_main(args, cpu)
+    else:
+        line = broadband_main(args, cpu)
+    print(json.dumps(line), flush=True)
     return 0
 
 
