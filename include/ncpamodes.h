@@ -166,6 +166,10 @@ int ncpa_modes_iterations(ncpa_modes_t *modes, int *iters);
 int ncpa_modes_device(ncpa_modes_t *modes, const int **dev_offsets, const double **dev_k2,
                       const double **dev_absorption, const double **dev_psi_sr);
 
+/* Effective sound speed c_eff(Hgt_i) [m/s] on the grid rows of one problem (host array of Nz),
+ * as getModalTrace leaves it (ESSModeSolver.cpp:544-556); used by the group-speed writer. */
+int ncpa_modes_ceff(ncpa_modes_t *modes, int problem, double *c_eff);
+
 /* Eigenvectors of every mode at grid rows row0 + k*stride (k < nrows), normalised so that
  * sum psi^2 dz = 1 with psi(zs) >= 0: dev_psi[k * total + mode] (device, nrows*total
  * doubles).  Used for 2-D fields and mode files (ModeSolver.cpp:513-540, 618-641). */

@@ -53,7 +53,40 @@ def build(verbose=False, jobs=8):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_cli(objdir, verbose)
     return LIB
+
+
+def build_cli(objdir, verbose=False):
+    """The Modess / ModBB command lines (host C++ over the C ABI), linked against the library."""
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    common = ["cli/params.cpp", "cli/ess_run.cpp"]
+    progs = {"Modess": common + ["cli/modess_main.cpp"],
+             "ModBB": common + ["cli/bb_run.cpp", "cli/modbb_main.cpp"]}
+    objs = {}
+    procs = []
+    for src in sorted({s for v in progs.values() for s in v}):
+        obj = os.path.join(objdir, "cli_" + os.path.basename(src) + ".o")
+        objs[src] = obj
+        cmd = ["g++", "-O2", "-std=c++17", "-c", os.path.join(HERE, src), "-o", obj, "-I", CSRC,
+               "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out.decode(errors="replace"))
+            raise RuntimeError("g++ failed: " + " ".join(cmd))
+    for name, srcs in progs.items():
+        exe = os.path.join(bindir, name)
+        cmd = ["g++", "-o", exe + ".tmp"] + [objs[s] for s in srcs] + \
+              ["-L", HERE, "-lncpamodes", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":

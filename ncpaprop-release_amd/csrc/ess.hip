@@ -996,6 +996,15 @@ int ncpa_modes_iterations(ncpa_modes_t *m, int *iters) {
     return NCPA_OK;
 }
 
+int ncpa_modes_ceff(ncpa_modes_t *m, int problem, double *c_eff) {
+    if (!m || !m->ctx || !c_eff || problem < 0 || problem >= m->P || !m->d_CE) return NCPA_ERR_ARG;
+    const size_t Nz = (size_t)m->grid.Nz;
+    NCPA_HIP(m->ctx, hipMemcpyAsync(c_eff, m->d_CE + (size_t)problem * Nz, Nz * sizeof(double),
+                                    hipMemcpyDeviceToHost, m->ctx->stream));
+    NCPA_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
+    return NCPA_OK;
+}
+
 int ncpa_modes_device(ncpa_modes_t *m, const int **off, const double **k2, const double **ab,
                       const double **psi_sr) {
     if (!m) return NCPA_ERR_ARG;

@@ -36,7 +36,7 @@ EXPORTS = [
     "ncpa_profile_read", "ncpa_profile_nodes", "ncpa_profile_admittance",
     "ncpa_grid_make", "ncpa_profile_sample",
     "ncpa_ess_solve", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
-    "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d", "ncpa_ess_eigen",
+    "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d", "ncpa_ess_eigen",
     "ncpa_fft_batch", "ncpa_kpert", "ncpa_bb_transfer", "ncpa_bb_pulse", "ncpa_bb_source",
 ]
 
@@ -107,6 +107,7 @@ def load():
         "ncpa_modes_iterations": (C.c_int, [vp, ip]),
         "ncpa_modes_rows": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, vp]),
         "ncpa_modes_device": (C.c_int, [vp, P(vp), P(vp), P(vp), P(vp)]),
+        "ncpa_modes_ceff": (C.c_int, [vp, C.c_int, dp]),
         "ncpa_tl1d": (C.c_int, [vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp]),
         "ncpa_field2d": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, vp]),
         "ncpa_ess_eigen": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int,
