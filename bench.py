@@ -414,12 +414,69 @@ def broadband_main(args, cpu):
     return line
 
 
+def serving_main(args):
+    """Config 4: Poisson request streams through the continuous-batching server."""
+    import numpy as np
+    from ncpamodes import serve
+    n_profiles = 64
+    streams = [(10.0, 150), (100.0, 1000), (500.0, 2000)]
+    srv = serve.Server(n_workers=3, precision=3, max_batch=64)
+    srv.warm(range(n_profiles))
+    warm = serve.make_requests(64, 1000.0, seed=99, n_profiles=n_profiles)
+    srv.serve(warm, realtime=False)
+    # capacity: the whole stream queued at t = 0
+    cap_reqs = serve.make_requests(2000, 500.0, seed=1, n_profiles=n_profiles)
+    for r in cap_reqs:
+        r.t_arrival = 0.0
+    wall = srv.serve(cap_reqs, realtime=False)
+    capacity = len(cap_reqs) / wall
+    lat = {}
+    for rate, n in streams:
+        reqs = serve.make_requests(n, rate, seed=int(rate), n_profiles=n_profiles)
+        w = srv.serve(reqs, realtime=True)
+        st = serve.latency_stats(reqs)
+        st["offered_per_s"] = rate
+        st["achieved_per_s"] = n / w
+        st["requests"] = n
+        lat["%g" % rate] = st
+    srv.close()
+    # plain per-request FP64 path: one request at a time, FP64 contraction, no batching
+    plain = serve.Server(n_workers=1, precision=0, max_batch=1)
+    plain.warm(range(n_profiles))
+    preqs = serve.make_requests(300, 500.0, seed=1, n_profiles=n_profiles)
+    for r in preqs:
+        r.t_arrival = 0.0
+    pw = plain.serve(preqs, realtime=False)
+    p10 = serve.make_requests(150, 10.0, seed=10, n_profiles=n_profiles)
+    plain.serve(p10, realtime=True)
+    plain_lat = serve.latency_stats(p10)
+    plain.close()
+    return {
+        "metric": METRIC,
+        "value": capacity,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": len(cap_reqs), "warmup": len(warm),
+        "ms_per_step": 1e3 / capacity,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+bf16x6",
+        "data": "synthetic",
+        "config": {"workload": "serving: Poisson request streams; per request a seeded profile (jet amplitudes "
+                               "+-40%, heights +-5 km, 64-profile pool resident on the device), f log-uniform "
+                               "0.05-2 Hz, azimuth U(0,360), max range U(200,1500) km, 80% 1-D TL (100 m steps) "
+                               "/ 20% 2-D field (500 depths x 1 km steps); 3 device contexts, batches <= 64",
+                   "parallelism": "single GPU", "Nz": NZ},
+        "latency": lat,
+        "fp64_plain_per_request": {"capacity_per_s": len(preqs) / pw, "latency_at_10_per_s": plain_lat},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["field", "broadband"], default="field")
+    ap.add_argument("--workload", choices=["field", "broadband", "serving"], default="field")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -431,6 +488,9 @@ def main():
               "(profile, frequency, azimuth) solves would shard across GPUs with no communication"
               % args.gpus, file=sys.stderr)
         return 2
+    if args.workload == "serving":
+        print(json.dumps(serving_main(args)), flush=True)
+        return 0
     cpu = None if args.no_cpu else cpu_baseline(args.workload, cpu_cores())
     if args.workload == "field":
         line = field_main(args, cpu)
