@@ -261,19 +261,25 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(const GemmProb *__restrict_
 // those with part-index sum < S: S = 2 -> v0w0 + v0w1 + v1w0 (3 MFMAs, ~16-bit products),
 // S = 3 -> adds v0w2 + v1w1 + v2w0 (6 MFMAs, ~24-bit, FP32-class).  Accumulation is FP32.
 //
-// Layouts (per problem p, written by k_asplit / k_esplit, zero padded):
-//   Ahat[s] : [Mp][Kp]  rows x modes, modes contiguous    (Mp = M rounded up to 128)
-//   Bhat[s] : [Np][Kp]  output columns x modes            (Np = 2 n_r rounded up to 128)
-// so both MFMA operands read 8 consecutive modes as one 16-byte vector.  Kp = K rounded up
-// to 32.  A block computes a 128 x 128 tile of C with 4 waves of 64 x 64 (4 x 4 MFMA
-// 16x16x32 tiles); tiles of the next k-step are fetched into registers while the current
-// k-step runs from LDS.
+// Layouts (per problem p, written by k_asplit / k_esplit, zero padded), tiled so that one
+// k-step of one 128-row tile is a single contiguous 8 KB chunk per part:
+//   Ahat[s] : [Mp/128][Kp/32][128][32]  rows x modes    (Mp = M rounded up to 128)
+//   Bhat[s] : [Np/128][Kp/32][128][32]  columns x modes (Np = 2 n_r rounded up to 128)
+// Both MFMA operands read 8 consecutive modes as one 16-byte vector.  Kp = K rounded up to
+// 32.  A block computes a 128 x 128 tile of C with 4 waves of 64 x 64 (4 x 4 MFMA 16x16x32
+// tiles); tiles of the next k-step are fetched into registers while the current k-step runs
+// from LDS.
 struct SplitProb {
     size_t a_off, b_off, c_off;   // element offsets of the problem's Ahat, Bhat blocks and C block
     int K, Kp;
 };
 
 constexpr int TBM = 128, TBN = 128, TKS = 32, TLD = TKS + 8;   // LDS row: 40 bf16 = 80 B
+
+// element offset of (row, k) in the tiled [row/128][k/32][row%128][k%32] layout
+__host__ __device__ __forceinline__ size_t tiled_index(int row, int k, int Kp) {
+    return (((size_t)(row >> 7) * (Kp / TKS) + (k >> 5)) * TBM + (row & 127)) * TKS + (k & 31);
+}
 
 __device__ __forceinline__ __bf16 bf_round(double v) { return (__bf16)(float)v; }
 
@@ -290,7 +296,7 @@ __global__ void k_asplit(const double *__restrict__ psi, size_t ldpsi, const Spl
         double v = (row < M && k < P.K) ? psi[(size_t)row * ldpsi + col_off[p] + k] : 0.0;
         for (int s = 0; s < S; ++s) {
             const __bf16 h = bf_round(v);
-            Ahat[s * split_stride + P.a_off + e] = h;
+            Ahat[s * split_stride + P.a_off + tiled_index(row, k, P.Kp)] = h;
             v -= (double)(float)h;
         }
     }
@@ -359,7 +365,7 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
             const int col = v >> 2, part = v & 3;
             const size_t n = 2 * (size_t)i0 + col;
             const uint4 val = *(const uint4 *)&T[s][col][part * 8];
-            *(uint4 *)&Bhat[s * split_stride + P.b_off + n * P.Kp + t.k0 + part * 8] = val;
+            *(uint4 *)&Bhat[s * split_stride + P.b_off + tiled_index((int)n, t.k0 + part * 8, P.Kp)] = val;
         }
 }
 
@@ -371,13 +377,12 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
     __shared__ __attribute__((aligned(16))) __bf16 As[S][TBM][TLD];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[S][TBN][TLD];
     const SplitProb P = sp[blockIdx.z];
-    // Blocks are dealt to the 8 XCDs round-robin (id % 8).  All M tiles of one N tile are
-    // kept on one XCD so its Bhat tile is fetched into one L2 only.
+    // Blocks are dealt to the 8 XCDs round-robin (id % 8).  N tiles run fastest, so the
+    // blocks resident on every XCD at a time share one or two M tiles and their Ahat tiles
+    // stay in that XCD's L2 while the Bhat tiles stream.
     const int n_tiles_n8 = (n_tiles_n + 7) & ~7;
-    const int tiles_m = gridDim.x / n_tiles_n8;
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int tm = slot % tiles_m;
-    const int tn = (slot / tiles_m) * 8 + xcd;
+    const int tm = blockIdx.x / n_tiles_n8;
+    const int tn = blockIdx.x % n_tiles_n8;
     if (tn >= n_tiles_n) return;
     const int m0 = tm * TBM, n0 = tn * TBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -399,8 +404,8 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
             for (int h = 0; h < 2; ++h) {
                 const int v = tid + 256 * h;
                 const int row = v >> 2, part = v & 3;
-                ra[s][h] = *(const uint4 *)&Ab[s * split_stride_a + (size_t)(m0 + row) * Kp + k0 + part * 8];
-                rb[s][h] = *(const uint4 *)&Bb[s * split_stride_b + (size_t)(n0 + row) * Kp + k0 + part * 8];
+                ra[s][h] = *(const uint4 *)&Ab[s * split_stride_a + tiled_index(m0 + row, k0 + part * 8, Kp)];
+                rb[s][h] = *(const uint4 *)&Bb[s * split_stride_b + tiled_index(n0 + row, k0 + part * 8, Kp)];
             }
     };
     auto stash = [&]() {
