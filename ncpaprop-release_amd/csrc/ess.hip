@@ -340,9 +340,10 @@ struct ModeWork {
 };
 
 #define NCPA_MAXIT 200
+#define NCPA_MS_PASSES 2
 
 // One lane per eigenvalue index j (ascending), 64 consecutive indices of one problem per
-// block.  Phase 1 bisects with count-only sweeps until [lo, hi) holds only eigenvalue j;
+// block.  Phase 0 narrows every bracket by multisection (see below); phase 1 bisects with count-only sweeps until [lo, hi) holds only eigenvalue j;
 // phase 2 runs Newton on det(T - s) (count kept as a safeguard) until the step is below
 // the working-precision tolerance; phase 3 is one sweep at the converged eigenvalue that
 // yields the absorption and the adjugate eigenvector products.  iters_out: sweeps used,
@@ -351,15 +352,44 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
                                               double *__restrict__ k2_out, double *__restrict__ abs_out,
                                               double *__restrict__ psis_out, double *__restrict__ psir_out,
                                               double *__restrict__ psisr_out, int *__restrict__ iters_out) {
+    __shared__ double ms_s[64];
+    __shared__ int ms_c[64];
     const ModeWork wk = work[blockIdx.x];
     const ModeProb P = mp[wk.prob];
-    const int j = wk.j0 + (int)threadIdx.x;
+    const int lane = (int)threadIdx.x;
+    const int j = wk.j0 + lane;
     const bool active = j < P.n_hi;
     double lo = P.lo, hi = P.hi;
     int cl = P.n_lo, ch = P.n_hi;
+    // phase 0: multisection.  The 64 lanes evaluate the count at 64 distinct shifts spread
+    // over the bracket of the block's eigenvalues; every lane then takes the tightest
+    // bracket of its own eigenvalue from the 64 counts.  Two passes isolate most eigenvalues,
+    // work that bisection spends ~log2(n) sweeps on per lane.
+    const int last = min(P.n_hi - 1, wk.j0 + 63) - wk.j0;   // last active lane
+    double blo = lo, bhi = hi;
+    for (int pass = 0; pass < NCPA_MS_PASSES; ++pass) {
+        const double sh = blo + (double)(lane + 1) * ((bhi - blo) * (1.0 / 65.0));
+        const int c = sturm_count(as_const(P.a), P.n, P.b2, sh);
+        ms_s[lane] = sh;
+        ms_c[lane] = c;
+        __syncthreads();
+        for (int l = 0; l < 64; ++l) {           // counts are non-decreasing in the shift
+            const int cc = ms_c[l];
+            const double ss = ms_s[l];
+            if (cc <= j) {
+                if (ss > lo) { lo = ss; cl = cc; }
+            } else if (ss < hi) {
+                hi = ss;
+                ch = cc;
+            }
+        }
+        __syncthreads();
+        blo = __shfl(lo, 0);
+        bhi = __shfl(hi, last);
+    }
     double sig = 0.5 * (lo + hi);
     bool done = !active;
-    int my_iters = 0;
+    int my_iters = NCPA_MS_PASSES;
     bool collapsed = false;
     for (int it = 0; it < NCPA_MAXIT; ++it) {
         if (__all(done)) break;
@@ -376,7 +406,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
                     if (fabs(dlt) <= P.tol) {
                         sig = cand;
                         done = true;
-                        my_iters = it + 1;
+                        my_iters = NCPA_MS_PASSES + it + 1;
                     } else if (cand > lo && cand < hi) {
                         sig = cand;
                     }
@@ -393,7 +423,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
             sig = 0.5 * (lo + hi);
             done = true;
             collapsed = (ch - cl) != 1;
-            my_iters = -(it + 1);
+            my_iters = -(NCPA_MS_PASSES + it + 1);
         }
     }
     if (!done) my_iters = -NCPA_MAXIT - 1;
