@@ -243,12 +243,41 @@ def run_field(ctx, args, prec, problems, grid):
     return res
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (copy-kernel calibrated); None when that file has no line for it."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 4 and cells[0] == kernel:
+                    return float(cells[3])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
+def copy_calibration(ctx, mib):
+    """One launch of the plain copy kernel over `mib` MiB (reference traffic for the counters)."""
+    n = mib * (1 << 20)
+    a, b = ctx.empty(n // 8), ctx.empty(n // 8)
+    ctx.check(ctx.lib.ncpa_dev_copy(ctx.h, ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), n))
+    ctx.synchronize()
+    a.free()
+    b.free()
+
+
 def field_main(args, cpu):
     import numpy as np
     import ncpamodes
     tmp = tempfile.mkdtemp(prefix="ncpa_bench_")
     paths = write_profiles(tmp)
     ctx = ncpamodes.Context(0)
+    if args.copy_calibration:
+        copy_calibration(ctx, args.copy_calibration)
     pids = [ctx.read_profile(p)[0] for p in paths]
     grid = ctx.grid(pids[0], Nz=NZ)
     problems = [ncpamodes.problem(pid, FREQ, az) for pid in pids for az in AZIMUTHS]
@@ -277,13 +306,13 @@ def field_main(args, cpu):
         roof = {"kernel": "k_gemm_split<%d> (2-D field contraction, %s)" % (3 if chosen == 3 else 2, PREC_NAMES[chosen])
                 if chosen >= 2 else "k_gemm_x4 (2-D field contraction, %s)" % PREC_NAMES[chosen],
                 "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
+                "frac": achieved / peak, "traffic": measured_traffic("k_gemm_split<%d>" % (3 if chosen == 3 else 2)),
                 "algorithmic": "4 x depths x modes x ranges real flops per problem (real x complex)"}
     else:
         avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
         achieved = FLOPS_PER_STURM_STEP * res["sturm_steps"] / avg_s / 1e12
         roof = {"kernel": "k_modes", "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f64"],
-                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"], "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"], "traffic": measured_traffic("k_modes"),
                 "algorithmic": "3 FP64 flops per Sturm step (row x sweep); FP64 VALU, MI355X FP64 peak"}
     modes_avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
     line = {
@@ -478,6 +507,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["field", "broadband", "serving"], default="field")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run the plain copy kernel over MIB MiB first (byte-counter calibration)")
     ap.add_argument("--cpu-worker", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker:

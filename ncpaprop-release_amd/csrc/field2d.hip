@@ -35,6 +35,7 @@ namespace ncpa {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 constexpr double F2_TWO_PI_HI = 6.283185307179586;
 constexpr double F2_TWO_PI_LO = 2.4492935982947064e-16;
@@ -377,12 +378,13 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
     __shared__ __attribute__((aligned(16))) __bf16 As[S][TBM][TLD];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[S][TBN][TLD];
     const SplitProb P = sp[blockIdx.z];
-    // Blocks are dealt to the 8 XCDs round-robin (id % 8).  N tiles run fastest, so the
-    // blocks resident on every XCD at a time share one or two M tiles and their Ahat tiles
-    // stay in that XCD's L2 while the Bhat tiles stream.
+    // Blocks are dealt to the 8 XCDs round-robin (id % 8).  All M tiles of one N tile are
+    // kept on one XCD, one after another, so its Bhat tile is fetched into one L2 only.
     const int n_tiles_n8 = (n_tiles_n + 7) & ~7;
-    const int tm = blockIdx.x / n_tiles_n8;
-    const int tn = blockIdx.x % n_tiles_n8;
+    const int tiles_m = gridDim.x / n_tiles_n8;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int tm = slot % tiles_m;
+    const int tn = (slot / tiles_m) * 8 + xcd;
     if (tn >= n_tiles_n) return;
     const int m0 = tm * TBM, n0 = tn * TBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -395,8 +397,10 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
     const __bf16 *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
     const int Kp = P.Kp;
     // each thread moves 2 x 16 B of A and 2 x 16 B of B per split and k-step:
-    // tile rows 128 x 4 vectors = 512 vectors per operand and split
-    uint4 ra[S][2], rb[S][2];
+    // tile rows 128 x 4 vectors = 512 vectors per operand and split.  Native vector type:
+    // arrays of the HIP uint4 struct were placed in scratch memory (a round trip through
+    // HBM every k-step); ext_vector_type arrays stay in registers.
+    u4 ra[S][2], rb[S][2];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < S; ++s)
@@ -404,8 +408,8 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
             for (int h = 0; h < 2; ++h) {
                 const int v = tid + 256 * h;
                 const int row = v >> 2, part = v & 3;
-                ra[s][h] = *(const uint4 *)&Ab[s * split_stride_a + tiled_index(m0 + row, k0 + part * 8, Kp)];
-                rb[s][h] = *(const uint4 *)&Bb[s * split_stride_b + tiled_index(n0 + row, k0 + part * 8, Kp)];
+                ra[s][h] = *(const u4 *)&Ab[s * split_stride_a + tiled_index(m0 + row, k0 + part * 8, Kp)];
+                rb[s][h] = *(const u4 *)&Bb[s * split_stride_b + tiled_index(n0 + row, k0 + part * 8, Kp)];
             }
     };
     auto stash = [&]() {
@@ -415,8 +419,8 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
             for (int h = 0; h < 2; ++h) {
                 const int v = tid + 256 * h;
                 const int row = v >> 2, part = v & 3;
-                *(uint4 *)&As[s][row][part * 8] = ra[s][h];
-                *(uint4 *)&Bs[s][row][part * 8] = rb[s][h];
+                *(u4 *)&As[s][row][part * 8] = ra[s][h];
+                *(u4 *)&Bs[s][row][part * 8] = rb[s][h];
             }
     };
     fetch(0);
