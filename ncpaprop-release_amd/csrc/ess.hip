@@ -247,6 +247,7 @@ __global__ void __launch_bounds__(256) k_interval(const ProbDev *__restrict__ pr
     __shared__ double shd[8];
     __shared__ int shi[8];
     __shared__ double s_lohi[2];
+    __shared__ double s_ring[NCPA_RING];
     const int p = blockIdx.x;
     const ProbDev pr = probs[p];
     const double *ce = CE + size_t(p) * Nz;
@@ -298,9 +299,11 @@ __global__ void __launch_bounds__(256) k_interval(const ProbDev *__restrict__ pr
     if (threadIdx.x == 0) { s_lohi[0] = lo; s_lohi[1] = hi; }
     __syncthreads();
     const double h2 = dz * dz;
-    if (threadIdx.x < 2) {
-        const int c = sturm_count(as_const(A + size_t(p) * Nz), Nz, (1.0 / h2) * (1.0 / h2), s_lohi[threadIdx.x]);
-        if (threadIdx.x == 0) out[p].n_lo = c; else out[p].n_hi = c;
+    if (threadIdx.x < 64) {   // the whole first wave sweeps (lanes 0 and 1 keep their counts)
+        const int c = sturm_count(A + size_t(p) * Nz, s_ring, Nz, (1.0 / h2) * (1.0 / h2),
+                                  s_lohi[threadIdx.x ? 1 : 0]);
+        if (threadIdx.x == 0) out[p].n_lo = c;
+        if (threadIdx.x == 1) out[p].n_hi = c;
     }
     if (threadIdx.x == 0) {
         out[p].k_min = k_min;
@@ -317,7 +320,9 @@ __global__ void __launch_bounds__(256) k_interval(const ProbDev *__restrict__ pr
 // Sturm counts at two shifts (lane 0: lo, lane 1: hi) for one matrix
 __global__ void k_counts2(const double *__restrict__ a, int n, double b2, double lo, double hi,
                           int *__restrict__ out) {
-    if (threadIdx.x < 2) out[threadIdx.x] = sturm_count(as_const(a), n, b2, threadIdx.x ? hi : lo);
+    __shared__ double s_ring[NCPA_RING];
+    const int c = sturm_count(a, s_ring, n, b2, threadIdx.x ? hi : lo);   // whole wave
+    if (threadIdx.x < 2) out[threadIdx.x] = c;
 }
 
 struct ModeProb {
@@ -335,9 +340,11 @@ struct ModeProb {
 };
 
 struct ModeWork {
-    int prob;
+    int prob;   // -1: padding block (no work)
     int j0;
 };
+
+#define NCPA_XCDS 8   // MI355X accelerator complex dies (workgroups dispatched round-robin)
 
 #define NCPA_MAXIT 200
 #define NCPA_MS_PASSES 2
@@ -354,7 +361,9 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
                                               double *__restrict__ psisr_out, int *__restrict__ iters_out) {
     __shared__ double ms_s[64];
     __shared__ int ms_c[64];
+    __shared__ double s_ring[2 * NCPA_RING];
     const ModeWork wk = work[blockIdx.x];
+    if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
     const int lane = (int)threadIdx.x;
     const int j = wk.j0 + lane;
@@ -369,7 +378,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     double blo = lo, bhi = hi;
     for (int pass = 0; pass < NCPA_MS_PASSES; ++pass) {
         const double sh = blo + (double)(lane + 1) * ((bhi - blo) * (1.0 / 65.0));
-        const int c = sturm_count(as_const(P.a), P.n, P.b2, sh);
+        const int c = sturm_count(P.a, s_ring, P.n, P.b2, sh);
         ms_s[lane] = sh;
         ms_c[lane] = c;
         __syncthreads();
@@ -395,7 +404,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
         if (__all(done)) break;
         const bool newton = !done && (ch - cl == 1);
         if (__any(newton)) {
-            const NewtonOut o = sturm_newton(as_const(P.a), P.n, P.b2, sig);
+            const NewtonOut o = sturm_newton(P.a, s_ring, P.n, P.b2, sig);
             if (!done) {
                 const double s0 = sig;
                 if (o.cnt <= j) { lo = sig; cl = o.cnt; } else { hi = sig; ch = o.cnt; }
@@ -413,7 +422,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
                 }
             }
         } else {
-            const int cnt = sturm_count(as_const(P.a), P.n, P.b2, sig);
+            const int cnt = sturm_count(P.a, s_ring, P.n, P.b2, sig);
             if (!done) {
                 if (cnt <= j) { lo = sig; cl = cnt; } else { hi = sig; ch = cnt; }
                 sig = 0.5 * (lo + hi);
@@ -428,9 +437,9 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     }
     if (!done) my_iters = -NCPA_MAXIT - 1;
     // phase 3: sensitivities at the converged eigenvalue (every lane, one sweep)
-    const FinalOut f = sturm_final(as_const(P.a), as_const(P.w), P.n, P.b2, sig, P.srow, P.rrow);
+    const FinalOut f = sturm_final(P.a, P.w, s_ring, P.n, P.b2, sig, P.srow, P.rrow);
     SVal th_s, th_r;
-    theta_forward(as_const(P.a), P.b2, sig, P.srow, P.rrow, th_s, th_r);
+    theta_forward(P.a, s_ring, P.b2, sig, P.srow, P.rrow, th_s, th_r);
     if (!active) return;
     const double mult = collapsed ? (double)(ch - cl) : 1.0;
     const double us2 = adjugate_product(th_s, f.phi_s1, f.q0, f.ex, 0, P.log2b) / mult;
@@ -475,6 +484,7 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
                                              double *__restrict__ PH, int *__restrict__ PHE,
                                              double *__restrict__ Q0, int *__restrict__ QEX) {
     const ModeWork wk = work[blockIdx.x];
+    if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
     const int j = wk.j0 + (int)threadIdx.x;
     const bool active = j < P.n_hi;
@@ -482,7 +492,8 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
     const double s = active ? k2[o] : 0.5 * (P.lo + P.hi);
     const int n = P.n;
     const double b2 = P.b2;
-    const cdptr a = as_const(P.a);
+    const double *a = P.a;
+    __shared__ double s_ring[NCPA_RING];
     const int last_rec = ra.row0 + (ra.nrec - 1) * ra.stride;
     // forward: theta_k for k = 0 .. n-2, stored at row k+1
     {
@@ -496,7 +507,7 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
         }
         if (P.srow == 0 && active) { TH[(size_t)ra.nrec * ra.ld + o] = 1.0; THE[(size_t)ra.nrec * ra.ld + o] = 0; }
         sweep_up(
-            a, n - 1,
+            a, s_ring, n - 1,
             [&](double ak, int k) {
                 const double t0 = fma(ak - s, t1, -b2 * t2);
                 t2 = t1;
@@ -514,7 +525,7 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
                     THE[(size_t)ra.nrec * ra.ld + o] = ex;
                 }
             },
-            [&]() {
+            [&](int) {
                 const int kk = pow2_exponent(t1);
                 t1 = ldexp(t1, -kk);
                 t2 = ldexp(t2, -kk);
@@ -533,7 +544,7 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
         }
         if (P.srow == n - 1 && active) { PH[(size_t)ra.nrec * ra.ld + o] = 1.0; PHE[(size_t)ra.nrec * ra.ld + o] = 0; }
         sweep_down(
-            a, n,
+            a, s_ring, n,
             [&](double ai, int i) {
                 const double am = ai - s;
                 const double p0 = fma(am, p1, -b2 * p2);
@@ -572,59 +583,126 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
 
 __device__ __forceinline__ double log2abs_s(double v, int e) { return log2(fabs(v)) + (double)e; }
 
-// one thread per mode; psi[k * ld + o] = u_k / sqrt(dz)
+// log2 |t p| + te + pe for factors of moderate size (each rescaled every 16 rows)
+__device__ __forceinline__ double log2abs_prod(double t, int te, double p, int pe) {
+    return log2(fabs(t * p)) + (double)(te + pe);
+}
+
+// Per mode: the reference row r (largest u_r^2 = -theta_{r-1} phi_{r+1} / q_0 among the
+// recorded rows), log2 u_r^2, the sign flip that makes u(source row) >= 0, and psi(zs).
+// Block = 4 waves x 64 modes; wave w scans slots w, w+4, ... and the 4 partial maxima are
+// merged in LDS.
+__global__ void __launch_bounds__(256) k_rows_ref(const double *__restrict__ TH, const int *__restrict__ THE,
+                                                  const double *__restrict__ PH, const int *__restrict__ PHE,
+                                                  const double *__restrict__ Q0, const int *__restrict__ QEX,
+                                                  const int *__restrict__ mode_prob,
+                                                  const ModeProb *__restrict__ mp, RowsArgs ra, size_t n_modes,
+                                                  int *__restrict__ ref_slot, double *__restrict__ ref_l2,
+                                                  double *__restrict__ flip_out, double *__restrict__ psi_src) {
+    __shared__ double s_l2[4][64];
+    __shared__ int s_k[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t o = blockIdx.x * (size_t)64 + lane;
+    const bool live = o < n_modes;
+    double l2r = -INFINITY;
+    int r = -1;
+    if (live) {
+        const double q0 = Q0[o];
+        const double lq = log2abs_s(q0, QEX[o]);
+        const double sq = copysign(1.0, q0);
+        for (int k = wave; k < ra.nrec; k += 4) {
+            const double t = TH[(size_t)k * ra.ld + o], p = PH[(size_t)k * ra.ld + o];
+            if (t == 0.0 || p == 0.0) continue;
+            if (-copysign(1.0, t) * copysign(1.0, p) * sq <= 0.0) continue;
+            const double l2 = log2abs_prod(t, THE[(size_t)k * ra.ld + o], p, PHE[(size_t)k * ra.ld + o]) - lq;
+            if (l2 > l2r) { l2r = l2; r = k; }
+        }
+    }
+    s_l2[wave][lane] = l2r;
+    s_k[wave][lane] = r;
+    __syncthreads();
+    if (wave != 0 || !live) return;
+    for (int w = 1; w < 4; ++w) {   // ties go to the lowest slot, as a sequential scan would
+        const double l2 = s_l2[w][lane];
+        const int k = s_k[w][lane];
+        if (k >= 0 && (l2 > l2r || (l2 == l2r && k < r))) { l2r = l2; r = k; }
+    }
+    const ModeProb P = mp[mode_prob[o]];
+    const double inv_sqrt_dz = sqrt(P.inv_dz);
+    ref_slot[o] = r;
+    ref_l2[o] = l2r;
+    if (r < 0) {
+        flip_out[o] = 0.0;
+        psi_src[o] = 0.0;
+        return;
+    }
+    const double q0 = Q0[o];
+    const double lq = log2abs_s(q0, QEX[o]);
+    const double sq = copysign(1.0, q0);
+    const int row_r = ra.row0 + r * ra.stride;
+    const size_t ro = (size_t)r * ra.ld + o, so = (size_t)ra.nrec * ra.ld + o;
+    // u(source row) u_r / u_r with the adjugate product of the sorted pair
+    double tv, pv;
+    int tev, pev, d;
+    if (P.srow <= row_r) { tv = TH[so]; tev = THE[so]; pv = PH[ro]; pev = PHE[ro]; d = row_r - P.srow; }
+    else { tv = TH[ro]; tev = THE[ro]; pv = PH[so]; pev = PHE[so]; d = P.srow - row_r; }
+    double us = 0.0;
+    if (tv != 0.0 && pv != 0.0) {
+        const double sg = ((d & 1) ? 1.0 : -1.0) * copysign(1.0, tv) * copysign(1.0, pv) * sq;
+        us = sg * exp2(log2abs_prod(tv, tev, pv, pev) + d * P.log2b - lq - 0.5 * l2r);
+    }
+    // psi(zs) from the reference row: accurate also for modes trapped aloft, where the
+    // square root of the adjugate diagonal at the ground would sit at its noise floor
+    psi_src[o] = fabs(us) * inv_sqrt_dz;
+    flip_out[o] = (us < 0.0) ? -inv_sqrt_dz : inv_sqrt_dz;
+}
+
+// psi[k * ld + o] = flip * u_k: one thread per (slot, mode); block = 64 modes x 4 slot
+// lanes, each thread 8 slots (k = 4 blockIdx.y * 8 + ...).
+#define NCPA_ROWS_PER_THREAD 8
 __global__ void __launch_bounds__(256) k_rows_combine(const double *__restrict__ TH, const int *__restrict__ THE,
                                                       const double *__restrict__ PH, const int *__restrict__ PHE,
                                                       const double *__restrict__ Q0, const int *__restrict__ QEX,
                                                       const int *__restrict__ mode_prob,
-                                                      const ModeProb *__restrict__ mp, RowsArgs ra,
-                                                      size_t n_modes, double *__restrict__ psi,
-                                                      double *__restrict__ psi_src) {
-    const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+                                                      const ModeProb *__restrict__ mp, RowsArgs ra, size_t n_modes,
+                                                      const int *__restrict__ ref_slot,
+                                                      const double *__restrict__ ref_l2,
+                                                      const double *__restrict__ flip_in, double *__restrict__ psi) {
+    const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    const size_t o = blockIdx.x * (size_t)64 + lane;
     if (o >= n_modes) return;
+    const int k0 = (blockIdx.y * 4 + sub) * NCPA_ROWS_PER_THREAD;
+    if (k0 >= ra.nrec) return;
+    const int k1 = min(ra.nrec, k0 + NCPA_ROWS_PER_THREAD);
+    const int r = ref_slot[o];
+    const double flip = flip_in[o];
+    if (r < 0) {
+        for (int k = k0; k < k1; ++k) psi[(size_t)k * ra.ld + o] = 0.0;
+        return;
+    }
     const ModeProb P = mp[mode_prob[o]];
     const double q0 = Q0[o];
     const double lq = log2abs_s(q0, QEX[o]);
     const double sq = copysign(1.0, q0);
-    // reference row: largest u^2 among the recorded rows
-    int r = -1;
-    double l2r = -INFINITY;
-    for (int k = 0; k < ra.nrec; ++k) {
-        const double t = TH[(size_t)k * ra.ld + o], p = PH[(size_t)k * ra.ld + o];
-        if (t == 0.0 || p == 0.0) continue;
-        if (-copysign(1.0, t) * copysign(1.0, p) * sq <= 0.0) continue;
-        const double l2 = log2abs_s(t, THE[(size_t)k * ra.ld + o]) + log2abs_s(p, PHE[(size_t)k * ra.ld + o]) - lq;
-        if (l2 > l2r) { l2r = l2; r = k; }
-    }
-    const double inv_sqrt_dz = sqrt(P.inv_dz);
-    if (r < 0) {
-        for (int k = 0; k < ra.nrec; ++k) psi[(size_t)k * ra.ld + o] = 0.0;
-        psi_src[o] = 0.0;
-        return;
-    }
+    const double l2r = ref_l2[o];
     const int row_r = ra.row0 + r * ra.stride;
-    const double tr = TH[(size_t)r * ra.ld + o], pr = PH[(size_t)r * ra.ld + o];
-    const int ter = THE[(size_t)r * ra.ld + o], per = PHE[(size_t)r * ra.ld + o];
-    auto u_at = [&](double t, int te, double p, int pe, int row) -> double {
-        // u_row u_r / u_r with the adjugate product for the sorted pair
+    const size_t ro = (size_t)r * ra.ld + o;
+    const double tr = TH[ro], pr = PH[ro];
+    const int ter = THE[ro], per = PHE[ro];
+    for (int k = k0; k < k1; ++k) {
+        const size_t ko = (size_t)k * ra.ld + o;
+        const int row = ra.row0 + k * ra.stride;
+        // u_row u_r / u_r with the adjugate product of the sorted pair
         double tv, pv;
         int tev, pev, d;
-        if (row <= row_r) { tv = t; tev = te; pv = pr; pev = per; d = row_r - row; }
-        else { tv = tr; tev = ter; pv = p; pev = pe; d = row - row_r; }
-        if (tv == 0.0 || pv == 0.0) return 0.0;
-        const double sg = ((d & 1) ? 1.0 : -1.0) * copysign(1.0, tv) * copysign(1.0, pv) * sq;
-        const double l2 = log2abs_s(tv, tev) + log2abs_s(pv, pev) + d * P.log2b - lq - 0.5 * l2r;
-        return sg * exp2(l2);
-    };
-    const size_t so = (size_t)ra.nrec * ra.ld + o;
-    const double us = u_at(TH[so], THE[so], PH[so], PHE[so], P.srow);
-    const double flip = (us < 0.0) ? -inv_sqrt_dz : inv_sqrt_dz;
-    // psi(zs) from the reference row: accurate also for modes trapped aloft, where the
-    // square root of the adjugate diagonal at the ground would sit at its noise floor
-    psi_src[o] = fabs(us) * inv_sqrt_dz;
-    for (int k = 0; k < ra.nrec; ++k) {
-        const size_t ko = (size_t)k * ra.ld + o;
-        psi[ko] = flip * u_at(TH[ko], THE[ko], PH[ko], PHE[ko], ra.row0 + k * ra.stride);
+        if (row <= row_r) { tv = TH[ko]; tev = THE[ko]; pv = pr; pev = per; d = row_r - row; }
+        else { tv = tr; tev = ter; pv = PH[ko]; pev = PHE[ko]; d = row - row_r; }
+        double u = 0.0;
+        if (tv != 0.0 && pv != 0.0) {
+            const double sg = ((d & 1) ? 1.0 : -1.0) * copysign(1.0, tv) * copysign(1.0, pv) * sq;
+            u = sg * exp2(log2abs_prod(tv, tev, pv, pev) + d * P.log2b - lq - 0.5 * l2r);
+        }
+        psi[ko] = flip * u;
     }
 }
 
@@ -658,18 +736,25 @@ __device__ __forceinline__ double reduce_2pi(double x) {
     return fma(-nq, TWO_PI_LO, r);
 }
 
+// One thread per NCPA_TL_R consecutive ranges r_j = (i0 + j + 1) dr.  Per mode the phasor
+// e^{i k r} is formed exactly (FP64 phase reduced mod 2 pi, sincos, exp) at the thread's
+// first range and advanced by the exact per-step factor e^{i k dr} for the others: a complex
+// multiply per term instead of a sincos and an exp (relative drift ~ NCPA_TL_R ulp).
+#define NCPA_TL_R 8
 __global__ void __launch_bounds__(256) k_tl1d(const TlProb *__restrict__ tp, const double *__restrict__ k2,
                                               const double *__restrict__ absn,
                                               const double *__restrict__ psisr, double dr, int n_r,
                                               int lossless, double ev_re, double ev_im,
                                               double *__restrict__ p_re, double *__restrict__ p_im,
                                               double *__restrict__ incoh) {
-    __shared__ double s_kr[256], s_ki[256], s_cr[256], s_ci[256], s_inc[256];
+    __shared__ double s_kr[256], s_ki[256], s_cr[256], s_ci[256], s_inc[256], s_mr[256], s_mi[256];
     const int p = blockIdx.y;
     const TlProb t = tp[p];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const double r = (i + 1) * dr;
-    double sr = 0.0, si = 0.0, sinc = 0.0;
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * NCPA_TL_R;
+    const double r0 = (i0 + 1) * dr;
+    double sr[NCPA_TL_R], si[NCPA_TL_R], sinc[NCPA_TL_R];
+#pragma unroll
+    for (int j = 0; j < NCPA_TL_R; ++j) sr[j] = si[j] = sinc[j] = 0.0;
     for (int m0 = 0; m0 < t.nm; m0 += 256) {
         const int m = m0 + (int)threadIdx.x;
         if (m < t.nm) {
@@ -692,36 +777,50 @@ __global__ void __launch_bounds__(256) k_tl1d(const TlProb *__restrict__ tp, con
                 ci = -pp * si_ / den;
                 cinc = pp * pp / hypot(kr, ki);
             }
+            double sn, cs;
+            sincos(reduce_2pi(kr * dr), &sn, &cs);
+            const double ad = exp(-ki * dr);
             s_kr[threadIdx.x] = kr;
             s_ki[threadIdx.x] = ki;
             s_cr[threadIdx.x] = cr;
             s_ci[threadIdx.x] = ci;
             s_inc[threadIdx.x] = cinc;
+            s_mr[threadIdx.x] = ad * cs;
+            s_mi[threadIdx.x] = ad * sn;
         }
         __syncthreads();
         const int cnt = min(256, t.nm - m0);
-        if (i < n_r) {
+        if (i0 < n_r) {
             for (int q = 0; q < cnt; ++q) {
-                const double ph = reduce_2pi(s_kr[q] * r);
                 double sn, cs;
-                sincos(ph, &sn, &cs);
-                const double amp = exp(-s_ki[q] * r);
-                const double er = amp * cs, ei = amp * sn;
-                sr = fma(s_cr[q], er, fma(-s_ci[q], ei, sr));
-                si = fma(s_cr[q], ei, fma(s_ci[q], er, si));
-                sinc = fma(s_inc[q], amp * amp, sinc);
+                sincos(reduce_2pi(s_kr[q] * r0), &sn, &cs);
+                const double amp = exp(-s_ki[q] * r0);
+                double zr = amp * cs, zi = amp * sn;
+                const double cr = s_cr[q], ci = s_ci[q], ci2 = s_inc[q], mr = s_mr[q], mi = s_mi[q];
+#pragma unroll
+                for (int j = 0; j < NCPA_TL_R; ++j) {
+                    sr[j] = fma(cr, zr, fma(-ci, zi, sr[j]));
+                    si[j] = fma(cr, zi, fma(ci, zr, si[j]));
+                    sinc[j] = fma(ci2, fma(zr, zr, zi * zi), sinc[j]);
+                    const double nr_ = fma(zr, mr, -zi * mi);
+                    zi = fma(zr, mi, zi * mr);
+                    zr = nr_;
+                }
             }
         }
         __syncthreads();
     }
-    if (i >= n_r) return;
-    const double isr = 1.0 / sqrt(r);
-    const double pr_ = 1000.0 * (ev_re * sr - ev_im * si) * isr;
-    const double pi_ = 1000.0 * (ev_re * si + ev_im * sr) * isr;
-    const size_t o = size_t(p) * n_r + i;
-    p_re[o] = pr_;
-    p_im[o] = pi_;
-    incoh[o] = 1000.0 * 4 * PI * sqrt(sinc) * sqrt(1. / 8. / PI / r);
+#pragma unroll
+    for (int j = 0; j < NCPA_TL_R; ++j) {
+        const int i = i0 + j;
+        if (i >= n_r) break;
+        const double r = (i + 1) * dr;
+        const double isr = 1.0 / sqrt(r);
+        const size_t o = size_t(p) * n_r + i;
+        p_re[o] = 1000.0 * (ev_re * sr[j] - ev_im * si[j]) * isr;
+        p_im[o] = 1000.0 * (ev_re * si[j] + ev_im * sr[j]) * isr;
+        incoh[o] = 1000.0 * 4 * PI * sqrt(sinc[j]) * sqrt(1. / 8. / PI / r);
+    }
 }
 
 }  // namespace ncpa
@@ -803,12 +902,31 @@ int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, con
         q.n_lo = v.n_lo;
         q.n_hi = v.n_lo + nm;
         q.out_off = total;
-        for (int j0 = v.n_lo; j0 < v.n_lo + nm; j0 += 64) work.push_back({p, j0});
         mode_prob.insert(mode_prob.end(), nm, p);
         total += nm;
     }
     m->offsets[P] = total;
     m->total = total;
+    // XCD-aware block order.  Workgroups go to the 8 XCDs round-robin (block b -> XCD b % 8),
+    // so every problem is given whole to one XCD (balancing 64-mode groups) and its blocks
+    // are placed at b = 8t + x: the waves sweeping one diagonal share that XCD's L2.
+    {
+        std::vector<std::vector<ModeWork>> per(NCPA_XCDS);
+        std::vector<long> load(NCPA_XCDS, 0);
+        for (int p = 0; p < P; ++p) {
+            const int nm = m->counts[p];
+            if (nm == 0) continue;
+            const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+            for (int j0 = m->iv[p].n_lo; j0 < m->iv[p].n_lo + nm; j0 += 64) per[x].push_back({p, j0});
+            load[x] += (nm + 63) / 64;
+        }
+        size_t L = 0;
+        for (auto &v : per) L = std::max(L, v.size());
+        work.assign(L * NCPA_XCDS, ModeWork{-1, 0});
+        for (size_t t = 0; t < L; ++t)
+            for (int x = 0; x < NCPA_XCDS; ++x)
+                if (t < per[x].size()) work[t * NCPA_XCDS + x] = per[x][t];
+    }
     m->n_work = work.size();
     const size_t tot_alloc = std::max(total, 1);
     m->d_k2 = pool_alloc<double>(ctx, tot_alloc);
@@ -847,11 +965,13 @@ int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, do
     int *THE = pool_alloc<int>(ctx, slots * ld), *PHE = pool_alloc<int>(ctx, slots * ld);
     double *Q0 = pool_alloc<double>(ctx, ld);
     int *QEX = pool_alloc<int>(ctx, ld);
+    int *RS = pool_alloc<int>(ctx, ld);
+    double *RL = pool_alloc<double>(ctx, ld), *FL = pool_alloc<double>(ctx, ld);
     auto cleanup = [&]() {
         ctx->pool.put(TH); ctx->pool.put(PH); ctx->pool.put(THE); ctx->pool.put(PHE);
-        ctx->pool.put(Q0); ctx->pool.put(QEX);
+        ctx->pool.put(Q0); ctx->pool.put(QEX); ctx->pool.put(RS); ctx->pool.put(RL); ctx->pool.put(FL);
     };
-    if (!TH || !PH || !THE || !PHE || !Q0 || !QEX) {
+    if (!TH || !PH || !THE || !PHE || !Q0 || !QEX || !RS || !RL || !FL) {
         cleanup();
         return set_error(ctx, NCPA_ERR_ALLOC, "eigenvector rows: scratch allocation failed");
     }
@@ -865,8 +985,12 @@ int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, do
         TimedLaunch tl(ctx, "rows");
         hipLaunchKernelGGL(k_rows, dim3((unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
                            m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
-        hipLaunchKernelGGL(k_rows_combine, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, ctx->stream, TH, THE,
-                           PH, PHE, Q0, QEX, m->d_mode_prob, m->d_mp, ra, ld, dev_psi, m->d_psis_rows);
+        const unsigned mb = (unsigned)((ld + 63) / 64);
+        hipLaunchKernelGGL(k_rows_ref, dim3(mb), dim3(256), 0, ctx->stream, TH, THE, PH, PHE, Q0, QEX,
+                           m->d_mode_prob, m->d_mp, ra, ld, RS, RL, FL, m->d_psis_rows);
+        const unsigned sb = (unsigned)((nrec + 4 * NCPA_ROWS_PER_THREAD - 1) / (4 * NCPA_ROWS_PER_THREAD));
+        hipLaunchKernelGGL(k_rows_combine, dim3(mb, sb), dim3(256), 0, ctx->stream, TH, THE, PH, PHE, Q0, QEX,
+                           m->d_mode_prob, m->d_mp, ra, ld, RS, RL, FL, dev_psi);
     }
     hipError_t e = hipGetLastError();
     cleanup();
@@ -1081,7 +1205,7 @@ int ncpa_tl1d(ncpa_ctx_t *ctx, ncpa_modes_t *m, double dr, int n_r, int lossless
     const std::complex<double> ev = 4 * PI * I * std::exp(-I * PI * 0.25) / std::sqrt(8.0 * PI);
     {
         TimedLaunch tl(ctx, "tl1d");
-        dim3 g((n_r + 255) / 256, m->P);
+        dim3 g((n_r + 256 * NCPA_TL_R - 1) / (256 * NCPA_TL_R), m->P);
         hipLaunchKernelGGL(k_tl1d, g, dim3(256), 0, ctx->stream, d_tp, m->d_k2, m->d_abs, m->d_psisr,
                            dr, n_r, lossless, ev.real(), ev.imag(), p_re, p_im, incoh);
     }

@@ -59,16 +59,19 @@ __device__ __forceinline__ void f2_csqrt(double x, double y, double &re, double 
 
 enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3 };
 
-// B rows for every mode: 1-D grid of total * ceil(nr/256) blocks
+// B rows for every mode: 1-D grid of total * ceil(nr / (256 * EMAT_R)) blocks; a thread
+// forms EMAT_R consecutive ranges, the phasor exact at the first and advanced by the exact
+// factor e^{i k dr} for the others.
+#define EMAT_R 8
 template <int PREC>
 __global__ void __launch_bounds__(256) k_emat(const double *__restrict__ k2, const double *__restrict__ absn,
                                               const double *__restrict__ psis, double dr, int nr, int lossless,
                                               double ev_re, double ev_im, void *__restrict__ bh,
                                               void *__restrict__ bl) {
-    const unsigned nb = (nr + 255) / 256;
+    const unsigned nb = (nr + 256 * EMAT_R - 1) / (256 * EMAT_R);
     const size_t o = blockIdx.x / nb;
-    const int i = (blockIdx.x % nb) * blockDim.x + threadIdx.x;
-    if (i >= nr) return;
+    const int i0 = ((blockIdx.x % nb) * blockDim.x + threadIdx.x) * EMAT_R;
+    if (i0 >= nr) return;
     double kr, ki;
     f2_csqrt(k2[o], absn[o], kr, ki);              // k_pert = sqrt(k^2 + i A)
     double cr, ci;
@@ -83,30 +86,40 @@ __global__ void __launch_bounds__(256) k_emat(const double *__restrict__ k2, con
         cr = psis[o] * sr / den;
         ci = -psis[o] * si / den;
     }
-    const double r = (i + 1) * dr;
+    // g = ev c
+    const double gr = cr * ev_re - ci * ev_im, gi = cr * ev_im + ci * ev_re;
+    const double r0 = (i0 + 1) * dr;
     double sn, cs;
-    sincos(f2_reduce(kr * r), &sn, &cs);
-    const double amp = exp(-ki * r) * (1000.0 / sqrt(r));
-    // (c e^{i k r}) (ev) amp
-    const double er = cs * amp, ei = sn * amp;
-    const double tr = cr * er - ci * ei, ti = cr * ei + ci * er;
-    const double vr = tr * ev_re - ti * ev_im, vi = tr * ev_im + ti * ev_re;
-    const size_t idx = o * (size_t)(2 * nr) + 2 * (size_t)i;
-    if (PREC == PREC_FP64) {
-        double *b = (double *)bh;
-        b[idx] = vr;
-        b[idx + 1] = vi;
-    } else if (PREC == PREC_FP32) {
-        float *b = (float *)bh;
-        b[idx] = (float)vr;
-        b[idx + 1] = (float)vi;
-    } else {
-        __bf16 *h = (__bf16 *)bh, *l = (__bf16 *)bl;
-        const __bf16 hr = (__bf16)(float)vr, hi = (__bf16)(float)vi;
-        h[idx] = hr;
-        h[idx + 1] = hi;
-        l[idx] = (__bf16)(float)(vr - (double)(float)hr);
-        l[idx + 1] = (__bf16)(float)(vi - (double)(float)hi);
+    sincos(f2_reduce(kr * r0), &sn, &cs);
+    const double a0 = exp(-ki * r0);
+    double zr = a0 * cs, zi = a0 * sn;
+    sincos(f2_reduce(kr * dr), &sn, &cs);
+    const double a1 = exp(-ki * dr);
+    const double mr = a1 * cs, mi = a1 * sn;
+    for (int j = 0; j < EMAT_R && i0 + j < nr; ++j) {
+        const int i = i0 + j;
+        const double sc = 1000.0 / sqrt((i + 1) * dr);
+        const double vr = (gr * zr - gi * zi) * sc, vi = (gr * zi + gi * zr) * sc;
+        const double nz = zr * mr - zi * mi;
+        zi = zr * mi + zi * mr;
+        zr = nz;
+        const size_t idx = o * (size_t)(2 * nr) + 2 * (size_t)i;
+        if (PREC == PREC_FP64) {
+            double *b = (double *)bh;
+            b[idx] = vr;
+            b[idx + 1] = vi;
+        } else if (PREC == PREC_FP32) {
+            float *b = (float *)bh;
+            b[idx] = (float)vr;
+            b[idx + 1] = (float)vi;
+        } else {
+            __bf16 *h = (__bf16 *)bh, *l = (__bf16 *)bl;
+            const __bf16 hr = (__bf16)(float)vr, hi = (__bf16)(float)vi;
+            h[idx] = hr;
+            h[idx + 1] = hi;
+            l[idx] = (__bf16)(float)(vr - (double)(float)hr);
+            l[idx + 1] = (__bf16)(float)(vi - (double)(float)hi);
+        }
     }
 }
 
@@ -337,19 +350,39 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
             ci = -psis[o] * si / den;
         }
     }
+    // 1000 / sqrt(r) of the tile's 64 ranges
+    __shared__ double s_sc[64];
+    if (threadIdx.x < 64) s_sc[threadIdx.x] = 1000.0 / sqrt((i0 + (int)threadIdx.x + 1) * dr);
+    // g = ev c; the thread's ranges are i0 + w + 8q: e^{i k r} exact at q = 0, then advanced
+    // by the exact factor e^{i k 8 dr} (a complex multiply instead of a sincos and an exp)
+    const double gr = cr * ev_re - ci * ev_im, gi = cr * ev_im + ci * ev_re;
+    double zr = 0.0, zi = 0.0, mr = 0.0, mi = 0.0;
+    if (live) {
+        const double r0 = (i0 + (threadIdx.x >> 5) + 1) * dr;
+        double sn, cs;
+        sincos(f2_reduce(kr * r0), &sn, &cs);
+        const double a0 = exp(-ki * r0);
+        zr = a0 * cs;
+        zi = a0 * sn;
+        sincos(f2_reduce(kr * (8.0 * dr)), &sn, &cs);
+        const double a8 = exp(-ki * (8.0 * dr));
+        mr = a8 * cs;
+        mi = a8 * sn;
+    }
+    __syncthreads();
     for (int q = 0; q < 8; ++q) {
         const int il = (threadIdx.x >> 5) + 8 * q;   // range within the tile, 0..63
         const int i = i0 + il;
         double vr = 0.0, vi = 0.0;
         if (live && i < nr) {
-            const double r = (i + 1) * dr;
-            double sn, cs;
-            sincos(f2_reduce(kr * r), &sn, &cs);
-            const double amp = exp(-ki * r) * (1000.0 / sqrt(r));
-            const double er = cs * amp, ei = sn * amp;
-            const double tr = cr * er - ci * ei, ti = cr * ei + ci * er;
-            vr = tr * ev_re - ti * ev_im;
-            vi = tr * ev_im + ti * ev_re;
+            const double sc = s_sc[il];
+            vr = (gr * zr - gi * zi) * sc;
+            vi = (gr * zi + gi * zr) * sc;
+        }
+        {
+            const double nz = zr * mr - zi * mi;
+            zi = zr * mi + zi * mr;
+            zr = nz;
         }
         for (int s = 0; s < S; ++s) {
             const __bf16 hr = bf_round(vr), hi = bf_round(vi);
@@ -592,7 +625,7 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, i
     const std::complex<double> ev = 4 * PI * I * std::exp(-I * PI * 0.25) / std::sqrt(8.0 * PI);
     if (total > 0) {
         TimedLaunch tl(ctx, "emat");
-        dim3 ge((unsigned)(((n_r + 255) / 256) * (size_t)total));
+        dim3 ge((unsigned)(((n_r + 256 * EMAT_R - 1) / (256 * EMAT_R)) * (size_t)total));
         if (precision == PREC_FP64)
             hipLaunchKernelGGL(k_emat<PREC_FP64>, ge, dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis, dr, n_r,
                                lossless, ev.real(), ev.imag(), bh, bl);

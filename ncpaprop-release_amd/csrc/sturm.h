@@ -18,18 +18,20 @@
 // come out tiny instead of at a noise floor.
 //
 // Memory: every lane of a wave sweeps the same problem, so the diagonal is wave-uniform.
-// It is streamed through the scalar data cache (constant address space loads, 16 rows per
-// chunk, the next chunk requested before the current one is consumed), so the per-row work
-// is pure FP64 VALU.
+// It is staged through a wave-private LDS ring of two 256-row tiles: the 64 lanes load the
+// next tile with coalesced vector loads (lane l takes rows t + l + 64u) one whole tile of
+// compute ahead of its use, then every lane reads the tile back with broadcast LDS reads,
+// 16 rows at a time, the next 16 requested before the current ones are consumed.  A sweep
+// must be entered by the full wave (all 64 lanes active): the lanes share the tile loads.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cfloat>
 
 namespace ncpa {
 
-typedef const __attribute__((address_space(4))) double *cdptr;
-
-__device__ __forceinline__ cdptr as_const(const double *p) { return (cdptr)(p); }
+#define NCPA_TILE 256                 // rows per LDS tile
+#define NCPA_RING (2 * NCPA_TILE)     // doubles of LDS per staged array (two tiles)
+#define NCPA_CHUNK 16                 // rows between rescales / sign-count flushes
 
 // exponent such that ldexp(x, -e) lies in [0.5, 1)
 __device__ __forceinline__ int pow2_exponent(double x) {
@@ -55,104 +57,215 @@ __device__ __forceinline__ int sign_changes(unsigned bits, int m) {
     return __builtin_popcount((bits ^ (bits >> 1)) & mask);
 }
 
-#define NCPA_CHUNK 16
+// LDS writes of one lane are read by the other lanes of the same wave: LDS executes a
+// wave's instructions in order, so only the compiler has to be kept from reordering.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One array staged tile by tile: load() issues the coalesced global loads of tile c into
+// registers, store() writes them to the ring slot of tile c.
+struct TileStage {
+    const double *g;
+    double *ring;
+    int n;
+    int lane;
+    double pre[NCPA_TILE / 64];
+    __device__ __forceinline__ void load(int c) {
+#pragma unroll
+        for (int u = 0; u < NCPA_TILE / 64; ++u) {
+            const int r = c * NCPA_TILE + lane + 64 * u;
+            pre[u] = (c >= 0 && r < n) ? g[r] : 0.0;
+        }
+    }
+    __device__ __forceinline__ void store(int c) {
+        double *s = ring + (c & 1) * NCPA_TILE;
+#pragma unroll
+        for (int u = 0; u < NCPA_TILE / 64; ++u) s[lane + 64 * u] = pre[u];
+    }
+};
+
+__device__ __forceinline__ void lds_chunk(double (&v)[NCPA_CHUNK], const double *s) {
+#pragma unroll
+    for (int u = 0; u < NCPA_CHUNK; ++u) v[u] = s[u];
+}
+
+// The first chunk of a tile passes through an empty asm statement: were it a plain load
+// like the prefetches inside the chunk loop, the compiler would merge the loop-carried pair
+// of loads into one load at the top of the loop and wait on it there, undoing the
+// prefetch.  (The asm only pins registers; it waits for this one chunk, once per tile.)
+__device__ __forceinline__ void lds_chunk_first(double (&v)[NCPA_CHUNK], const double *s) {
+#pragma unroll
+    for (int u = 0; u < NCPA_CHUNK; ++u) {
+        v[u] = s[u];
+        asm volatile("" : "+v"(v[u]));
+    }
+}
+
+// The full 16-row chunks of one tile, in sweep order: chunk k starts at row
+// cb0 - 16k (Down) or cb0 + 16k (up); s points at the tile's first row `base`.  Two register
+// buffers alternate: the next chunk is read from LDS before the current one is stepped
+// through (the scheduling barrier keeps the reads ahead of the arithmetic).
+template <bool Down, class Step>
+__device__ __forceinline__ void step_chunk(const double (&v)[NCPA_CHUNK], int cb, Step &step) {
+    if (Down) {
+#pragma unroll
+        for (int u = NCPA_CHUNK - 1; u >= 0; --u) step(v[u], cb + u);
+    } else {
+#pragma unroll
+        for (int u = 0; u < NCPA_CHUNK; ++u) step(v[u], cb + u);
+    }
+}
+
+// s_waitcnt lgkmcnt(0) (vmcnt and expcnt left at their maxima): the LDS reads issued a
+// whole chunk earlier have returned; waiting here, before the next reads are issued, keeps
+// the compiler's own wait from also covering the reads just issued.
+__device__ __forceinline__ void wait_lds() { __builtin_amdgcn_s_waitcnt(0xc07f); }
+
+template <bool Down, class Step, class ChunkEnd>
+__device__ __forceinline__ void tile_chunks(const double *s, int base, int cb0, int nch, Step &step,
+                                            ChunkEnd &chunk_end) {
+    if (nch <= 0) return;
+    const int dir = Down ? -NCPA_CHUNK : NCPA_CHUNK;
+    double va[NCPA_CHUNK], vb[NCPA_CHUNK];
+    lds_chunk_first(va, s + (cb0 - base));
+    int k = 0;
+    for (; k + 1 < nch; k += 2) {
+        const int c0 = cb0 + k * dir, c1 = c0 + dir;
+        lds_chunk(vb, s + (c1 - base));
+        __builtin_amdgcn_sched_barrier(0);
+        step_chunk<Down>(va, c0, step);
+        chunk_end(NCPA_CHUNK);
+        __builtin_amdgcn_sched_barrier(0);
+        wait_lds();
+        const int c2 = (k + 2 < nch) ? c1 + dir : c1;
+        lds_chunk(va, s + (c2 - base));
+        __builtin_amdgcn_sched_barrier(0);
+        step_chunk<Down>(vb, c1, step);
+        chunk_end(NCPA_CHUNK);
+        __builtin_amdgcn_sched_barrier(0);
+        wait_lds();
+    }
+    if (k < nch) {
+        step_chunk<Down>(va, cb0 + k * dir, step);
+        chunk_end(NCPA_CHUNK);
+    }
+}
 
 // Drive `step(a_i, i)` over rows i = n-1 .. 0, then `chunk_end(rows_in_chunk)` after each
-// chunk of rows; the diagonal comes through double-buffered scalar loads.
+// chunk of rows: first the n % 16 top rows, then chunks [16c, 16c+15] downwards.
 template <class Step, class ChunkEnd>
-__device__ __forceinline__ void sweep_down(cdptr a, int n, Step step, ChunkEnd chunk_end) {
-    const int nfull = n / NCPA_CHUNK;          // chunks [16c, 16c+15], c < nfull
-    const int rem = n - nfull * NCPA_CHUNK;    // top rows n-1 .. 16*nfull
-    if (rem) {
-        for (int i = n - 1; i >= nfull * NCPA_CHUNK; --i) step(a[i], i);
-        chunk_end(rem);
-    }
-    if (nfull == 0) return;
-    double buf[NCPA_CHUNK];
-    int c = nfull - 1;
-#pragma unroll
-    for (int u = 0; u < NCPA_CHUNK; ++u) buf[u] = a[c * NCPA_CHUNK + u];
-    for (; c >= 0; --c) {
-        double nxt[NCPA_CHUNK];
-        const int cn = c > 0 ? c - 1 : 0;
-#pragma unroll
-        for (int u = 0; u < NCPA_CHUNK; ++u) nxt[u] = a[cn * NCPA_CHUNK + u];
-#pragma unroll
-        for (int u = NCPA_CHUNK - 1; u >= 0; --u) step(buf[u], c * NCPA_CHUNK + u);
-        chunk_end(NCPA_CHUNK);
-#pragma unroll
-        for (int u = 0; u < NCPA_CHUNK; ++u) buf[u] = nxt[u];
-    }
-}
-
-// two-array variant (diagonal and absorption weights), 8-row chunks
-#define NCPA_CHUNK2 8
-template <class Step, class ChunkEnd>
-__device__ __forceinline__ void sweep_down2(cdptr a, cdptr w, int n, Step step, ChunkEnd chunk_end) {
-    const int nfull = n / NCPA_CHUNK2;
-    const int rem = n - nfull * NCPA_CHUNK2;
-    if (rem) {
-        for (int i = n - 1; i >= nfull * NCPA_CHUNK2; --i) step(a[i], w[i], i);
-        chunk_end(rem);
-    }
-    if (nfull == 0) return;
-    double ba[NCPA_CHUNK2], bw[NCPA_CHUNK2];
-    int c = nfull - 1;
-#pragma unroll
-    for (int u = 0; u < NCPA_CHUNK2; ++u) {
-        ba[u] = a[c * NCPA_CHUNK2 + u];
-        bw[u] = w[c * NCPA_CHUNK2 + u];
-    }
-    for (; c >= 0; --c) {
-        double na[NCPA_CHUNK2], nw[NCPA_CHUNK2];
-        const int cn = c > 0 ? c - 1 : 0;
-#pragma unroll
-        for (int u = 0; u < NCPA_CHUNK2; ++u) {
-            na[u] = a[cn * NCPA_CHUNK2 + u];
-            nw[u] = w[cn * NCPA_CHUNK2 + u];
+__device__ __forceinline__ void sweep_down(const double *a, double *ring, int n, Step step, ChunkEnd chunk_end) {
+    if (n <= 0) return;
+    TileStage st{a, ring, n, (int)(threadIdx.x & 63)};
+    const int ntile = (n + NCPA_TILE - 1) / NCPA_TILE;
+    st.load(ntile - 1);
+    st.store(ntile - 1);
+    st.load(ntile - 2);
+    wave_lds_sync();
+    const int rem = n % NCPA_CHUNK;
+    for (int c = ntile - 1; c >= 0; --c) {
+        const double *s = ring + (c & 1) * NCPA_TILE;
+        const int base = c * NCPA_TILE;
+        int top = min(n, base + NCPA_TILE);
+        if (c == ntile - 1 && rem) {
+            for (int i = n - 1; i >= n - rem; --i) step(s[i - base], i);
+            chunk_end(rem);
+            top = n - rem;
         }
-#pragma unroll
-        for (int u = NCPA_CHUNK2 - 1; u >= 0; --u) step(ba[u], bw[u], c * NCPA_CHUNK2 + u);
-        chunk_end(NCPA_CHUNK2);
-#pragma unroll
-        for (int u = 0; u < NCPA_CHUNK2; ++u) {
-            ba[u] = na[u];
-            bw[u] = nw[u];
+        tile_chunks<true>(s, base, top - NCPA_CHUNK, (top - base) / NCPA_CHUNK, step, chunk_end);
+        if (c > 0) {
+            wave_lds_sync();
+            st.store(c - 1);
+            st.load(c - 2);
+            wave_lds_sync();
         }
     }
 }
 
-// Drive `step(a_i, i)` over rows i = 0 .. m-1 (ascending), `chunk_end()` after every
-// 16 rows; same scalar double-buffered streaming as sweep_down.
+// two-array variant (diagonal and absorption weights), same chunking; ring2 holds two
+// rings of NCPA_RING doubles
 template <class Step, class ChunkEnd>
-__device__ __forceinline__ void sweep_up(cdptr a, int m, Step step, ChunkEnd chunk_end) {
+__device__ __forceinline__ void sweep_down2(const double *a, const double *w, double *ring2, int n, Step step,
+                                            ChunkEnd chunk_end) {
+    if (n <= 0) return;
+    const int lane = (int)(threadIdx.x & 63);
+    TileStage sa{a, ring2, n, lane}, sw{w, ring2 + NCPA_RING, n, lane};
+    const int ntile = (n + NCPA_TILE - 1) / NCPA_TILE;
+    sa.load(ntile - 1);
+    sw.load(ntile - 1);
+    sa.store(ntile - 1);
+    sw.store(ntile - 1);
+    sa.load(ntile - 2);
+    sw.load(ntile - 2);
+    wave_lds_sync();
+    const int rem = n % NCPA_CHUNK;
+    for (int c = ntile - 1; c >= 0; --c) {
+        const double *s = ring2 + (c & 1) * NCPA_TILE;
+        const double *t = s + NCPA_RING;
+        const int base = c * NCPA_TILE;
+        int top = min(n, base + NCPA_TILE);
+        if (c == ntile - 1 && rem) {
+            for (int i = n - 1; i >= n - rem; --i) step(s[i - base], t[i - base], i);
+            chunk_end(rem);
+            top = n - rem;
+        }
+        for (int cb = top - NCPA_CHUNK; cb >= base; cb -= NCPA_CHUNK) {
+            double va[NCPA_CHUNK], vw[NCPA_CHUNK];
+            lds_chunk(va, s + (cb - base));
+            lds_chunk(vw, t + (cb - base));
+#pragma unroll
+            for (int u = NCPA_CHUNK - 1; u >= 0; --u) step(va[u], vw[u], cb + u);
+            chunk_end(NCPA_CHUNK);
+        }
+        if (c > 0) {
+            wave_lds_sync();
+            sa.store(c - 1);
+            sw.store(c - 1);
+            sa.load(c - 2);
+            sw.load(c - 2);
+            wave_lds_sync();
+        }
+    }
+}
+
+// Drive `step(a_i, i)` over rows i = 0 .. m-1 (ascending), `chunk_end(16)` after every full
+// 16 rows (the last m % 16 rows get no chunk_end); same staging as sweep_down.
+template <class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_up(const double *a, double *ring, int m, Step step, ChunkEnd chunk_end) {
+    if (m <= 0) return;
+    TileStage st{a, ring, m, (int)(threadIdx.x & 63)};
+    const int ntile = (m + NCPA_TILE - 1) / NCPA_TILE;
+    st.load(0);
+    st.store(0);
+    st.load(1 < ntile ? 1 : -1);
+    wave_lds_sync();
     const int nfull = m / NCPA_CHUNK;
-    if (nfull > 0) {
-        double buf[NCPA_CHUNK];
-#pragma unroll
-        for (int u = 0; u < NCPA_CHUNK; ++u) buf[u] = a[u];
-        for (int c = 0; c < nfull; ++c) {
-            double nxt[NCPA_CHUNK];
-            const int cn = (c + 1 < nfull) ? c + 1 : c;
-#pragma unroll
-            for (int u = 0; u < NCPA_CHUNK; ++u) nxt[u] = a[cn * NCPA_CHUNK + u];
-#pragma unroll
-            for (int u = 0; u < NCPA_CHUNK; ++u) step(buf[u], c * NCPA_CHUNK + u);
-            chunk_end();
-#pragma unroll
-            for (int u = 0; u < NCPA_CHUNK; ++u) buf[u] = nxt[u];
+    for (int c = 0; c < ntile; ++c) {
+        const double *s = ring + (c & 1) * NCPA_TILE;
+        const int base = c * NCPA_TILE;
+        const int endc = max(base, min(nfull * NCPA_CHUNK, base + NCPA_TILE));   // end of full chunks
+        tile_chunks<false>(s, base, base, (endc - base) / NCPA_CHUNK, step, chunk_end);
+        const int endt = min(m, base + NCPA_TILE);
+        for (int i = endc; i < endt; ++i) step(s[i - base], i);
+        if (c + 1 < ntile) {
+            wave_lds_sync();
+            st.store(c + 1);
+            st.load(c + 2 < ntile ? c + 2 : -1);
+            wave_lds_sync();
         }
     }
-    for (int i = nfull * NCPA_CHUNK; i < m; ++i) step(a[i], i);
 }
 
-// count of eigenvalues below s
-__device__ __forceinline__ int sturm_count(cdptr a, int n, double b2, double s) {
+// count of eigenvalues below s (full-wave entry; s may differ per lane)
+__device__ __forceinline__ int sturm_count(const double *a, double *ring, int n, double b2, double s) {
     double p1 = 1.0, p2 = 0.0;
     int cnt = 0;
     unsigned bits = 0u;   // bit 0: sign of the newest continuant; starts with phi_n = +1
     sweep_down(
-        a, n,
+        a, ring, n,
         [&](double ai, int) {
             const double p0 = fma(ai - s, p1, -b2 * p2);
             bits = push_sign(bits, p0);
@@ -175,12 +288,12 @@ struct NewtonOut {
 };
 
 // count plus det and d det/ds (Newton step -p0/q0)
-__device__ __forceinline__ NewtonOut sturm_newton(cdptr a, int n, double b2, double s) {
+__device__ __forceinline__ NewtonOut sturm_newton(const double *a, double *ring, int n, double b2, double s) {
     double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
     int cnt = 0;
     unsigned bits = 0u;
     sweep_down(
-        a, n,
+        a, ring, n,
         [&](double ai, int) {
             const double am = ai - s;
             const double p0 = fma(am, p1, -b2 * p2);
@@ -220,15 +333,15 @@ struct FinalOut {
 };
 
 // backward sweep with absorption derivative and the records phi_{s+1}, phi_{r+1}
-__device__ __forceinline__ FinalOut sturm_final(cdptr a, cdptr w, int n, double b2, double s, int srow,
-                                                int rrow) {
+__device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w, double *ring2, int n, double b2,
+                                                double s, int srow, int rrow) {
     double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0, e1 = 0.0, e2 = 0.0;
     int ex = 0;
     FinalOut o;
     o.phi_s1 = SVal{1.0, 0};
     o.phi_r1 = SVal{1.0, 0};
     sweep_down2(
-        a, w, n,
+        a, w, ring2, n,
         [&](double ai, double wi, int i) {
             const double am = ai - s;
             const double p0 = fma(am, p1, -b2 * p2);
@@ -261,26 +374,28 @@ __device__ __forceinline__ FinalOut sturm_final(cdptr a, cdptr w, int n, double 
 }
 
 // forward continuants theta_{s-1}, theta_{r-1} (theta_{-1} = 1); rows 0 .. max(s,r)-1
-__device__ __forceinline__ void theta_forward(cdptr a, double b2, double s, int srow, int rrow, SVal &th_s,
-                                              SVal &th_r) {
+__device__ __forceinline__ void theta_forward(const double *a, double *ring, double b2, double s, int srow, int rrow,
+                                              SVal &th_s, SVal &th_r) {
     th_s = SVal{1.0, 0};
     th_r = SVal{1.0, 0};
     const int top = srow > rrow ? srow : rrow;
     double t1 = 1.0, t2 = 0.0;
     int ex = 0;
-    for (int k = 0; k < top; ++k) {
-        const double t0 = fma(a[k] - s, t1, -b2 * t2);
-        t2 = t1;
-        t1 = t0;
-        if (k == srow - 1) th_s = SVal{t1, ex};
-        if (k == rrow - 1) th_r = SVal{t1, ex};
-        if ((k & (NCPA_CHUNK - 1)) == NCPA_CHUNK - 1) {
+    sweep_up(
+        a, ring, top,
+        [&](double ak, int k) {
+            const double t0 = fma(ak - s, t1, -b2 * t2);
+            t2 = t1;
+            t1 = t0;
+            if (k == srow - 1) th_s = SVal{t1, ex};
+            if (k == rrow - 1) th_r = SVal{t1, ex};
+        },
+        [&](int) {
             const int kk = pow2_exponent(t1);
             t1 = ldexp(t1, -kk);
             t2 = ldexp(t2, -kk);
             ex += kk;
-        }
-    }
+        });
 }
 
 // u_lo u_hi = -(-1)^(hi-lo) b^(hi-lo) theta_{lo-1} phi_{hi+1} / q_0
