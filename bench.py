@@ -46,11 +46,13 @@ MAXRANGE_M = 1000e3
 FIELD_ROWS, FIELD_STRIDE = 1500, 13        # 1500 depths, 0 - 146 km AGL
 FLOPS_PER_STURM_STEP = 3.0                 # one row of one sweep
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
-PREC_NAMES = {0: "f64", 1: "f32", 2: "bf16x3", 3: "bf16x6"}
-PREC_PEAK = {0: "f64", 1: "f32", 2: "bf16", 3: "bf16"}
-# candidate field precisions, fastest first (measured on MI355X: split-BF16 3 products <
-# split-BF16 6 products ~ FP32 MFMA < FP64 MFMA)
-CANDIDATES = [2, 3, 1]
+PREC_NAMES = {0: "f64", 1: "f32", 2: "bf16x3", 3: "bf16x6", 4: "fp16x3", 5: "fp16x4"}
+PREC_PEAK = {0: "f64", 1: "f32", 2: "bf16", 3: "bf16", 4: "bf16", 5: "bf16"}   # f16 MFMA rate = bf16
+# split-precision kernel (parts, products, FP16) per precision
+PREC_KERNEL = {2: "k_gemm_split<2, 3, false>", 3: "k_gemm_split<3, 6, false>", 4: "k_gemm_split<2, 3, true>",
+               5: "k_gemm_split<2, 4, true>"}
+# candidate field precisions, fastest first (3 products < 4 products < 6 products < FP32 MFMA)
+CANDIDATES = [2, 4, 5, 3, 1]
 TL_GATE_DB = 0.1
 
 BB_NF = 1024
@@ -303,10 +305,10 @@ def field_main(args, cpu):
         avg_s = fams["field2d"][0] / max(fams["field2d"][1], 1) / 1e3
         achieved = res["field_flops"] / avg_s / 1e12
         peak = PEAK_TFLOPS[PREC_PEAK[chosen]]
-        roof = {"kernel": "k_gemm_split<%d> (2-D field contraction, %s)" % (3 if chosen == 3 else 2, PREC_NAMES[chosen])
-                if chosen >= 2 else "k_gemm_x4 (2-D field contraction, %s)" % PREC_NAMES[chosen],
+        kname = PREC_KERNEL.get(chosen, "k_gemm_x4<%s>" % ("double" if chosen == 0 else "float"))
+        roof = {"kernel": "%s (2-D field contraction, %s)" % (kname, PREC_NAMES[chosen]),
                 "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": measured_traffic("k_gemm_split<%d>" % (3 if chosen == 3 else 2)),
+                "frac": achieved / peak, "traffic": measured_traffic(kname),
                 "algorithmic": "4 x depths x modes x ranges real flops per problem (real x complex)"}
     else:
         avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3

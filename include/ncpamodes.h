@@ -188,11 +188,13 @@ int ncpa_tl1d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, double dr, int n_r, int loss
 #define NCPA_PREC_FP32   1   /* FP32 MFMA, complex64 output */
 #define NCPA_PREC_BF16X3 2   /* split-BF16, 2 parts, 3 products (hi*hi + hi*lo + lo*hi), FP32 accumulate */
 #define NCPA_PREC_BF16X6 3   /* split-BF16, 3 parts, 6 products (FP32-class accuracy), FP32 accumulate */
+#define NCPA_PREC_FP16X3 4   /* split-FP16, 2 parts, 3 products (~2^-22), power-of-two row/range scaling, FP32 accumulate */
+#define NCPA_PREC_FP16X4 5   /* split-FP16, 2 parts, 4 products (exact part products), scaled, FP32 accumulate */
 
 /* 2-D field of every problem (ModeSolver::getTLoss2D, ModeSolver.cpp:524-544) at the rows of
  * dev_psi_rows (from ncpa_modes_rows, nrows x total) and ranges r_i = (i+1) dr, i < n_r:
  * dev_out[p][k][2i + {0,1}] = Re, Im of p(r_i, z_k), scaled by 1000 and 4 pi as written by
- * the reference; element type double (FP64) or float (FP32, BF16X3).  lossless != 0 uses
+ * the reference; element type double (FP64) or float (every other precision).  lossless != 0 uses
  * Re(k_pert).  The phase k r is formed in FP64 and reduced mod 2 pi in every precision. */
 int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, const double *dev_psi_rows, int nrows,
                  double dr, int n_r, int lossless, int precision, void *dev_out);

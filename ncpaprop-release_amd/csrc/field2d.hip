@@ -9,15 +9,15 @@
 // so C[k][2i], C[k][2i+1] is the field itself.  The phase k_m r_i is formed in FP64 and
 // reduced mod 2 pi before the sincos; B is then stored in the contraction's precision.
 //
-// Three contraction precisions, same tiling (block 64 x 64 of C, 4 waves of 32 x 32, each a
-// 2 x 2 grid of 16 x 16 MFMA tiles, A and B tiles staged in LDS):
-//   FP64    v_mfma_f64_16x16x4f64, FP64 output
-//   FP32    v_mfma_f32_16x16x4f32, FP32 output
-//   BF16X3  A = Ah + Al, B = Bh + Bl split into bf16 pairs, C = Ah Bh + Ah Bl + Al Bh with
-//           v_mfma_f32_16x16x32_bf16 and FP32 accumulation, FP32 output
+// Contraction precisions:
+//   FP64    v_mfma_f64_16x16x4f64, FP64 output      (block 64 x 64, operands in LDS)
+//   FP32    v_mfma_f32_16x16x4f32, FP32 output      (same tiling)
+//   split   BF16x3, BF16x6, FP16x3, FP16x4: operands split into 16-bit parts, products on
+//           v_mfma_f32_16x16x32_{bf16,f16} with FP32 accumulation, FP32 output (block
+//           128 x 128, tiled operand layout; see "split 16-bit contractions" below)
 // MFMA operand layouts (checked on gfx950 with dense operands): lane l holds A row l%16 /
 // B column l%16 at k-group l/16 (one k per lane for FP64/FP32 16x16x4, 8 consecutive k for
-// bf16 16x16x32).  D lane l holds column l%16; its register v is row 4(l/16) + v for the
+// the 16-bit 16x16x32 forms).  D lane l holds column l%16; its register v is row 4(l/16) + v for the
 // FP32 and BF16 instructions but row (l/16) + 4v for v_mfma_f64_16x16x4f64.
 #include <hip/hip_runtime.h>
 
@@ -35,6 +35,7 @@ namespace ncpa {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 constexpr double F2_TWO_PI_HI = 6.283185307179586;
@@ -57,7 +58,7 @@ __device__ __forceinline__ void f2_csqrt(double x, double y, double &re, double 
     }
 }
 
-enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3 };
+enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3, PREC_FP16X3 = 4, PREC_FP16X4 = 5 };
 
 // B rows for every mode: 1-D grid of total * ceil(nr / (256 * EMAT_R)) blocks; a thread
 // forms EMAT_R consecutive ranges, the phasor exact at the first and advanced by the exact
@@ -66,8 +67,7 @@ enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3 };
 template <int PREC>
 __global__ void __launch_bounds__(256) k_emat(const double *__restrict__ k2, const double *__restrict__ absn,
                                               const double *__restrict__ psis, double dr, int nr, int lossless,
-                                              double ev_re, double ev_im, void *__restrict__ bh,
-                                              void *__restrict__ bl) {
+                                              double ev_re, double ev_im, void *__restrict__ bh) {
     const unsigned nb = (nr + 256 * EMAT_R - 1) / (256 * EMAT_R);
     const size_t o = blockIdx.x / nb;
     const int i0 = ((blockIdx.x % nb) * blockDim.x + threadIdx.x) * EMAT_R;
@@ -108,34 +108,18 @@ __global__ void __launch_bounds__(256) k_emat(const double *__restrict__ k2, con
             double *b = (double *)bh;
             b[idx] = vr;
             b[idx + 1] = vi;
-        } else if (PREC == PREC_FP32) {
+        } else {
             float *b = (float *)bh;
             b[idx] = (float)vr;
             b[idx + 1] = (float)vi;
-        } else {
-            __bf16 *h = (__bf16 *)bh, *l = (__bf16 *)bl;
-            const __bf16 hr = (__bf16)(float)vr, hi = (__bf16)(float)vi;
-            h[idx] = hr;
-            h[idx + 1] = hi;
-            l[idx] = (__bf16)(float)(vr - (double)(float)hr);
-            l[idx + 1] = (__bf16)(float)(vi - (double)(float)hi);
         }
     }
 }
 
-// A (rows x total) from the FP64 eigenvector rows, in the contraction precision
-template <int PREC>
-__global__ void k_amat(const double *__restrict__ psi, size_t n, void *__restrict__ ah, void *__restrict__ al) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double v = psi[i];
-        if (PREC == PREC_FP32) {
-            ((float *)ah)[i] = (float)v;
-        } else {
-            const __bf16 h = (__bf16)(float)v;
-            ((__bf16 *)ah)[i] = h;
-            ((__bf16 *)al)[i] = (__bf16)(float)(v - (double)(float)h);
-        }
-    }
+// A (rows x total) from the FP64 eigenvector rows, in FP32
+__global__ void k_amat(const double *__restrict__ psi, size_t n, float *__restrict__ ah) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        ah[i] = (float)psi[i];
 }
 
 struct GemmProb {
@@ -202,78 +186,18 @@ __global__ void __launch_bounds__(256) k_gemm_x4(const GemmProb *__restrict__ gp
             }
 }
 
-// split-BF16 (3 products) MFMA GEMM, FP32 accumulate and output
-__global__ void __launch_bounds__(256) k_gemm_bf16x3(const GemmProb *__restrict__ gp, const __bf16 *__restrict__ Ah,
-                                                     const __bf16 *__restrict__ Al, size_t lda,
-                                                     const __bf16 *__restrict__ Bh, const __bf16 *__restrict__ Bl,
-                                                     size_t ldb, float *__restrict__ C, size_t ldc, int M, int N) {
-    constexpr int KS = 32;
-    // k contiguous for both operands so each lane reads its 8 k-values as one 16-byte vector
-    __shared__ __attribute__((aligned(16))) __bf16 Ash[BM][KS + 8], Asl[BM][KS + 8];
-    __shared__ __attribute__((aligned(16))) __bf16 Bsh[BN][KS + 8], Bsl[BN][KS + 8];
-    const GemmProb P = gp[blockIdx.z];
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    f4 acc[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = f4{0, 0, 0, 0};
-    const __bf16 zero = (__bf16)0.0f;
-    for (int k0 = 0; k0 < P.K; k0 += KS) {
-        for (int e = tid; e < BM * KS; e += 256) {
-            const int r = e / KS, c = e % KS;
-            const int gr = m0 + r, gc = k0 + c;
-            const bool ok = gr < M && gc < P.K;
-            const size_t gi = (size_t)gr * lda + P.a_off + gc;
-            Ash[r][c] = ok ? Ah[gi] : zero;
-            Asl[r][c] = ok ? Al[gi] : zero;
-        }
-        for (int e = tid; e < KS * BN; e += 256) {
-            const int r = e / BN, c = e % BN;     // r: k, c: column
-            const int gr = k0 + r, gc = n0 + c;
-            const bool ok = gr < P.K && gc < N;
-            const size_t gi = (P.b_off + gr) * ldb + gc;
-            Bsh[c][r] = ok ? Bh[gi] : zero;
-            Bsl[c][r] = ok ? Bl[gi] : zero;
-        }
-        __syncthreads();
-        const int kq = 8 * (lane >> 4);
-        bf8 ah[2], al[2], bh[2], bl[2];
-        for (int a = 0; a < 2; ++a) {
-            const int r = wm + a * 16 + (lane & 15);
-            ah[a] = *(const bf8 *)&Ash[r][kq];
-            al[a] = *(const bf8 *)&Asl[r][kq];
-        }
-        for (int b = 0; b < 2; ++b) {
-            const int c = wn + b * 16 + (lane & 15);
-            bh[b] = *(const bf8 *)&Bsh[c][kq];
-            bl[b] = *(const bf8 *)&Bsl[c][kq];
-        }
-        for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b) {
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-            }
-        __syncthreads();
-    }
-    float *Cp = C + P.c_off;
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int v = 0; v < 4; ++v) {
-                const int r = m0 + wm + a * 16 + 4 * (lane >> 4) + v;
-                const int c = n0 + wn + b * 16 + (lane & 15);
-                if (r < M && c < N) Cp[(size_t)r * ldc + c] = acc[a][b][v];
-            }
-}
-
-
 // ------------------------------------------------------------------------------------
-// split-BF16 contraction, tuned path
+// split 16-bit contractions
 // ------------------------------------------------------------------------------------
-// Operands are split v = v0 + v1 (+ v2) into bf16 parts (S = 2 or 3).  The products kept are
-// those with part-index sum < S: S = 2 -> v0w0 + v0w1 + v1w0 (3 MFMAs, ~16-bit products),
-// S = 3 -> adds v0w2 + v1w1 + v2w0 (6 MFMAs, ~24-bit, FP32-class).  Accumulation is FP32.
+// Operands are split v = v0 + v1 (+ v2) into 16-bit parts (S = 2 or 3), BF16 or FP16.
+//   BF16, S = 2: v0w0 + v0w1 + v1w0 (3 MFMAs, ~2^-16 relative)
+//   BF16, S = 3: adds v0w2 + v1w1 + v2w0 (6 MFMAs, ~2^-24, FP32-class)
+//   FP16, S = 2: v0w0 + v0w1 + v1w0 (3 MFMAs, ~2^-22) or with v1w1 (4 MFMAs: the products
+//                of the parts are exact, only the FP32 accumulation rounds).  FP16 has a
+//                narrow range, so A rows and B columns are scaled by powers of two first
+//                (largest |value| of the row / range to [2^14, 2^15)) and the scales are
+//                undone on the FP32 output: C[z][r] = acc 2^-(SA[z] + SB[r]).
+// Accumulation is FP32 in every case.
 //
 // Layouts (per problem p, written by k_asplit / k_esplit, zero padded), tiled so that one
 // k-step of one 128-row tile is a single contiguous 8 KB chunk per part:
@@ -288,74 +212,172 @@ struct SplitProb {
     int K, Kp;
 };
 
-constexpr int TBM = 128, TBN = 128, TKS = 32, TLD = TKS + 8;   // LDS row: 40 bf16 = 80 B
+constexpr int TBM = 128, TBN = 128, TKS = 32, TLD = TKS + 8;   // LDS row: 40 halves = 80 B
 
 // element offset of (row, k) in the tiled [row/128][k/32][row%128][k%32] layout
 __host__ __device__ __forceinline__ size_t tiled_index(int row, int k, int Kp) {
     return (((size_t)(row >> 7) * (Kp / TKS) + (k >> 5)) * TBM + (row & 127)) * TKS + (k & 31);
 }
 
-__device__ __forceinline__ __bf16 bf_round(double v) { return (__bf16)(float)v; }
+// One 16-bit part of a split: the part's bits and its value.  BF16 (8-bit significand,
+// FP32 range) or FP16 (11-bit significand; operands are pre-scaled by powers of two so
+// the largest magnitudes sit at 2^14).
+template <bool F16>
+__device__ __forceinline__ unsigned short split_part(double v, double &back) {
+    if (F16) {
+        const _Float16 h = (_Float16)v;
+        back = (double)h;
+        return __builtin_bit_cast(unsigned short, h);
+    }
+    const __bf16 h = (__bf16)(float)v;
+    back = (double)(float)h;
+    return __builtin_bit_cast(unsigned short, h);
+}
 
-// A split: one thread per (row, k) of the padded block of problem p (grid.y = p)
-template <int S>
+#define SPLIT_TOP 14   // scaled operands: max |value| in [2^14, 2^15), FP16 max 65504
+
+// exponent e with max * 2^e in [2^14, 2^15) (0 for an all-zero set)
+__device__ __forceinline__ int split_scale(double mx) {
+    return (mx > 0.0 && isfinite(mx)) ? SPLIT_TOP - ilogb(mx) : 0;
+}
+
+// FP16 only: per (problem, row) scale of A from the largest |psi| of the row.  One wave
+// per row; grid (ceil(Mp / 4), P), block 256.
+__global__ void __launch_bounds__(256) k_rowscale(const double *__restrict__ psi, size_t ldpsi,
+                                                  const SplitProb *__restrict__ sp, const int *__restrict__ col_off,
+                                                  int M, int Mp, int *__restrict__ SA) {
+    const int p = blockIdx.y;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= Mp) return;
+    const SplitProb P = sp[p];
+    double mx = 0.0;
+    if (row < M)
+        for (int k = lane; k < P.K; k += 64) mx = fmax(mx, fabs(psi[(size_t)row * ldpsi + col_off[p] + k]));
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) mx = fmax(mx, __shfl_xor(mx, d));
+    if (lane == 0) SA[(size_t)p * Mp + row] = split_scale(mx);
+}
+
+// A split: one thread per (row, k) of the padded block of problem p (grid.y = p); with
+// F16 every row is first scaled by 2^SA[row].
+template <int S, bool F16>
 __global__ void k_asplit(const double *__restrict__ psi, size_t ldpsi, const SplitProb *__restrict__ sp,
                          const int *__restrict__ col_off, int M, int Mp, size_t split_stride,
-                         __bf16 *__restrict__ Ahat) {
+                         const int *__restrict__ SA, unsigned short *__restrict__ Ahat) {
     const int p = blockIdx.y;
     const SplitProb P = sp[p];
     const size_t n = (size_t)Mp * P.Kp;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         const int row = (int)(e / P.Kp), k = (int)(e % P.Kp);
         double v = (row < M && k < P.K) ? psi[(size_t)row * ldpsi + col_off[p] + k] : 0.0;
+        if (F16) v = ldexp(v, SA[(size_t)p * Mp + row]);
         for (int s = 0; s < S; ++s) {
-            const __bf16 h = bf_round(v);
-            Ahat[s * split_stride + P.a_off + tiled_index(row, k, P.Kp)] = h;
-            v -= (double)(float)h;
+            double back;
+            Ahat[s * split_stride + P.a_off + tiled_index(row, k, P.Kp)] = split_part<F16>(v, back);
+            v -= back;
         }
     }
 }
 
+// Per mode: k_pert = kr + i ki and g = ev psi(zs) / sqrt(k_pert), the mode's factor in
+// B[m, r] = g e^{i k r} 1000 / sqrt(r).
+__device__ __forceinline__ void mode_factor(double k2, double absn, double psis, int lossless, double ev_re,
+                                            double ev_im, double &kr, double &ki, double &gr, double &gi) {
+    f2_csqrt(k2, absn, kr, ki);
+    double cr, ci;
+    if (lossless) {
+        ki = 0.0;
+        cr = psis / sqrt(kr);
+        ci = 0.0;
+    } else {
+        double sr, si;
+        f2_csqrt(kr, ki, sr, si);
+        const double den = sr * sr + si * si;
+        cr = psis * sr / den;
+        ci = -psis * si / den;
+    }
+    gr = cr * ev_re - ci * ev_im;
+    gi = cr * ev_im + ci * ev_re;
+}
+
+// FP16 only: per mode log2 |g| and ki (the envelope |B[m, r]| = |g| e^{-ki r} 1000/sqrt(r))
+__global__ void __launch_bounds__(256) k_mode_env(const double *__restrict__ k2, const double *__restrict__ absn,
+                                                  const double *__restrict__ psis, size_t n, int lossless,
+                                                  double ev_re, double ev_im, double *__restrict__ lg,
+                                                  double *__restrict__ kim) {
+    const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    double kr, ki, gr, gi;
+    mode_factor(k2[o], absn[o], psis[o], lossless, ev_re, ev_im, kr, ki, gr, gi);
+    const double a = hypot(gr, gi);
+    lg[o] = a > 0.0 ? log2(a) : -INFINITY;
+    kim[o] = ki;
+}
+
+// FP16 only: per (problem, range) scale of B from the envelope max_m |B[m, r]|;
+// grid (ceil(nr / 256), P)
+__global__ void __launch_bounds__(256) k_colscale(const SplitProb *__restrict__ sp, const int *__restrict__ mode_off,
+                                                  const double *__restrict__ lg, const double *__restrict__ kim,
+                                                  double dr, int nr, int nr_pad, int *__restrict__ SB) {
+    __shared__ double s_lg[256], s_ki[256];
+    const int p = blockIdx.y;
+    const SplitProb P = sp[p];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const double r = (i + 1) * dr;
+    double best = -INFINITY;
+    for (int m0 = 0; m0 < P.K; m0 += 256) {
+        const int m = m0 + threadIdx.x;
+        if (m < P.K) {
+            s_lg[threadIdx.x] = lg[mode_off[p] + m];
+            s_ki[threadIdx.x] = kim[mode_off[p] + m];
+        }
+        __syncthreads();
+        const int cnt = min(256, P.K - m0);
+        for (int q = 0; q < cnt; ++q) best = fmax(best, fma(-s_ki[q], r * 1.4426950408889634, s_lg[q]));
+        __syncthreads();
+    }
+    if (i >= nr_pad) return;
+    // log2 max|B| = best + log2(1000 / sqrt(r)); the recurrence may exceed the envelope by
+    // a few ulp, far inside the factor 2 of headroom below 65504
+    const int e = (i < nr && best > -INFINITY) ? SPLIT_TOP - (int)floor(best + log2(1000.0 / sqrt(r))) : 0;
+    SB[(size_t)p * nr_pad + i] = e;
+}
+
 // E split: block = 32 modes x 64 ranges of one problem; phasors formed in FP64, split to
-// bf16 parts and written transposed ([column][mode]) through LDS.
+// 16-bit parts and written transposed ([column][mode]) through LDS.
 struct ETile {
     int p, k0;
 };
 
-template <int S>
+template <int S, bool F16>
 __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles, const SplitProb *__restrict__ sp,
                                                 const int *__restrict__ mode_off, const double *__restrict__ k2,
                                                 const double *__restrict__ absn, const double *__restrict__ psis,
                                                 double dr, int nr, int lossless, double ev_re, double ev_im,
-                                                size_t split_stride, __bf16 *__restrict__ Bhat) {
-    __shared__ __attribute__((aligned(16))) __bf16 T[S][128][TKS + 8];
+                                                size_t split_stride, const int *__restrict__ SB, int nr_pad,
+                                                unsigned short *__restrict__ Bhat) {
+    __shared__ __attribute__((aligned(16))) unsigned short T[S][128][TKS + 8];
+    __shared__ double s_sc[64];
     const ETile t = tiles[blockIdx.y];
     const SplitProb P = sp[t.p];
     const int i0 = blockIdx.x * 64;
     const int mloc = threadIdx.x & 31;       // mode within the tile
     const int k = t.k0 + mloc;               // mode within the problem
-    double kr = 0, ki = 0, cr = 0, ci = 0;
+    double kr = 0, ki = 0, gr = 0, gi = 0;
     const bool live = k < P.K;
     if (live) {
         const size_t o = (size_t)mode_off[t.p] + k;
-        f2_csqrt(k2[o], absn[o], kr, ki);
-        if (lossless) {
-            ki = 0.0;
-            cr = psis[o] / sqrt(kr);
-        } else {
-            double sr, si;
-            f2_csqrt(kr, ki, sr, si);
-            const double den = sr * sr + si * si;
-            cr = psis[o] * sr / den;
-            ci = -psis[o] * si / den;
-        }
+        mode_factor(k2[o], absn[o], psis[o], lossless, ev_re, ev_im, kr, ki, gr, gi);
     }
-    // 1000 / sqrt(r) of the tile's 64 ranges
-    __shared__ double s_sc[64];
-    if (threadIdx.x < 64) s_sc[threadIdx.x] = 1000.0 / sqrt((i0 + (int)threadIdx.x + 1) * dr);
-    // g = ev c; the thread's ranges are i0 + w + 8q: e^{i k r} exact at q = 0, then advanced
-    // by the exact factor e^{i k 8 dr} (a complex multiply instead of a sincos and an exp)
-    const double gr = cr * ev_re - ci * ev_im, gi = cr * ev_im + ci * ev_re;
+    // 1000 / sqrt(r) of the tile's 64 ranges (times 2^SB with F16)
+    if (threadIdx.x < 64) {
+        const int i = i0 + (int)threadIdx.x;
+        double sc = 1000.0 / sqrt((i + 1) * dr);
+        if (F16 && i < nr_pad) sc = ldexp(sc, SB[(size_t)t.p * nr_pad + i]);
+        s_sc[threadIdx.x] = sc;
+    }
+    // the thread's ranges are i0 + w + 8q: e^{i k r} exact at q = 0, then advanced by the
+    // exact factor e^{i k 8 dr} (a complex multiply instead of a sincos and an exp)
     double zr = 0.0, zi = 0.0, mr = 0.0, mi = 0.0;
     if (live) {
         const double r0 = (i0 + (threadIdx.x >> 5) + 1) * dr;
@@ -385,11 +407,11 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
             zr = nz;
         }
         for (int s = 0; s < S; ++s) {
-            const __bf16 hr = bf_round(vr), hi = bf_round(vi);
-            T[s][2 * il][mloc] = hr;
-            T[s][2 * il + 1][mloc] = hi;
-            vr -= (double)(float)hr;
-            vi -= (double)(float)hi;
+            double br, bi;
+            T[s][2 * il][mloc] = split_part<F16>(vr, br);
+            T[s][2 * il + 1][mloc] = split_part<F16>(vi, bi);
+            vr -= br;
+            vi -= bi;
         }
     }
     __syncthreads();
@@ -403,13 +425,25 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
         }
 }
 
-template <int S>
-__global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restrict__ sp, const __bf16 *__restrict__ Ahat,
-                                                       const __bf16 *__restrict__ Bhat, size_t split_stride_a,
-                                                       size_t split_stride_b, float *__restrict__ C, size_t ldc,
-                                                       int M, int N, int n_tiles_n) {
-    __shared__ __attribute__((aligned(16))) __bf16 As[S][TBM][TLD];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[S][TBN][TLD];
+template <bool F16>
+__device__ __forceinline__ f4 mfma16(u4 a, u4 b, f4 c) {
+    if (F16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a), __builtin_bit_cast(h8, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
+
+// S parts per operand, NP products (smallest first): S = 2 -> 3 (hi*hi + hi*lo + lo*hi) or
+// 4 (+ lo*lo); S = 3 -> 6 (all products with i + j <= 2).  With F16, C = acc 2^-(SA + SB).
+template <int S, int NP, bool F16>
+__global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restrict__ sp,
+                                                       const unsigned short *__restrict__ Ahat,
+                                                       const unsigned short *__restrict__ Bhat,
+                                                       size_t split_stride_a, size_t split_stride_b,
+                                                       float *__restrict__ C, size_t ldc, int M, int N,
+                                                       int n_tiles_n, const int *__restrict__ SA, int Mp,
+                                                       const int *__restrict__ SB, int nr_pad) {
+    __shared__ __attribute__((aligned(16))) unsigned short As[S][TBM][TLD];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[S][TBN][TLD];
     const SplitProb P = sp[blockIdx.z];
     // Blocks are dealt to the 8 XCDs round-robin (id % 8).  All M tiles of one N tile are
     // kept on one XCD, one after another, so its Bhat tile is fetched into one L2 only.
@@ -427,7 +461,7 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
-    const __bf16 *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
+    const unsigned short *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
     const int Kp = P.Kp;
     // each thread moves 2 x 16 B of A and 2 x 16 B of B per split and k-step:
     // tile rows 128 x 4 vectors = 512 vectors per operand and split.  Native vector type:
@@ -463,13 +497,13 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
         __syncthreads();
         if (k0 + TKS < Kp) fetch(k0 + TKS);
         const int kq = 8 * (lane >> 4);
-        bf8 af[S][4], bfr[S][4];
+        u4 af[S][4], bfr[S][4];
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-                af[s][a] = *(const bf8 *)&As[s][wm + a * 16 + (lane & 15)][kq];
-                bfr[s][a] = *(const bf8 *)&Bs[s][wn + a * 16 + (lane & 15)][kq];
+                af[s][a] = *(const u4 *)&As[s][wm + a * 16 + (lane & 15)][kq];
+                bfr[s][a] = *(const u4 *)&Bs[s][wn + a * 16 + (lane & 15)][kq];
             }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -477,26 +511,33 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
             for (int b = 0; b < 4; ++b) {
                 // smallest products first
                 if constexpr (S == 3) {
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2][a], bfr[0][b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][a], bfr[1][b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[S - 1][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = mfma16<F16>(af[2][a], bfr[0][b], acc[a][b]);
+                    acc[a][b] = mfma16<F16>(af[1][a], bfr[1][b], acc[a][b]);
+                    acc[a][b] = mfma16<F16>(af[0][a], bfr[2][b], acc[a][b]);
                 }
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][a], bfr[0][b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[1][b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][a], bfr[0][b], acc[a][b], 0, 0, 0);
+                if constexpr (S == 2 && NP == 4) acc[a][b] = mfma16<F16>(af[1][a], bfr[1][b], acc[a][b]);
+                acc[a][b] = mfma16<F16>(af[1][a], bfr[0][b], acc[a][b]);
+                acc[a][b] = mfma16<F16>(af[0][a], bfr[1][b], acc[a][b]);
+                acc[a][b] = mfma16<F16>(af[0][a], bfr[0][b], acc[a][b]);
             }
     }
     float *Cp = C + P.c_off;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < 4; ++b) {
+            const int c = n0 + wn + b * 16 + (lane & 15);
+            const int sb = (F16 && c < N) ? SB[(size_t)blockIdx.z * nr_pad + (c >> 1)] : 0;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int r = m0 + wm + a * 16 + 4 * (lane >> 4) + v;
-                const int c = n0 + wn + b * 16 + (lane & 15);
-                if (r < M && c < N) Cp[(size_t)r * ldc + c] = acc[a][b][v];
+                if (r < M && c < N) {
+                    float x = acc[a][b][v];
+                    if (F16) x = ldexpf(x, -(SA[(size_t)blockIdx.z * Mp + r] + sb));
+                    Cp[(size_t)r * ldc + c] = x;
+                }
             }
+        }
 }
 
 }  // namespace ncpa
@@ -511,12 +552,13 @@ int ncpa_modes_internal(ncpa_modes_t *m, int *P, int *total, const int **h_offse
 
 namespace {
 
-template <int S>
+template <int S, int NP, bool F16>
 int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *hcnt, const int *d_off,
                   const double *d_k2, const double *d_abs, const double *d_psis, const double *psi, int M,
                   double dr, int n_r, int lossless, float *out) {
     const int N = 2 * n_r;
     const int Mp = (M + TBM - 1) / TBM * TBM, Np = (N + TBN - 1) / TBN * TBN;
+    const int nr_pad = Np / 2;
     std::vector<SplitProb> sp(P);
     std::vector<ETile> tiles;
     size_t sa = 0, sb = 0;
@@ -529,15 +571,23 @@ int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *h
     }
     SplitProb *d_sp = pool_alloc<SplitProb>(ctx, P);
     ETile *d_tiles = pool_alloc<ETile>(ctx, std::max<size_t>(tiles.size(), 1));
-    __bf16 *Ahat = pool_alloc<__bf16>(ctx, std::max<size_t>(S * sa, 8));
-    __bf16 *Bhat = pool_alloc<__bf16>(ctx, std::max<size_t>(S * sb, 8));
+    unsigned short *Ahat = pool_alloc<unsigned short>(ctx, std::max<size_t>(S * sa, 8));
+    unsigned short *Bhat = pool_alloc<unsigned short>(ctx, std::max<size_t>(S * sb, 8));
+    int *SA = F16 ? pool_alloc<int>(ctx, (size_t)P * Mp) : nullptr;
+    int *SB = F16 ? pool_alloc<int>(ctx, (size_t)P * nr_pad) : nullptr;
+    double *LG = F16 ? pool_alloc<double>(ctx, std::max(total, 1)) : nullptr;
+    double *KI = F16 ? pool_alloc<double>(ctx, std::max(total, 1)) : nullptr;
     auto cleanup = [&]() {
         ctx->pool.put(d_sp);
         ctx->pool.put(d_tiles);
         ctx->pool.put(Ahat);
         ctx->pool.put(Bhat);
+        ctx->pool.put(SA);
+        ctx->pool.put(SB);
+        ctx->pool.put(LG);
+        ctx->pool.put(KI);
     };
-    if (!d_sp || !d_tiles || !Ahat || !Bhat) {
+    if (!d_sp || !d_tiles || !Ahat || !Bhat || (F16 && (!SA || !SB || !LG || !KI))) {
         cleanup();
         return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_field2d: allocation failed");
     }
@@ -552,18 +602,26 @@ int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *h
     const std::complex<double> ev = 4 * PI * I * std::exp(-I * PI * 0.25) / std::sqrt(8.0 * PI);
     if (!tiles.empty()) {
         TimedLaunch tl(ctx, "emat");
-        hipLaunchKernelGGL(k_asplit<S>, dim3(64, P), dim3(256), 0, ctx->stream, psi, (size_t)total, d_sp, d_off, M, Mp,
-                           sa, Ahat);
-        hipLaunchKernelGGL(k_esplit<S>, dim3((n_r + 63) / 64, (unsigned)tiles.size()), dim3(256), 0, ctx->stream,
-                           d_tiles, d_sp, d_off, d_k2, d_abs, d_psis, dr, n_r, lossless, ev.real(), ev.imag(), sb,
-                           Bhat);
+        if (F16) {
+            hipLaunchKernelGGL(k_rowscale, dim3((Mp + 3) / 4, P), dim3(256), 0, ctx->stream, psi, (size_t)total, d_sp,
+                               d_off, M, Mp, SA);
+            hipLaunchKernelGGL(k_mode_env, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis,
+                               (size_t)total, lossless, ev.real(), ev.imag(), LG, KI);
+            hipLaunchKernelGGL(k_colscale, dim3((nr_pad + 255) / 256, P), dim3(256), 0, ctx->stream, d_sp, d_off, LG,
+                               KI, dr, n_r, nr_pad, SB);
+        }
+        hipLaunchKernelGGL((k_asplit<S, F16>), dim3(64, P), dim3(256), 0, ctx->stream, psi, (size_t)total, d_sp,
+                           d_off, M, Mp, sa, SA, Ahat);
+        hipLaunchKernelGGL((k_esplit<S, F16>), dim3((n_r + 63) / 64, (unsigned)tiles.size()), dim3(256), 0,
+                           ctx->stream, d_tiles, d_sp, d_off, d_k2, d_abs, d_psis, dr, n_r, lossless, ev.real(),
+                           ev.imag(), sb, SB, nr_pad, Bhat);
     }
     {
         TimedLaunch tl(ctx, "field2d");
         const int tn = Np / TBN, tm = Mp / TBM;
         dim3 g((unsigned)(tm * ((tn + 7) & ~7)), 1, (unsigned)P);
-        hipLaunchKernelGGL(k_gemm_split<S>, g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb, out, (size_t)N, M,
-                           N, tn);
+        hipLaunchKernelGGL((k_gemm_split<S, NP, F16>), g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb, out,
+                           (size_t)N, M, N, tn, SA, Mp, SB, nr_pad);
     }
     e = hipGetLastError();
     cleanup();
@@ -578,7 +636,7 @@ extern "C" {
 int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, int nrows, double dr, int n_r,
                  int lossless, int precision, void *dev_out) {
     if (!ctx || !m || !dev_psi_rows || nrows <= 0 || n_r <= 0 || !(dr > 0.0) || !dev_out ||
-        precision < 0 || precision > 3)
+        precision < 0 || precision > PREC_FP16X4)
         return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d: bad arguments");
     int P = 0, total = 0;
     const int *hoff = nullptr, *hcnt = nullptr;
@@ -591,28 +649,30 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, i
         const int *d_off = nullptr;
         ncpa_modes_device(m, &d_off, nullptr, nullptr, nullptr);
         if (precision == PREC_BF16X3)
-            return field2d_split<2>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, nrows, dr,
-                                    n_r, lossless, (float *)dev_out);
-        return field2d_split<3>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, nrows, dr, n_r,
-                                lossless, (float *)dev_out);
+            return field2d_split<2, 3, false>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows,
+                                              nrows, dr, n_r, lossless, (float *)dev_out);
+        if (precision == PREC_BF16X6)
+            return field2d_split<3, 6, false>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows,
+                                              nrows, dr, n_r, lossless, (float *)dev_out);
+        if (precision == PREC_FP16X3)
+            return field2d_split<2, 3, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows,
+                                             nrows, dr, n_r, lossless, (float *)dev_out);
+        return field2d_split<2, 4, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, nrows,
+                                         dr, n_r, lossless, (float *)dev_out);
     }
     const size_t ldb = 2 * (size_t)n_r, ldc = ldb, lda = (size_t)std::max(total, 1);
     std::vector<GemmProb> gp(P);
     for (int p = 0; p < P; ++p) gp[p] = {(size_t)hoff[p], (size_t)hoff[p], (size_t)p * nrows * ldc, hcnt[p]};
     GemmProb *d_gp = pool_alloc<GemmProb>(ctx, P);
-    const size_t esz = precision == PREC_FP64 ? 8 : (precision == PREC_FP32 ? 4 : 2);
+    const size_t esz = precision == PREC_FP64 ? 8 : 4;
     void *bh = ctx->pool.get(std::max<size_t>(total, 1) * ldb * esz);
-    void *bl = precision == PREC_BF16X3 ? ctx->pool.get(std::max<size_t>(total, 1) * ldb * esz) : nullptr;
-    void *ah = precision == PREC_FP64 ? nullptr : ctx->pool.get((size_t)nrows * lda * esz);
-    void *al = precision == PREC_BF16X3 ? ctx->pool.get((size_t)nrows * lda * esz) : nullptr;
+    float *ah = precision == PREC_FP64 ? nullptr : pool_alloc<float>(ctx, (size_t)nrows * lda);
     auto cleanup = [&]() {
         ctx->pool.put(d_gp);
         ctx->pool.put(bh);
-        ctx->pool.put(bl);
         ctx->pool.put(ah);
-        ctx->pool.put(al);
     };
-    if (!d_gp || !bh || (precision == PREC_BF16X3 && (!bl || !al)) || (precision != PREC_FP64 && !ah)) {
+    if (!d_gp || !bh || (precision != PREC_FP64 && !ah)) {
         cleanup();
         return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_field2d: allocation failed");
     }
@@ -628,18 +688,12 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, i
         dim3 ge((unsigned)(((n_r + 256 * EMAT_R - 1) / (256 * EMAT_R)) * (size_t)total));
         if (precision == PREC_FP64)
             hipLaunchKernelGGL(k_emat<PREC_FP64>, ge, dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis, dr, n_r,
-                               lossless, ev.real(), ev.imag(), bh, bl);
-        else if (precision == PREC_FP32)
-            hipLaunchKernelGGL(k_emat<PREC_FP32>, ge, dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis, dr, n_r,
-                               lossless, ev.real(), ev.imag(), bh, bl);
+                               lossless, ev.real(), ev.imag(), bh);
         else
-            hipLaunchKernelGGL(k_emat<PREC_BF16X3>, ge, dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis, dr, n_r,
-                               lossless, ev.real(), ev.imag(), bh, bl);
-        const size_t na = (size_t)nrows * lda;
+            hipLaunchKernelGGL(k_emat<PREC_FP32>, ge, dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis, dr, n_r,
+                               lossless, ev.real(), ev.imag(), bh);
         if (precision == PREC_FP32)
-            hipLaunchKernelGGL(k_amat<PREC_FP32>, dim3(1024), dim3(256), 0, ctx->stream, dev_psi_rows, na, ah, al);
-        else if (precision == PREC_BF16X3)
-            hipLaunchKernelGGL(k_amat<PREC_BF16X3>, dim3(1024), dim3(256), 0, ctx->stream, dev_psi_rows, na, ah, al);
+            hipLaunchKernelGGL(k_amat, dim3(1024), dim3(256), 0, ctx->stream, dev_psi_rows, (size_t)nrows * lda, ah);
     }
     {
         TimedLaunch tl(ctx, "field2d");
@@ -647,13 +701,9 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, i
         if (precision == PREC_FP64)
             hipLaunchKernelGGL((k_gemm_x4<double, 16>), gg, dim3(256), 0, ctx->stream, d_gp, dev_psi_rows, lda,
                                (const double *)bh, ldb, (double *)dev_out, ldc, nrows, (int)ldb);
-        else if (precision == PREC_FP32)
+        else
             hipLaunchKernelGGL((k_gemm_x4<float, 16>), gg, dim3(256), 0, ctx->stream, d_gp, (const float *)ah, lda,
                                (const float *)bh, ldb, (float *)dev_out, ldc, nrows, (int)ldb);
-        else
-            hipLaunchKernelGGL(k_gemm_bf16x3, gg, dim3(256), 0, ctx->stream, d_gp, (const __bf16 *)ah,
-                               (const __bf16 *)al, lda, (const __bf16 *)bh, (const __bf16 *)bl, ldb,
-                               (float *)dev_out, ldc, nrows, (int)ldb);
     }
     e = hipGetLastError();
     cleanup();

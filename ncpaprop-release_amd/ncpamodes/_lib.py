@@ -21,6 +21,8 @@ NCPA_PREC_FP64 = 0
 NCPA_PREC_FP32 = 1
 NCPA_PREC_BF16X3 = 2
 NCPA_PREC_BF16X6 = 3
+NCPA_PREC_FP16X3 = 4
+NCPA_PREC_FP16X4 = 5
 
 NCPA_F_WKB = 1
 NCPA_F_WVNUM_FILTER = 2
