@@ -361,6 +361,8 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
                                               double *__restrict__ psisr_out, int *__restrict__ iters_out) {
     __shared__ double ms_s[64];
     __shared__ int ms_c[64];
+    __shared__ double hb_lo[64], hb_hi[64];   // brackets of the unconverged eigenvalues
+    __shared__ int h_list[64];                // their lanes, in lane order
     __shared__ double s_ring[2 * NCPA_RING];
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob < 0) return;
@@ -400,40 +402,76 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     bool done = !active;
     int my_iters = NCPA_MS_PASSES;
     bool collapsed = false;
+    // phases 1-2.  Lanes whose eigenvalue has converged (and lanes past the problem's last
+    // index) do not idle: each iteration they are dealt round-robin to the unconverged
+    // eigenvalues of the block and evaluate extra shifts spread over those brackets; every
+    // unconverged lane then tightens its bracket from all 64 (shift, count) pairs, so the
+    // stragglers that set the block's duration converge several bits per sweep.
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
     for (int it = 0; it < NCPA_MAXIT; ++it) {
-        if (__all(done)) break;
+        const unsigned long long um = __ballot(!done);
+        if (um == 0ull) break;
+        const int nu = __popcll(um);
+        if (!done) {
+            hb_lo[lane] = lo;
+            hb_hi[lane] = hi;
+            h_list[__popcll(um & below)] = lane;
+        }
+        __syncthreads();
+        double sh = sig;
+        if (done) {
+            const int h = __popcll(~um & below);        // rank among the helpers
+            const int idx = h % nu, kk = h / nu;
+            const int nh = (64 - nu - idx + nu - 1) / nu;  // helpers dealt to eigenvalue idx
+            const int u = h_list[idx];
+            const double a = hb_lo[u], b = hb_hi[u];
+            sh = a + (double)(kk + 1) * ((b - a) / (double)(nh + 1));
+        }
         const bool newton = !done && (ch - cl == 1);
+        int cnt;
+        double p0 = 0.0, q0 = 0.0;
         if (__any(newton)) {
-            const NewtonOut o = sturm_newton(P.a, s_ring, P.n, P.b2, sig);
-            if (!done) {
-                const double s0 = sig;
-                if (o.cnt <= j) { lo = sig; cl = o.cnt; } else { hi = sig; ch = o.cnt; }
-                sig = 0.5 * (lo + hi);
-                if (newton && o.q0 != 0.0 && isfinite(o.p0) && isfinite(o.q0)) {
-                    const double dlt = -o.p0 / o.q0;
-                    const double cand = s0 + dlt;
-                    if (fabs(dlt) <= P.tol) {
-                        sig = cand;
-                        done = true;
-                        my_iters = NCPA_MS_PASSES + it + 1;
-                    } else if (cand > lo && cand < hi) {
-                        sig = cand;
-                    }
+            const NewtonOut o = sturm_newton(P.a, s_ring, P.n, P.b2, sh);
+            cnt = o.cnt;
+            p0 = o.p0;
+            q0 = o.q0;
+        } else {
+            cnt = sturm_count(P.a, s_ring, P.n, P.b2, sh);
+        }
+        ms_s[lane] = sh;
+        ms_c[lane] = cnt;
+        __syncthreads();
+        if (!done) {
+            for (int l = 0; l < 64; ++l) {           // counts are non-decreasing in the shift
+                const int cc = ms_c[l];
+                const double ss = ms_s[l];
+                if (cc <= j) {
+                    if (ss > lo) { lo = ss; cl = cc; }
+                } else if (ss < hi) {
+                    hi = ss;
+                    ch = cc;
                 }
             }
-        } else {
-            const int cnt = sturm_count(P.a, s_ring, P.n, P.b2, sig);
-            if (!done) {
-                if (cnt <= j) { lo = sig; cl = cnt; } else { hi = sig; ch = cnt; }
+            sig = 0.5 * (lo + hi);
+            if (newton && q0 != 0.0 && isfinite(p0) && isfinite(q0)) {
+                const double dlt = -p0 / q0;
+                const double cand = sh + dlt;
+                if (fabs(dlt) <= P.tol) {
+                    sig = cand;
+                    done = true;
+                    my_iters = NCPA_MS_PASSES + it + 1;
+                } else if (cand > lo && cand < hi) {
+                    sig = cand;
+                }
+            }
+            if (!done && (hi - lo) <= P.tol) {
                 sig = 0.5 * (lo + hi);
+                done = true;
+                collapsed = (ch - cl) != 1;
+                my_iters = -(NCPA_MS_PASSES + it + 1);
             }
         }
-        if (!done && (hi - lo) <= P.tol) {
-            sig = 0.5 * (lo + hi);
-            done = true;
-            collapsed = (ch - cl) != 1;
-            my_iters = -(NCPA_MS_PASSES + it + 1);
-        }
+        __syncthreads();
     }
     if (!done) my_iters = -NCPA_MAXIT - 1;
     // phase 3: sensitivities at the converged eigenvalue (every lane, one sweep)
@@ -478,61 +516,66 @@ struct RowsArgs {
     size_t ld;      // scratch / output leading dimension (total modes)
 };
 
+// The sweeps keep their row loops free of branches: every row's continuant goes to a
+// 16-row LDS buffer (one column per lane) and the recorded rows of a chunk are copied out
+// once the chunk is done, before its rescale.
 __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
                                              const double *__restrict__ k2, RowsArgs ra,
                                              double *__restrict__ TH, int *__restrict__ THE,
                                              double *__restrict__ PH, int *__restrict__ PHE,
                                              double *__restrict__ Q0, int *__restrict__ QEX) {
+    __shared__ double s_ring[NCPA_RING];
+    __shared__ double rec[NCPA_CHUNK][64];
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
-    const int j = wk.j0 + (int)threadIdx.x;
+    const int lane = (int)threadIdx.x;
+    const int j = wk.j0 + lane;
     const bool active = j < P.n_hi;
     const size_t o = (size_t)P.out_off + (size_t)(active ? (j - P.n_lo) : 0);
     const double s = active ? k2[o] : 0.5 * (P.lo + P.hi);
     const int n = P.n;
     const double b2 = P.b2;
     const double *a = P.a;
-    __shared__ double s_ring[NCPA_RING];
+    const size_t src = (size_t)ra.nrec * ra.ld + o;   // slot of the source row
     const int last_rec = ra.row0 + (ra.nrec - 1) * ra.stride;
-    // forward: theta_k for k = 0 .. n-2, stored at row k+1
+    // forward: theta_k for k = 0 .. n-2, recorded at row k+1
     {
         double t1 = 1.0, t2 = 0.0;
         int ex = 0;
-        int slot = 0, next = ra.row0;
+        int slot = 0, next = ra.row0;   // next recorded row and its slot
         if (next == 0) {
             if (active) { TH[o] = 1.0; THE[o] = 0; }
             slot = 1;
             next = ra.nrec > 1 ? ra.row0 + ra.stride : INT_MAX;
         }
-        if (P.srow == 0 && active) { TH[(size_t)ra.nrec * ra.ld + o] = 1.0; THE[(size_t)ra.nrec * ra.ld + o] = 0; }
+        if (P.srow == 0 && active) { TH[src] = 1.0; THE[src] = 0; }
         sweep_up(
             a, s_ring, n - 1,
             [&](double ak, int k) {
                 const double t0 = fma(ak - s, t1, -b2 * t2);
                 t2 = t1;
                 t1 = t0;
-                if (k + 1 == next) {
+                rec[k & (NCPA_CHUNK - 1)][lane] = t1;
+            },
+            [&](int lo, int m) {
+                // rows lo+1 .. lo+m now hold theta_{row-1}
+                for (; next <= lo + m; ++slot, next = slot < ra.nrec ? next + ra.stride : INT_MAX)
                     if (active) {
-                        TH[(size_t)slot * ra.ld + o] = t1;
+                        TH[(size_t)slot * ra.ld + o] = rec[(next - 1) & (NCPA_CHUNK - 1)][lane];
                         THE[(size_t)slot * ra.ld + o] = ex;
                     }
-                    ++slot;
-                    next = slot < ra.nrec ? next + ra.stride : INT_MAX;
+                if (P.srow >= lo + 1 && P.srow <= lo + m && active) {
+                    TH[src] = rec[(P.srow - 1) & (NCPA_CHUNK - 1)][lane];
+                    THE[src] = ex;
                 }
-                if (k + 1 == P.srow && active) {
-                    TH[(size_t)ra.nrec * ra.ld + o] = t1;
-                    THE[(size_t)ra.nrec * ra.ld + o] = ex;
-                }
-            },
-            [&](int) {
                 const int kk = pow2_exponent(t1);
                 t1 = ldexp(t1, -kk);
                 t2 = ldexp(t2, -kk);
                 ex += kk;
             });
     }
-    // backward: phi_i for i = n-1 .. 0, stored at row i-1; q for the normalisation
+    // backward: phi_i for i = n-1 .. 0, recorded at row i-1; q for the normalisation
     {
         double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
         int ex = 0;
@@ -542,7 +585,7 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
             --slot;
             next = slot >= 0 ? next - ra.stride : INT_MIN;
         }
-        if (P.srow == n - 1 && active) { PH[(size_t)ra.nrec * ra.ld + o] = 1.0; PHE[(size_t)ra.nrec * ra.ld + o] = 0; }
+        if (P.srow == n - 1 && active) { PH[src] = 1.0; PHE[src] = 0; }
         sweep_down(
             a, s_ring, n,
             [&](double ai, int i) {
@@ -553,20 +596,19 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
                 p1 = p0;
                 q2 = q1;
                 q1 = q0;
-                if (i - 1 == next) {
+                rec[i & (NCPA_CHUNK - 1)][lane] = p1;
+            },
+            [&](int lo, int m) {
+                // rows lo-1 .. lo+m-2 now hold phi_{row+1}
+                for (; next >= lo - 1; --slot, next = slot >= 0 ? next - ra.stride : INT_MIN)
                     if (active) {
-                        PH[(size_t)slot * ra.ld + o] = p1;
+                        PH[(size_t)slot * ra.ld + o] = rec[(next + 1) & (NCPA_CHUNK - 1)][lane];
                         PHE[(size_t)slot * ra.ld + o] = ex;
                     }
-                    --slot;
-                    next = slot >= 0 ? next - ra.stride : INT_MIN;
+                if (P.srow >= lo - 1 && P.srow <= lo + m - 2 && active) {
+                    PH[src] = rec[(P.srow + 1) & (NCPA_CHUNK - 1)][lane];
+                    PHE[src] = ex;
                 }
-                if (i - 1 == P.srow && active) {
-                    PH[(size_t)ra.nrec * ra.ld + o] = p1;
-                    PHE[(size_t)ra.nrec * ra.ld + o] = ex;
-                }
-            },
-            [&](int) {
                 const int k = pow2_exponent(p1);
                 p1 = ldexp(p1, -k);
                 p2 = ldexp(p2, -k);

@@ -137,25 +137,25 @@ __device__ __forceinline__ void tile_chunks(const double *s, int base, int cb0, 
         lds_chunk(vb, s + (c1 - base));
         __builtin_amdgcn_sched_barrier(0);
         step_chunk<Down>(va, c0, step);
-        chunk_end(NCPA_CHUNK);
+        chunk_end(c0, NCPA_CHUNK);
         __builtin_amdgcn_sched_barrier(0);
         wait_lds();
         const int c2 = (k + 2 < nch) ? c1 + dir : c1;
         lds_chunk(va, s + (c2 - base));
         __builtin_amdgcn_sched_barrier(0);
         step_chunk<Down>(vb, c1, step);
-        chunk_end(NCPA_CHUNK);
+        chunk_end(c1, NCPA_CHUNK);
         __builtin_amdgcn_sched_barrier(0);
         wait_lds();
     }
     if (k < nch) {
         step_chunk<Down>(va, cb0 + k * dir, step);
-        chunk_end(NCPA_CHUNK);
+        chunk_end(cb0 + k * dir, NCPA_CHUNK);
     }
 }
 
-// Drive `step(a_i, i)` over rows i = n-1 .. 0, then `chunk_end(rows_in_chunk)` after each
-// chunk of rows: first the n % 16 top rows, then chunks [16c, 16c+15] downwards.
+// Drive `step(a_i, i)` over rows i = n-1 .. 0, then `chunk_end(lo, m)` after each chunk of
+// rows [lo, lo + m): first the n % 16 top rows, then chunks [16c, 16c+15] downwards.
 template <class Step, class ChunkEnd>
 __device__ __forceinline__ void sweep_down(const double *a, double *ring, int n, Step step, ChunkEnd chunk_end) {
     if (n <= 0) return;
@@ -172,7 +172,7 @@ __device__ __forceinline__ void sweep_down(const double *a, double *ring, int n,
         int top = min(n, base + NCPA_TILE);
         if (c == ntile - 1 && rem) {
             for (int i = n - 1; i >= n - rem; --i) step(s[i - base], i);
-            chunk_end(rem);
+            chunk_end(n - rem, rem);
             top = n - rem;
         }
         tile_chunks<true>(s, base, top - NCPA_CHUNK, (top - base) / NCPA_CHUNK, step, chunk_end);
@@ -186,10 +186,12 @@ __device__ __forceinline__ void sweep_down(const double *a, double *ring, int n,
 }
 
 // two-array variant (diagonal and absorption weights), same chunking; ring2 holds two
-// rings of NCPA_RING doubles
-template <class Step, class ChunkEnd>
+// rings of NCPA_RING doubles.  Chunks for which `rec(lo)` holds are stepped with
+// `step_rec` (a step that may record values at given rows), all others with `step`: the
+// per-row row tests stay out of the chunks that need none.
+template <class Step, class StepRec, class RecPred, class ChunkEnd>
 __device__ __forceinline__ void sweep_down2(const double *a, const double *w, double *ring2, int n, Step step,
-                                            ChunkEnd chunk_end) {
+                                            StepRec step_rec, RecPred rec, ChunkEnd chunk_end) {
     if (n <= 0) return;
     const int lane = (int)(threadIdx.x & 63);
     TileStage sa{a, ring2, n, lane}, sw{w, ring2 + NCPA_RING, n, lane};
@@ -208,17 +210,22 @@ __device__ __forceinline__ void sweep_down2(const double *a, const double *w, do
         const int base = c * NCPA_TILE;
         int top = min(n, base + NCPA_TILE);
         if (c == ntile - 1 && rem) {
-            for (int i = n - 1; i >= n - rem; --i) step(s[i - base], t[i - base], i);
-            chunk_end(rem);
+            for (int i = n - 1; i >= n - rem; --i) step_rec(s[i - base], t[i - base], i);
+            chunk_end(n - rem, rem);
             top = n - rem;
         }
         for (int cb = top - NCPA_CHUNK; cb >= base; cb -= NCPA_CHUNK) {
             double va[NCPA_CHUNK], vw[NCPA_CHUNK];
             lds_chunk(va, s + (cb - base));
             lds_chunk(vw, t + (cb - base));
+            if (rec(cb)) {
 #pragma unroll
-            for (int u = NCPA_CHUNK - 1; u >= 0; --u) step(va[u], vw[u], cb + u);
-            chunk_end(NCPA_CHUNK);
+                for (int u = NCPA_CHUNK - 1; u >= 0; --u) step_rec(va[u], vw[u], cb + u);
+            } else {
+#pragma unroll
+                for (int u = NCPA_CHUNK - 1; u >= 0; --u) step(va[u], vw[u], cb + u);
+            }
+            chunk_end(cb, NCPA_CHUNK);
         }
         if (c > 0) {
             wave_lds_sync();
@@ -231,8 +238,8 @@ __device__ __forceinline__ void sweep_down2(const double *a, const double *w, do
     }
 }
 
-// Drive `step(a_i, i)` over rows i = 0 .. m-1 (ascending), `chunk_end(16)` after every full
-// 16 rows (the last m % 16 rows get no chunk_end); same staging as sweep_down.
+// Drive `step(a_i, i)` over rows i = 0 .. m-1 (ascending), `chunk_end(lo, m)` after every
+// chunk [lo, lo + m) of 16 rows and after the last m % 16 rows; same staging as sweep_down.
 template <class Step, class ChunkEnd>
 __device__ __forceinline__ void sweep_up(const double *a, double *ring, int m, Step step, ChunkEnd chunk_end) {
     if (m <= 0) return;
@@ -250,6 +257,7 @@ __device__ __forceinline__ void sweep_up(const double *a, double *ring, int m, S
         tile_chunks<false>(s, base, base, (endc - base) / NCPA_CHUNK, step, chunk_end);
         const int endt = min(m, base + NCPA_TILE);
         for (int i = endc; i < endt; ++i) step(s[i - base], i);
+        if (endt > endc) chunk_end(endc, endt - endc);
         if (c + 1 < ntile) {
             wave_lds_sync();
             st.store(c + 1);
@@ -272,7 +280,7 @@ __device__ __forceinline__ int sturm_count(const double *a, double *ring, int n,
             p2 = p1;
             p1 = p0;
         },
-        [&](int m) {
+        [&](int, int m) {
             cnt += sign_changes(bits, m);
             bits &= 1u;
             const int k = pow2_exponent(p1);
@@ -304,7 +312,7 @@ __device__ __forceinline__ NewtonOut sturm_newton(const double *a, double *ring,
             q2 = q1;
             q1 = q0;
         },
-        [&](int m) {
+        [&](int, int m) {
             cnt += sign_changes(bits, m);
             bits &= 1u;
             const int k = pow2_exponent(p1);
@@ -340,23 +348,29 @@ __device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w
     FinalOut o;
     o.phi_s1 = SVal{1.0, 0};
     o.phi_r1 = SVal{1.0, 0};
+    auto step = [&](double ai, double wi, int) {
+        const double am = ai - s;
+        const double p0 = fma(am, p1, -b2 * p2);
+        const double q0 = fma(am, q1, fma(-b2, q2, -p1));
+        const double e0 = fma(am, e1, fma(-b2, e2, wi * p1));
+        p2 = p1;
+        p1 = p0;
+        q2 = q1;
+        q1 = q0;
+        e2 = e1;
+        e1 = e0;
+    };
     sweep_down2(
-        a, w, ring2, n,
+        a, w, ring2, n, step,
         [&](double ai, double wi, int i) {
-            const double am = ai - s;
-            const double p0 = fma(am, p1, -b2 * p2);
-            const double q0 = fma(am, q1, fma(-b2, q2, -p1));
-            const double e0 = fma(am, e1, fma(-b2, e2, wi * p1));
-            p2 = p1;
-            p1 = p0;
-            q2 = q1;
-            q1 = q0;
-            e2 = e1;
-            e1 = e0;
+            step(ai, wi, i);
             if (i == srow + 1) o.phi_s1 = SVal{p1, ex};
             if (i == rrow + 1) o.phi_r1 = SVal{p1, ex};
         },
-        [&](int) {
+        [&](int lo) {
+            return (unsigned)(srow + 1 - lo) < (unsigned)NCPA_CHUNK || (unsigned)(rrow + 1 - lo) < (unsigned)NCPA_CHUNK;
+        },
+        [&](int, int) {
             const int k = pow2_exponent(p1);
             p1 = ldexp(p1, -k);
             p2 = ldexp(p2, -k);
@@ -390,7 +404,7 @@ __device__ __forceinline__ void theta_forward(const double *a, double *ring, dou
             if (k == srow - 1) th_s = SVal{t1, ex};
             if (k == rrow - 1) th_r = SVal{t1, ex};
         },
-        [&](int) {
+        [&](int, int) {
             const int kk = pow2_exponent(t1);
             t1 = ldexp(t1, -kk);
             t2 = ldexp(t2, -kk);
