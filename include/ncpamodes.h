@@ -257,6 +257,56 @@ int ncpa_bb_source(ncpa_ctx_t *ctx, int kind, const double *f_vec, int n_freq, i
                    double f_center, const double *input, int n_input, double *spectrum,
                    double *pulse);
 
+/* ------------------------------------------------------------------------------------------
+ * Broadband job plans.  The batched form of ModBB --dispersion followed by --propagation
+ * (ModBB_main.cpp:33-121: one ESSModeSolver::solve per frequency, then
+ * ModalBroadbandPropagator::calculate_waveform, ModalBroadbandPropagator.cpp:140-243, over
+ * the dispersion file).  Problem p = s (n_freq - 1) + i - 1 is set s (e.g. an azimuth) at
+ * f_vec[i]; every eigenvalue of every problem is found in one persistent launch and the
+ * waveforms are synthesised without a count on the host, so the whole job is captured once
+ * as a HIP graph and replayed by ncpa_bb_plan_run.  Creation runs the batch's interval
+ * counts once to size the mode buffers and, with tune != 0, times the eigenvalue kernel's
+ * variants (multisection passes 1-3 x persistent grid of 4, 8, 15 blocks per CU) on the
+ * batch and keeps the fastest.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct ncpa_bb_plan ncpa_bb_plan_t;
+
+typedef struct {
+    int n_sets;               /* dispersion sets (e.g. azimuths) */
+    int n_freq;               /* frequency bins including DC: f_vec[0] = 0 */
+    const double *f_vec;      /* host [n_freq] */
+    const double *spectrum;   /* host complex S[n_freq], interleaved (ncpa_bb_source) */
+    const double *ranges_m;   /* host [n_ranges], > 0 */
+    int n_ranges;
+    const double *rho_ratio;  /* host [n_sets]: rho_r / rho_s */
+    double max_celerity;      /* t0 = range / max_celerity */
+    int nfft;                 /* >= n_freq (the reference's rule: max(nfft, 4 n_freq)) */
+    int zero_attn;            /* != 0: Im k = 0 in the transfer functions */
+    int max_modes_total;      /* mode capacity; 0: 1.5x the modes the batch has at creation */
+    int tune;                 /* != 0: autotune the eigenvalue kernel on this batch */
+} ncpa_bb_job_t;
+
+int ncpa_bb_plan_create(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems,
+                        const ncpa_bb_job_t *job, ncpa_bb_plan_t **plan);
+/* New problem records (same count, grid and job) for the next runs, e.g. other profiles;
+ * a batch with more modes than the capacity makes ncpa_bb_plan_result fail. */
+int ncpa_bb_plan_set_problems(ncpa_bb_plan_t *plan, const ncpa_problem_t *problems);
+/* Enqueues one replay of the job's graph on the context stream (returns at once). */
+int ncpa_bb_plan_run(ncpa_bb_plan_t *plan);
+/* Waits for the last run and returns its device results (owned by the plan):
+ * dev_wave [n_sets][n_ranges][nfft] = 2 Re(df FFT(e^{-2 pi i f t0} S H)) as ncpa_bb_pulse;
+ * k_pert (dev_kre, dev_kim), psi_S psi_R (dev_psi_sr) and mode offsets (dev_offsets,
+ * [n_problems + 1]) of every mode, problem-major, ascending k.  Any pointer may be NULL.
+ * NCPA_ERR_MAX_MODES if a problem or the batch exceeded its limit (nothing computed). */
+int ncpa_bb_plan_result(ncpa_bb_plan_t *plan, const double **dev_wave, const double **dev_kre,
+                        const double **dev_kim, const double **dev_psi_sr, const int **dev_offsets,
+                        int *total_modes);
+/* The configuration in use and the tuner's measurements: log holds (passes, blocks, ms)
+ * triples; *n_log in: capacity in triples, out: triples measured. */
+int ncpa_bb_plan_tuning(ncpa_bb_plan_t *plan, int *ms_passes, int *grid_blocks, double *modes_ms,
+                        double *log, int *n_log);
+void ncpa_bb_plan_destroy(ncpa_bb_plan_t *plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,7 @@
 
 #include "common.h"
 #include "ctx.h"
+#include "internal.h"
 
 namespace ncpa {
 
@@ -53,10 +54,11 @@ __device__ __forceinline__ cplx expi2pi(double x) {
     return make_double2(c, s);
 }
 
-__global__ void k_kpert(int n, const double *__restrict__ k2, const double *__restrict__ absorption,
-                        double *__restrict__ kre, double *__restrict__ kim) {
+// n_dev (optional): the count written on the device; n is then the buffer capacity
+__global__ void k_kpert(int n, const int *__restrict__ n_dev, const double *__restrict__ k2,
+                        const double *__restrict__ absorption, double *__restrict__ kre, double *__restrict__ kim) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (n_dev && i >= *n_dev)) return;
     double re, im;
     bb_csqrt(k2[i], absorption[i], re, im);
     kre[i] = re;
@@ -316,7 +318,13 @@ int make_plan(ncpa_ctx *ctx, int n, FftPlan &p) {
     return NCPA_OK;
 }
 
-int launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, int sign) {
+}  // namespace
+
+namespace ncpa {
+// Sets the FFT kernel up for n-point transforms (its LDS attribute when 2 n complex doubles
+// need more than 64 KB) and reports the global scratch it needs: 0 when the ping-pong
+// buffers fit in LDS, else 4 n batch doubles.
+int bb_prepare_fft(ncpa_ctx *ctx, int n, int batch, size_t *scratch_doubles) {
     FftPlan plan;
     int rc = make_plan(ctx, n, plan);
     if (rc) return rc;
@@ -324,22 +332,68 @@ int launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, i
     NCPA_HIP(ctx, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     const size_t lds = 2 * (size_t)n * sizeof(cplx);
     const bool use_lds = lds <= (size_t)max_lds;
-    double *scratch = nullptr;
-    if (!use_lds) {
-        scratch = pool_alloc<double>(ctx, 4 * (size_t)n * batch);
-        if (!scratch) return set_error(ctx, NCPA_ERR_ALLOC, "fft: scratch allocation failed");
-    } else if (lds > 65536) {
+    if (use_lds && lds > 65536)
         NCPA_HIP(ctx, hipFuncSetAttribute((const void *)k_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
+    *scratch_doubles = use_lds ? 0 : 4 * (size_t)n * batch;
+    return NCPA_OK;
+}
+
+// batch unnormalised n-point FFTs with scratch from bb_prepare_fft (no allocation)
+int bb_launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, int sign, double *scratch) {
+    FftPlan plan;
+    int rc = make_plan(ctx, n, plan);
+    if (rc) return rc;
+    const size_t lds = 2 * (size_t)n * sizeof(cplx);
+    const bool use_lds = scratch == nullptr;
     {
         TimedLaunch tl(ctx, "fft");
         hipLaunchKernelGGL(k_fft, dim3(batch), dim3(256), use_lds ? lds : 0, ctx->stream, plan, in, out, scratch,
                            use_lds ? 1 : 0, (double)(sign < 0 ? -1.0 : 1.0));
     }
     hipError_t e = hipGetLastError();
-    ctx->pool.put(scratch);
     if (e != hipSuccess) return hip_error(ctx, e, "k_fft launch");
     return NCPA_OK;
+}
+
+void bb_launch_kpert(hipStream_t st, int n_cap, const int *n_dev, const double *k2, const double *absn, double *kre,
+                     double *kim) {
+    hipLaunchKernelGGL(k_kpert, dim3((n_cap + 255) / 256), dim3(256), 0, st, n_cap, n_dev, k2, absn, kre, kim);
+}
+
+void bb_launch_transfer(hipStream_t st, int n_bins, int n_freq, const int *beg, const int *cnt, const double *kre,
+                        const double *kim, const double *psr, int zero_attn, const double *ranges, int n_ranges,
+                        const double *fac, double *H) {
+    const int smooth = (int)std::floor(0.1 * n_freq);   // ModalBroadbandPropagator.cpp:272
+    hipLaunchKernelGGL(k_bb_transfer, dim3((unsigned)n_bins), dim3(256), 0, st, n_freq, beg, cnt, kre, kim, psr,
+                       zero_attn ? 1 : 0, ranges, n_ranges, fac, smooth, H);
+}
+
+void bb_launch_arg(hipStream_t st, int n_tr, int n_freq, int nfft, const double *H, const double *S, const double *f,
+                   const double *t0, int n_ranges, double *arg) {
+    const size_t n = (size_t)n_tr * nfft;
+    hipLaunchKernelGGL(k_bb_arg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n_tr, n_freq, nfft, H, S, f, t0,
+                       n_ranges, arg);
+}
+
+void bb_launch_out(hipStream_t st, size_t n, const double *spec, double df, double *wave) {
+    hipLaunchKernelGGL(k_bb_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, spec, df, wave);
+}
+}  // namespace ncpa
+
+namespace {
+
+int launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, int sign) {
+    size_t need = 0;
+    int rc = bb_prepare_fft(ctx, n, batch, &need);
+    if (rc) return rc;
+    double *scratch = nullptr;
+    if (need) {
+        scratch = pool_alloc<double>(ctx, need);
+        if (!scratch) return set_error(ctx, NCPA_ERR_ALLOC, "fft: scratch allocation failed");
+    }
+    rc = bb_launch_fft(ctx, in, out, n, batch, sign, scratch);
+    ctx->pool.put(scratch);
+    return rc;
 }
 
 template <class T>
@@ -409,8 +463,8 @@ int ncpa_kpert(ncpa_ctx_t *ctx, const double *dev_k2, const double *dev_absorpti
     if (n < 0 || (n > 0 && (!dev_k2 || !dev_absorption || !dev_kre || !dev_kim)))
         return set_error(ctx, NCPA_ERR_ARG, "ncpa_kpert: bad arguments");
     if (n == 0) return NCPA_OK;
-    hipLaunchKernelGGL(k_kpert, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dev_k2, dev_absorption, dev_kre,
-                       dev_kim);
+    hipLaunchKernelGGL(k_kpert, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, nullptr, dev_k2, dev_absorption,
+                       dev_kre, dev_kim);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(ctx, e, "k_kpert launch");
     return NCPA_OK;

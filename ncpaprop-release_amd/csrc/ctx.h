@@ -47,6 +47,7 @@ struct ncpa_ctx {
     std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> events;
     std::vector<ncpa::ProfileDev> profiles;
     ncpa::DevicePool pool;
+    int ms_passes = 2;     // multisection passes of the eigenvalue kernel (autotuned by plans)
 };
 
 namespace ncpa {

@@ -27,32 +27,9 @@
 #include "common.h"
 #include "ctx.h"
 #include "sturm.h"
+#include "internal.h"
 
 namespace ncpa {
-
-// ------------------------------------------------------------------------------------
-// device-side problem record (built on the host per batch)
-// ------------------------------------------------------------------------------------
-struct ProbDev {
-    const double *prof;   // packed profile nodes
-    int n_nodes;
-    int flags;
-    double omega;
-    double az_rad;
-    double alpha_freq;
-    double admittance;
-    double zs, zr;
-    double c_min, c_max;
-    int user_alpha;
-    int srow, rrow;
-};
-
-struct IntervalOut {
-    double k_min, k_max, lo, hi;
-    int n_lo, n_hi;
-    int used_wkb;
-    int pad;
-};
 
 // ------------------------------------------------------------------------------------
 // Sutherland-Bass attenuation, z [km], T [K], P [Pa], D [kg/m3] (AtmosphericModel.cpp:86-219)
@@ -325,29 +302,8 @@ __global__ void k_counts2(const double *__restrict__ a, int n, double b2, double
     if (threadIdx.x < 2) out[threadIdx.x] = c;
 }
 
-struct ModeProb {
-    const double *a;
-    const double *w;
-    double b2;
-    double log2b;
-    double lo, hi;
-    double tol;       // absolute convergence tolerance on lambda
-    double inv_dz;
-    int n;
-    int srow, rrow;
-    int n_lo, n_hi;
-    int out_off;
-};
-
-struct ModeWork {
-    int prob;   // -1: padding block (no work)
-    int j0;
-};
-
-#define NCPA_XCDS 8   // MI355X accelerator complex dies (workgroups dispatched round-robin)
 
 #define NCPA_MAXIT 200
-#define NCPA_MS_PASSES 2
 
 // One lane per eigenvalue index j (ascending), 64 consecutive indices of one problem per
 // block.  Phase 0 narrows every bracket by multisection (see below); phase 1 bisects with count-only sweeps until [lo, hi) holds only eigenvalue j;
@@ -355,18 +311,19 @@ struct ModeWork {
 // the working-precision tolerance; phase 3 is one sweep at the converged eigenvalue that
 // yields the absorption and the adjugate eigenvector products.  iters_out: sweeps used,
 // negative when the bracket collapsed before Newton converged (cluster at working precision).
-__global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
-                                              double *__restrict__ k2_out, double *__restrict__ abs_out,
-                                              double *__restrict__ psis_out, double *__restrict__ psir_out,
-                                              double *__restrict__ psisr_out, int *__restrict__ iters_out) {
-    __shared__ double ms_s[64];
-    __shared__ int ms_c[64];
-    __shared__ double hb_lo[64], hb_hi[64];   // brackets of the unconverged eigenvalues
-    __shared__ int h_list[64];                // their lanes, in lane order
-    __shared__ double s_ring[2 * NCPA_RING];
-    const ModeWork wk = work[blockIdx.x];
-    if (wk.prob < 0) return;
-    const ModeProb P = mp[wk.prob];
+struct ModesShared {
+    double ms_s[64];
+    int ms_c[64];
+    double hb_lo[64], hb_hi[64];   // brackets of the unconverged eigenvalues
+    int h_list[64];                // their lanes, in lane order
+    double ring[2 * NCPA_RING];
+};
+
+template <int MS>
+__device__ __forceinline__ void modes_item(const ModeProb &P, const ModeWork wk, ModesShared &sh_,
+                                           const ModesOut &out) {
+    double *ms_s = sh_.ms_s, *hb_lo = sh_.hb_lo, *hb_hi = sh_.hb_hi, *s_ring = sh_.ring;
+    int *ms_c = sh_.ms_c, *h_list = sh_.h_list;
     const int lane = (int)threadIdx.x;
     const int j = wk.j0 + lane;
     const bool active = j < P.n_hi;
@@ -378,7 +335,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     // work that bisection spends ~log2(n) sweeps on per lane.
     const int last = min(P.n_hi - 1, wk.j0 + 63) - wk.j0;   // last active lane
     double blo = lo, bhi = hi;
-    for (int pass = 0; pass < NCPA_MS_PASSES; ++pass) {
+    for (int pass = 0; pass < MS; ++pass) {
         const double sh = blo + (double)(lane + 1) * ((bhi - blo) * (1.0 / 65.0));
         const int c = sturm_count(P.a, s_ring, P.n, P.b2, sh);
         ms_s[lane] = sh;
@@ -400,7 +357,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     }
     double sig = 0.5 * (lo + hi);
     bool done = !active;
-    int my_iters = NCPA_MS_PASSES;
+    int my_iters = MS;
     bool collapsed = false;
     // phases 1-2.  Lanes whose eigenvalue has converged (and lanes past the problem's last
     // index) do not idle: each iteration they are dealt round-robin to the unconverged
@@ -459,7 +416,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
                 if (fabs(dlt) <= P.tol) {
                     sig = cand;
                     done = true;
-                    my_iters = NCPA_MS_PASSES + it + 1;
+                    my_iters = MS + it + 1;
                 } else if (cand > lo && cand < hi) {
                     sig = cand;
                 }
@@ -468,7 +425,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
                 sig = 0.5 * (lo + hi);
                 done = true;
                 collapsed = (ch - cl) != 1;
-                my_iters = -(NCPA_MS_PASSES + it + 1);
+                my_iters = -(MS + it + 1);
             }
         }
         __syncthreads();
@@ -478,7 +435,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     const FinalOut f = sturm_final(P.a, P.w, s_ring, P.n, P.b2, sig, P.srow, P.rrow);
     SVal th_s, th_r;
     theta_forward(P.a, s_ring, P.b2, sig, P.srow, P.rrow, th_s, th_r);
-    if (!active) return;
+    if (!active) return;   // (whole-wave work above is done; nothing of this lane is left)
     const double mult = collapsed ? (double)(ch - cl) : 1.0;
     const double us2 = adjugate_product(th_s, f.phi_s1, f.q0, f.ex, 0, P.log2b) / mult;
     const double ur2 = adjugate_product(th_r, f.phi_r1, f.q0, f.ex, 0, P.log2b) / mult;
@@ -488,14 +445,156 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     else
         usr = adjugate_product(th_r, f.phi_s1, f.q0, f.ex, P.srow - P.rrow, P.log2b) / mult;
     const int o = P.out_off + (j - P.n_lo);
-    k2_out[o] = sig;
-    abs_out[o] = (f.q0 != 0.0) ? -f.e0 / f.q0 : 0.0;
+    out.k2[o] = sig;
+    out.absn[o] = (f.q0 != 0.0) ? -f.e0 / f.q0 : 0.0;
     const double ps = sqrt(fmax(us2, 0.0) * P.inv_dz);
     const double pr = sqrt(fmax(ur2, 0.0) * P.inv_dz);
-    psis_out[o] = ps;
-    psir_out[o] = (usr < 0.0) ? -pr : pr;
-    psisr_out[o] = usr * P.inv_dz;
-    iters_out[o] = my_iters;
+    out.psis[o] = ps;
+    out.psir[o] = (usr < 0.0) ? -pr : pr;
+    out.psisr[o] = usr * P.inv_dz;
+    out.iters[o] = my_iters;
+}
+
+// Two launch forms.  Eager: one block per work item (grid = n_work).  Persistent (queue
+// given): a fixed grid whose blocks pull items from per-XCD counters -- block b, running
+// on XCD x = b % 8, takes items 8t + x for t = queue[x]++ -- until the list written on the
+// device by k_plan_build (n_work_dev) is exhausted, so no count is needed on the host.
+template <int MS>
+__global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
+                                              int n_work, const int *__restrict__ n_work_dev, int *queue,
+                                              ModesOut out) {
+    __shared__ ModesShared sh;
+    if (!queue) {
+        if (blockIdx.x >= (unsigned)n_work) return;
+        const ModeWork wk = work[blockIdx.x];
+        if (wk.prob >= 0) modes_item<MS>(mp[wk.prob], wk, sh, out);
+        return;
+    }
+    const int nw = *n_work_dev;
+    const int x = blockIdx.x & (NCPA_XCDS - 1);
+    for (;;) {
+        int t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(&queue[x], 1);
+        t = __shfl(t, 0);
+        const int w = NCPA_XCDS * t + x;
+        if (w >= nw) break;
+        const ModeWork wk = work[w];
+        if (wk.prob >= 0) modes_item<MS>(mp[wk.prob], wk, sh, out);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// device-side work list (the host copy of it is what launch_modes builds)
+// ------------------------------------------------------------------------------------
+// One block of 1024 threads.  Thread t owns the contiguous problem chunk [t c, t c + c);
+// block-wide exclusive scans give the mode offsets and, per XCD class x = p % 8, the
+// position of each problem's 64-mode groups in that XCD's part of the interleaved list.
+// Also writes ModeProb records, the broadband index arrays (set s, bin i: problem
+// s (n_freq - 1) + i - 1, bin 0 is DC with no modes) and resets the persistent queue.
+__global__ void __launch_bounds__(1024) k_plan_build(const ProbDev *__restrict__ probs,
+                                                     const IntervalOut *__restrict__ iv, int P, int Nz, double dz,
+                                                     const double *__restrict__ A, const double *__restrict__ W,
+                                                     int cap_modes, int cap_work, int n_sets, int n_freq,
+                                                     ModeProb *__restrict__ mp, int *__restrict__ offsets,
+                                                     ModeWork *__restrict__ work, int *__restrict__ mbeg,
+                                                     int *__restrict__ mcnt, PlanMeta *__restrict__ meta) {
+    __shared__ int s_sum[1024];
+    __shared__ int s_grp[NCPA_XCDS][1024];
+    __shared__ int s_flag, s_nw, s_tot, s_max;
+    const int t = threadIdx.x;
+    const int c = (P + 1023) / 1024;
+    const int p0 = min(P, t * c), p1 = min(P, p0 + c);
+    if (t == 0) { s_flag = 0; s_max = 0; }
+    __syncthreads();
+    int msum = 0, gsum[NCPA_XCDS], mx = 0;
+#pragma unroll
+    for (int x = 0; x < NCPA_XCDS; ++x) gsum[x] = 0;
+    for (int p = p0; p < p1; ++p) {
+        int nm = max(0, iv[p].n_hi - iv[p].n_lo);
+        if (nm > NCPA_MAX_MODES) { atomicOr(&s_flag, 1); nm = 0; }
+        msum += nm;
+        mx = max(mx, nm);
+        gsum[p & (NCPA_XCDS - 1)] += (nm + 63) / 64;
+    }
+    s_sum[t] = msum;
+#pragma unroll
+    for (int x = 0; x < NCPA_XCDS; ++x) s_grp[x][t] = gsum[x];
+    atomicMax(&s_max, mx);
+    __syncthreads();
+    // inclusive scans (Hillis-Steele) of the 1 + 8 per-thread sums
+    for (int d = 1; d < 1024; d <<= 1) {
+        int v = (t >= d) ? s_sum[t - d] : 0, g[NCPA_XCDS];
+#pragma unroll
+        for (int x = 0; x < NCPA_XCDS; ++x) g[x] = (t >= d) ? s_grp[x][t - d] : 0;
+        __syncthreads();
+        s_sum[t] += v;
+#pragma unroll
+        for (int x = 0; x < NCPA_XCDS; ++x) s_grp[x][t] += g[x];
+        __syncthreads();
+    }
+    if (t == 1023) {
+        int gmax = 0;
+        for (int x = 0; x < NCPA_XCDS; ++x) gmax = max(gmax, s_grp[x][1023]);
+        s_tot = s_sum[1023];
+        s_nw = NCPA_XCDS * gmax;
+        if (s_tot > cap_modes || s_nw > cap_work) s_flag |= 2;
+    }
+    __syncthreads();
+    const bool bad = s_flag != 0;
+    const int nw = bad ? 0 : s_nw;
+    for (int i = t; i < nw; i += 1024) work[i] = ModeWork{-1, 0};
+    __syncthreads();
+    int moff = s_sum[t] - msum, gpos[NCPA_XCDS];
+#pragma unroll
+    for (int x = 0; x < NCPA_XCDS; ++x) gpos[x] = s_grp[x][t] - gsum[x];
+    const double h2 = dz * dz, b = 1.0 / h2;
+    for (int p = p0; p < p1; ++p) {
+        const IntervalOut v = iv[p];
+        int nm = max(0, v.n_hi - v.n_lo);
+        if (nm > NCPA_MAX_MODES || bad) nm = 0;
+        ModeProb q;
+        q.a = A + (size_t)p * Nz;
+        q.w = W + (size_t)p * Nz;
+        q.b2 = b * b;
+        q.log2b = log2(b);
+        q.lo = v.lo;
+        q.hi = v.hi;
+        q.tol = 8.0 * DBL_EPSILON * (4.0 / h2 + fmax(fabs(v.hi), fabs(v.lo)));
+        q.inv_dz = 1.0 / dz;
+        q.n = Nz;
+        q.srow = probs[p].srow;
+        q.rrow = probs[p].rrow;
+        q.n_lo = v.n_lo;
+        q.n_hi = v.n_lo + nm;
+        q.out_off = moff;
+        mp[p] = q;
+        offsets[p] = moff;
+        const int x = p & (NCPA_XCDS - 1);
+        for (int g = 0; g < (nm + 63) / 64; ++g) work[NCPA_XCDS * (gpos[x] + g) + x] = ModeWork{p, v.n_lo + 64 * g};
+        gpos[x] += (nm + 63) / 64;
+        if (n_freq > 1) {
+            const int sset = p / (n_freq - 1), bin = p - sset * (n_freq - 1) + 1;
+            if (sset < n_sets) {
+                mbeg[sset * n_freq + bin] = moff;
+                mcnt[sset * n_freq + bin] = nm;
+                if (bin == 1) {
+                    mbeg[sset * n_freq] = 0;
+                    mcnt[sset * n_freq] = 0;
+                }
+            }
+        }
+        moff += nm;
+    }
+    if (t == 0) {
+        offsets[P] = bad ? 0 : s_tot;
+        meta->n_work = nw;
+        meta->total = s_tot;   // modes found, also when they are not computed (overflow)
+        meta->overflow = s_flag;
+        meta->max_nm = s_max;
+#pragma unroll
+        for (int x = 0; x < NCPA_XCDS; ++x) meta->queue[x] = 0;
+    }
 }
 
 
@@ -870,6 +969,35 @@ __global__ void __launch_bounds__(256) k_tl1d(const TlProb *__restrict__ tp, con
 // ======================================================================================
 // host side
 // ======================================================================================
+namespace ncpa {
+// k_modes with MS multisection passes (1..3, chosen by the autotuner; 2 by default)
+void launch_k_modes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, int n_work,
+                    const int *n_work_dev, int *queue, const ModesOut &mo) {
+    if (ms <= 1)
+        hipLaunchKernelGGL(k_modes<1>, grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, queue, mo);
+    else if (ms == 2)
+        hipLaunchKernelGGL(k_modes<2>, grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, queue, mo);
+    else
+        hipLaunchKernelGGL(k_modes<3>, grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, queue, mo);
+}
+
+void launch_plan_build(hipStream_t st, const ProbDev *probs, const IntervalOut *iv, int P, int Nz, double dz,
+                       const double *A, const double *W, int cap_modes, int cap_work, int n_sets, int n_freq,
+                       ModeProb *mp, int *offsets, ModeWork *work, int *mbeg, int *mcnt, PlanMeta *meta) {
+    hipLaunchKernelGGL(k_plan_build, dim3(1), dim3(1024), 0, st, probs, iv, P, Nz, dz, A, W, cap_modes, cap_work,
+                       n_sets, n_freq, mp, offsets, work, mbeg, mcnt, meta);
+}
+
+void launch_solve_front(hipStream_t st, const ProbDev *probs, int P, int max_nodes, const ncpa_grid_t &grid,
+                        double *d_alpha, double *A, double *W, double *CE, IntervalOut *iv) {
+    dim3 ga((max_nodes + 255) / 256, P);
+    hipLaunchKernelGGL(k_alpha_nodes, ga, dim3(256), 0, st, probs, max_nodes, d_alpha);
+    dim3 gs((grid.Nz + 255) / 256, P);
+    hipLaunchKernelGGL(k_setup_grid, gs, dim3(256), 0, st, probs, grid.Nz, grid.z_min, grid.dz, d_alpha, max_nodes,
+                       A, W, CE);
+    hipLaunchKernelGGL(k_interval, dim3(P), dim3(256), 0, st, probs, grid.Nz, grid.dz, A, CE, iv);
+}
+}  // namespace ncpa
 using namespace ncpa;
 
 struct ncpa_modes {
@@ -990,8 +1118,9 @@ int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, con
         NCPA_HIP(ctx, hipMemcpyAsync(m->d_work, work.data(), sizeof(ModeWork) * work.size(), hipMemcpyHostToDevice, ctx->stream));
         NCPA_HIP(ctx, hipMemcpyAsync(m->d_mode_prob, mode_prob.data(), sizeof(int) * total, hipMemcpyHostToDevice, ctx->stream));
         TimedLaunch tl(ctx, "modes");
-        hipLaunchKernelGGL(k_modes, dim3((unsigned)work.size()), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
-                           m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters);
+        const ModesOut mo{m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters};
+        launch_k_modes(ctx->ms_passes, dim3((unsigned)work.size()), ctx->stream, m->d_mp, m->d_work,
+                       (int)work.size(), nullptr, nullptr, mo);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(ctx, e, "k_modes launch");
@@ -1042,22 +1171,15 @@ int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, do
 
 }  // namespace
 
-extern "C" {
-
-int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems,
-                   int n_problems, ncpa_modes_t **out) {
-    if (!ctx || !problems || n_problems <= 0 || !out) return set_error(ctx, NCPA_ERR_ARG, "ncpa_ess_solve: bad arguments");
-    *out = nullptr;
-    int rc = check_grid(ctx, grid);
-    if (rc) return rc;
-    NCPA_HIP(ctx, hipSetDevice(ctx->device));
+namespace ncpa {
+// Device problem records of a batch (the per-problem part of ESSModeSolver::setParams and
+// solve: omega, azimuth, source/receiver rows (int) ceil(z/dz), ModeSolver.cpp:306-307).
+int build_problems(ncpa_ctx *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems, int n_problems,
+                   ProbDev *hp, int *max_nodes) {
     const int Nz = grid->Nz;
     const double dz = grid->dz;
-    const int P = n_problems;
-    std::vector<ProbDev> hp(P);
-    std::vector<int> srow(P), rrow(P);
-    int max_nodes = 0;
-    for (int p = 0; p < P; ++p) {
+    *max_nodes = 0;
+    for (int p = 0; p < n_problems; ++p) {
         const ncpa_problem_t &q = problems[p];
         if (q.profile < 0 || q.profile >= (int)ctx->profiles.size())
             return set_error(ctx, NCPA_ERR_ARG, "ncpa_ess_solve: unknown profile id");
@@ -1082,9 +1204,32 @@ int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_
         d.rrow = (int)std::ceil(q.zr / dz);
         if (d.srow < 0 || d.srow >= Nz || d.rrow < 0 || d.rrow >= Nz)
             return set_error(ctx, NCPA_ERR_ARG, "ncpa_ess_solve: source/receiver outside the grid");
-        srow[p] = d.srow;
-        rrow[p] = d.rrow;
-        max_nodes = std::max(max_nodes, pd.n);
+        *max_nodes = std::max(*max_nodes, pd.n);
+    }
+    return NCPA_OK;
+}
+}  // namespace ncpa
+
+extern "C" {
+
+int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems,
+                   int n_problems, ncpa_modes_t **out) {
+    if (!ctx || !problems || n_problems <= 0 || !out) return set_error(ctx, NCPA_ERR_ARG, "ncpa_ess_solve: bad arguments");
+    *out = nullptr;
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    NCPA_HIP(ctx, hipSetDevice(ctx->device));
+    const int Nz = grid->Nz;
+    const double dz = grid->dz;
+    const int P = n_problems;
+    std::vector<ProbDev> hp(P);
+    std::vector<int> srow(P), rrow(P);
+    int max_nodes = 0;
+    rc = build_problems(ctx, grid, problems, P, hp.data(), &max_nodes);
+    if (rc) return rc;
+    for (int p = 0; p < P; ++p) {
+        srow[p] = hp[p].srow;
+        rrow[p] = hp[p].rrow;
     }
     ncpa_modes *m = new ncpa_modes();
     m->ctx = ctx;

@@ -40,6 +40,8 @@ EXPORTS = [
     "ncpa_ess_solve", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
     "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d", "ncpa_ess_eigen",
     "ncpa_fft_batch", "ncpa_kpert", "ncpa_bb_transfer", "ncpa_bb_pulse", "ncpa_bb_source",
+    "ncpa_bb_plan_create", "ncpa_bb_plan_set_problems", "ncpa_bb_plan_run", "ncpa_bb_plan_result",
+    "ncpa_bb_plan_tuning", "ncpa_bb_plan_destroy",
 ]
 
 
@@ -53,6 +55,13 @@ class Problem(C.Structure):
                 ("azimuth", C.c_double), ("alpha_freq", C.c_double), ("zs", C.c_double),
                 ("zr", C.c_double), ("admittance", C.c_double), ("c_min", C.c_double),
                 ("c_max", C.c_double)]
+
+
+class BbJob(C.Structure):
+    _fields_ = [("n_sets", C.c_int), ("n_freq", C.c_int), ("f_vec", C.POINTER(C.c_double)),
+                ("spectrum", C.POINTER(C.c_double)), ("ranges_m", C.POINTER(C.c_double)),
+                ("n_ranges", C.c_int), ("rho_ratio", C.POINTER(C.c_double)), ("max_celerity", C.c_double),
+                ("nfft", C.c_int), ("zero_attn", C.c_int), ("max_modes_total", C.c_int), ("tune", C.c_int)]
 
 
 class ProfileInfo(C.Structure):
@@ -122,6 +131,12 @@ def load():
                                     vp]),
         "ncpa_bb_source": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, C.c_double, dp, C.c_int, dp,
                                      dp]),
+        "ncpa_bb_plan_create": (C.c_int, [vp, P(Grid), P(Problem), P(BbJob), P(vp)]),
+        "ncpa_bb_plan_set_problems": (C.c_int, [vp, P(Problem)]),
+        "ncpa_bb_plan_run": (C.c_int, [vp]),
+        "ncpa_bb_plan_result": (C.c_int, [vp, P(vp), P(vp), P(vp), P(vp), P(vp), ip]),
+        "ncpa_bb_plan_tuning": (C.c_int, [vp, ip, ip, dp, dp, ip]),
+        "ncpa_bb_plan_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

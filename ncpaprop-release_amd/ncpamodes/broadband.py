@@ -13,7 +13,7 @@ import math
 
 import numpy as np
 
-from ._lib import NCPA_SRC
+from ._lib import NCPA_SRC, BbJob, Problem
 
 
 def _dp(a):
@@ -186,3 +186,85 @@ def synthesize_from_modes(ctx, modes, n_sets, f, ranges_m, rho_ratio, S, max_cel
     W = pulse_device(ctx, n_sets, H, S, f, ranges_m, max_cel, nfft)
     H.free()
     return W, (kre, kim)
+
+
+class BroadbandPlan:
+    """A whole broadband job on the device as one captured HIP graph (ncpa_bb_plan_*).
+
+    problems[s * (Nfreq - 1) + i - 1] is set s (e.g. an azimuth) at f[i] (f[0] = 0 is DC).
+    run() replays the graph (dispersion solve of every problem, transfer functions at the
+    ranges, pulse synthesis); wave() waits and returns the waveforms [n_sets, n_ranges, NFFT].
+    """
+
+    def __init__(self, ctx, grid, problems, n_sets, f, S, ranges_m, rho_ratio, max_cel, nfft,
+                 zero_attn=False, max_modes_total=0, tune=True):
+        self.ctx = ctx
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        S = np.ascontiguousarray(S, dtype=np.complex128).view(np.float64)
+        ranges_m = np.ascontiguousarray(ranges_m, dtype=np.float64)
+        rho = np.ascontiguousarray(np.broadcast_to(rho_ratio, (n_sets,)), dtype=np.float64)
+        self._keep = (f, S, ranges_m, rho)
+        job = BbJob(n_sets, f.size, _dp(f), _dp(S), _dp(ranges_m), ranges_m.size, _dp(rho), max_cel, nfft,
+                    1 if zero_attn else 0, max_modes_total, 1 if tune else 0)
+        self.shape = (n_sets, ranges_m.size, nfft)
+        self.P = n_sets * (f.size - 1)
+        if len(problems) != self.P:
+            raise ValueError("a plan needs n_sets * (len(f) - 1) problems")
+        arr = (Problem * self.P)(*problems)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.ncpa_bb_plan_create(ctx.h, C.byref(grid), arr, C.byref(job), C.byref(h)))
+        self.h = h
+
+    def set_problems(self, problems):
+        arr = (Problem * self.P)(*problems)
+        self.ctx.check(self.ctx.lib.ncpa_bb_plan_set_problems(self.h, arr))
+
+    def run(self):
+        self.ctx.check(self.ctx.lib.ncpa_bb_plan_run(self.h))
+
+    def result(self):
+        """(device pointers dict, total modes) of the last run; waits for it."""
+        ptr = [C.c_void_p() for _ in range(5)]
+        total = C.c_int()
+        self.ctx.check(self.ctx.lib.ncpa_bb_plan_result(self.h, *[C.byref(p) for p in ptr], C.byref(total)))
+        names = ("wave", "kre", "kim", "psi_sr", "offsets")
+        return {n: p.value for n, p in zip(names, ptr)}, total.value
+
+    def wave(self):
+        ptrs, _ = self.result()
+        out = np.empty(self.shape)
+        self.ctx.check(self.ctx.lib.ncpa_dev_download(self.ctx.h, out.ctypes.data_as(C.c_void_p),
+                                                      C.c_void_p(ptrs["wave"]), out.nbytes))
+        return out
+
+    def modes(self):
+        """Host copies (offsets, k_re, k_im, psi_sr) of the last run."""
+        ptrs, total = self.result()
+        offs = np.empty(self.P + 1, dtype=np.int32)
+        arrs = [np.empty(total) for _ in range(3)]
+        lib, h = self.ctx.lib, self.ctx.h
+        self.ctx.check(lib.ncpa_dev_download(h, offs.ctypes.data_as(C.c_void_p), C.c_void_p(ptrs["offsets"]),
+                                             offs.nbytes))
+        for a, n in zip(arrs, ("kre", "kim", "psi_sr")):
+            if total:
+                self.ctx.check(lib.ncpa_dev_download(h, a.ctypes.data_as(C.c_void_p), C.c_void_p(ptrs[n]), a.nbytes))
+        return offs, arrs[0], arrs[1], arrs[2]
+
+    def tuning(self):
+        ms, blocks, tms, n = C.c_int(), C.c_int(), C.c_double(), C.c_int(64)
+        log = np.zeros(3 * 64)
+        self.ctx.check(self.ctx.lib.ncpa_bb_plan_tuning(self.h, C.byref(ms), C.byref(blocks), C.byref(tms),
+                                                         _dp(log), C.byref(n)))
+        return {"ms_passes": ms.value, "grid_blocks": blocks.value, "modes_ms": tms.value,
+                "log": log[:3 * n.value].reshape(-1, 3).tolist()}
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ncpa_bb_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
