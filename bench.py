@@ -431,7 +431,10 @@ def broadband_main(args, cpu):
         "plan": {"create_s": plan_s, "ms_passes": tuning["ms_passes"], "grid_blocks": tuning["grid_blocks"],
                  "k_modes_ms": tuning["modes_ms"],
                  "tuner_ms": {"%d/%d" % (int(a), int(b)): round(c, 3) for a, b, c in tuning["log"]}},
-        "roofline": {"kernel": "k_modes<%d> (persistent)" % tuning["ms_passes"], "bound": "mfma",
+        "roofline": {"kernel": "k_modes<%d> (%s)" % (tuning["ms_passes"], "one block per work-list slot"
+                                                     if tuning["grid_blocks"] == 0 else
+                                                     "persistent, %d blocks" % tuning["grid_blocks"]),
+                     "bound": "mfma",
                      "achieved": achieved, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS["f64"], "traffic": None,
                      "algorithmic": "3 FP64 flops per Sturm step (row x sweep); FP64 VALU"},

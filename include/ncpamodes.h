@@ -266,8 +266,9 @@ int ncpa_bb_source(ncpa_ctx_t *ctx, int kind, const double *f_vec, int n_freq, i
  * waveforms are synthesised without a count on the host, so the whole job is captured once
  * as a HIP graph and replayed by ncpa_bb_plan_run.  Creation runs the batch's interval
  * counts once to size the mode buffers and, with tune != 0, times the eigenvalue kernel's
- * variants (multisection passes 1-3 x persistent grid of 4, 8, 15 blocks per CU) on the
- * batch and keeps the fastest.
+ * variants on the batch and keeps the fastest: multisection passes 1-3 x launch form (a
+ * persistent grid of 4, 8 or 15 blocks per CU pulling work from per-XCD queues, or one
+ * block per work-list slot; grid_blocks = 0 reports the latter).
  * ------------------------------------------------------------------------------------------ */
 typedef struct ncpa_bb_plan ncpa_bb_plan_t;
 
@@ -302,7 +303,8 @@ int ncpa_bb_plan_result(ncpa_bb_plan_t *plan, const double **dev_wave, const dou
                         const double **dev_kim, const double **dev_psi_sr, const int **dev_offsets,
                         int *total_modes);
 /* The configuration in use and the tuner's measurements: log holds (passes, blocks, ms)
- * triples; *n_log in: capacity in triples, out: triples measured. */
+ * triples (blocks 0: one block per work-list slot); *n_log in: capacity in triples, out:
+ * triples measured. */
 int ncpa_bb_plan_tuning(ncpa_bb_plan_t *plan, int *ms_passes, int *grid_blocks, double *modes_ms,
                         double *log, int *n_log);
 void ncpa_bb_plan_destroy(ncpa_bb_plan_t *plan);

@@ -161,16 +161,18 @@ __global__ void k_setup_grid(const ProbDev *__restrict__ probs, int Nz, double z
     const double h2 = dz * dz;
     const double z = z_min + i * dz;
     const double ceff = steffen_eval(zn, n, ce, z);
-    double diag = (pr.omega / ceff) * (pr.omega / ceff);
-    if (i == 0) diag = (1.0 / (dz * pr.admittance + 1)) / h2 + diag;  // ESSModeSolver.cpp:558-559
-    double a = -2.0 / h2 + diag;
+    // row of dz^2 T: -2 + dz^2 diag, with diag = (omega/c_eff)^2 + the ground term on row 0
+    // (ESSModeSolver.cpp:544-559, EigenEngine.cpp:57-79)
+    double diag = h2 * ((pr.omega / ceff) * (pr.omega / ceff));
+    if (i == 0) diag = 1.0 / (dz * pr.admittance + 1) + diag;
+    double a = -2.0 + diag;
     if (i == Nz - 1) {
         // last row takes diag[Nz-2] (stale value[1], EigenEngine.cpp:57-71)
         const double zp = z_min + (Nz - 2) * dz;
         const double cp = steffen_eval(zn, n, ce, zp);
-        double dp = (pr.omega / cp) * (pr.omega / cp);
-        if (Nz - 2 == 0) dp = (1.0 / (dz * pr.admittance + 1)) / h2 + dp;
-        a = -2.0 / h2 + dp;
+        double dp = h2 * ((pr.omega / cp) * (pr.omega / cp));
+        if (Nz - 2 == 0) dp = 1.0 / (dz * pr.admittance + 1) + dp;
+        a = -2.0 + dp;
     }
     const double alpha = steffen_eval(zn, n, ArrNode{alpha_nodes + size_t(p) * ldn}, z);
     const double Pz = steffen_eval(zn, n, ArrNode{pr.prof + PF_P * n}, z);
@@ -277,8 +279,7 @@ __global__ void __launch_bounds__(256) k_interval(const ProbDev *__restrict__ pr
     __syncthreads();
     const double h2 = dz * dz;
     if (threadIdx.x < 64) {   // the whole first wave sweeps (lanes 0 and 1 keep their counts)
-        const int c = sturm_count(A + size_t(p) * Nz, s_ring, Nz, (1.0 / h2) * (1.0 / h2),
-                                  s_lohi[threadIdx.x ? 1 : 0]);
+        const int c = sturm_count(A + size_t(p) * Nz, s_ring, Nz, h2 * s_lohi[threadIdx.x ? 1 : 0]);
         if (threadIdx.x == 0) out[p].n_lo = c;
         if (threadIdx.x == 1) out[p].n_hi = c;
     }
@@ -295,10 +296,9 @@ __global__ void __launch_bounds__(256) k_interval(const ProbDev *__restrict__ pr
 // eigenvalues + sensitivities: one 64-lane block per (problem, 64 consecutive indices)
 // ------------------------------------------------------------------------------------
 // Sturm counts at two shifts (lane 0: lo, lane 1: hi) for one matrix
-__global__ void k_counts2(const double *__restrict__ a, int n, double b2, double lo, double hi,
-                          int *__restrict__ out) {
+__global__ void k_counts2(const double *__restrict__ a, int n, double lo, double hi, int *__restrict__ out) {
     __shared__ double s_ring[NCPA_RING];
-    const int c = sturm_count(a, s_ring, n, b2, threadIdx.x ? hi : lo);   // whole wave
+    const int c = sturm_count(a, s_ring, n, threadIdx.x ? hi : lo);   // whole wave
     if (threadIdx.x < 2) out[threadIdx.x] = c;
 }
 
@@ -337,7 +337,7 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const ModeWork wk,
     double blo = lo, bhi = hi;
     for (int pass = 0; pass < MS; ++pass) {
         const double sh = blo + (double)(lane + 1) * ((bhi - blo) * (1.0 / 65.0));
-        const int c = sturm_count(P.a, s_ring, P.n, P.b2, sh);
+        const int c = sturm_count(P.a, s_ring, P.n, sh);
         ms_s[lane] = sh;
         ms_c[lane] = c;
         __syncthreads();
@@ -365,6 +365,7 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const ModeWork wk,
     // unconverged lane then tightens its bracket from all 64 (shift, count) pairs, so the
     // stragglers that set the block's duration converge several bits per sweep.
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    double prev_step = INFINITY;   // the last Newton step taken (INFINITY after a bisection)
     for (int it = 0; it < NCPA_MAXIT; ++it) {
         const unsigned long long um = __ballot(!done);
         if (um == 0ull) break;
@@ -388,12 +389,12 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const ModeWork wk,
         int cnt;
         double p0 = 0.0, q0 = 0.0;
         if (__any(newton)) {
-            const NewtonOut o = sturm_newton(P.a, s_ring, P.n, P.b2, sh);
+            const NewtonOut o = sturm_newton(P.a, s_ring, P.n, sh);
             cnt = o.cnt;
             p0 = o.p0;
             q0 = o.q0;
         } else {
-            cnt = sturm_count(P.a, s_ring, P.n, P.b2, sh);
+            cnt = sturm_count(P.a, s_ring, P.n, sh);
         }
         ms_s[lane] = sh;
         ms_c[lane] = cnt;
@@ -410,17 +411,26 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const ModeWork wk,
                 }
             }
             sig = 0.5 * (lo + hi);
+            double step = INFINITY;
             if (newton && q0 != 0.0 && isfinite(p0) && isfinite(q0)) {
                 const double dlt = -p0 / q0;
                 const double cand = sh + dlt;
-                if (fabs(dlt) <= P.tol) {
+                const double adl = fabs(dlt);
+                // Converged when the step is at the tolerance, or when two successive Newton
+                // steps show the quadratic regime (ratio below 1e-3) and predict the next
+                // step, |dlt|^3 / |dlt_prev|^2, at the tolerance: the sweep that would only
+                // confirm it is skipped.
+                const double rat = adl / prev_step;   // 0 after a bisection: no estimate yet
+                if (adl <= P.tol || (prev_step < INFINITY && rat <= 1e-3 && adl * rat * rat <= P.tol)) {
                     sig = cand;
                     done = true;
                     my_iters = MS + it + 1;
                 } else if (cand > lo && cand < hi) {
                     sig = cand;
+                    step = adl;
                 }
             }
+            prev_step = step;
             if (!done && (hi - lo) <= P.tol) {
                 sig = 0.5 * (lo + hi);
                 done = true;
@@ -432,20 +442,20 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const ModeWork wk,
     }
     if (!done) my_iters = -NCPA_MAXIT - 1;
     // phase 3: sensitivities at the converged eigenvalue (every lane, one sweep)
-    const FinalOut f = sturm_final(P.a, P.w, s_ring, P.n, P.b2, sig, P.srow, P.rrow);
+    const FinalOut f = sturm_final(P.a, P.w, s_ring, P.n, sig, P.srow, P.rrow);
     SVal th_s, th_r;
-    theta_forward(P.a, s_ring, P.b2, sig, P.srow, P.rrow, th_s, th_r);
+    theta_forward(P.a, s_ring, sig, P.srow, P.rrow, th_s, th_r);
     if (!active) return;   // (whole-wave work above is done; nothing of this lane is left)
     const double mult = collapsed ? (double)(ch - cl) : 1.0;
-    const double us2 = adjugate_product(th_s, f.phi_s1, f.q0, f.ex, 0, P.log2b) / mult;
-    const double ur2 = adjugate_product(th_r, f.phi_r1, f.q0, f.ex, 0, P.log2b) / mult;
+    const double us2 = adjugate_product(th_s, f.phi_s1, f.q0, f.ex, 0) / mult;
+    const double ur2 = adjugate_product(th_r, f.phi_r1, f.q0, f.ex, 0) / mult;
     double usr;
     if (P.srow <= P.rrow)
-        usr = adjugate_product(th_s, f.phi_r1, f.q0, f.ex, P.rrow - P.srow, P.log2b) / mult;
+        usr = adjugate_product(th_s, f.phi_r1, f.q0, f.ex, P.rrow - P.srow) / mult;
     else
-        usr = adjugate_product(th_r, f.phi_s1, f.q0, f.ex, P.srow - P.rrow, P.log2b) / mult;
+        usr = adjugate_product(th_r, f.phi_s1, f.q0, f.ex, P.srow - P.rrow) / mult;
     const int o = P.out_off + (j - P.n_lo);
-    out.k2[o] = sig;
+    out.k2[o] = sig * P.inv_h2;   // lambda = lambda' / dz^2
     out.absn[o] = (f.q0 != 0.0) ? -f.e0 / f.q0 : 0.0;
     const double ps = sqrt(fmax(us2, 0.0) * P.inv_dz);
     const double pr = sqrt(fmax(ur2, 0.0) * P.inv_dz);
@@ -455,21 +465,31 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const ModeWork wk,
     out.iters[o] = my_iters;
 }
 
-// Two launch forms.  Eager: one block per work item (grid = n_work).  Persistent (queue
-// given): a fixed grid whose blocks pull items from per-XCD counters -- block b, running
-// on XCD x = b % 8, takes items 8t + x for t = queue[x]++ -- until the list written on the
-// device by k_plan_build (n_work_dev) is exhausted, so no count is needed on the host.
+// Two launch forms of the eigenvalue kernel, both over the XCD-interleaved work list
+// (item 8t + x belongs to XCD x; workgroups go to the XCDs round-robin, block b -> b % 8):
+//   k_modes             one block per item, block b takes item b.  The item count comes from
+//                       the host (eager solves) or from the device (n_work_dev, written by
+//                       k_plan_build) with the grid sized by the plan's capacity: blocks
+//                       past the count exit at once, so the launch needs no host count.
+//   k_modes_persistent  a fixed grid whose blocks pull items 8t + x, t = queue[x]++, until
+//                       the device count is exhausted.  (Its loop costs registers: ~200
+//                       VGPRs against ~120 for k_modes; the plan's autotuner times both.)
 template <int MS>
 __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
-                                              int n_work, const int *__restrict__ n_work_dev, int *queue,
-                                              ModesOut out) {
+                                              int n_work, const int *__restrict__ n_work_dev, ModesOut out) {
     __shared__ ModesShared sh;
-    if (!queue) {
-        if (blockIdx.x >= (unsigned)n_work) return;
-        const ModeWork wk = work[blockIdx.x];
-        if (wk.prob >= 0) modes_item<MS>(mp[wk.prob], wk, sh, out);
-        return;
-    }
+    const int nw = n_work_dev ? *n_work_dev : n_work;
+    if (blockIdx.x >= (unsigned)nw) return;
+    const ModeWork wk = work[blockIdx.x];
+    if (wk.prob >= 0) modes_item<MS>(mp[wk.prob], wk, sh, out);
+}
+
+template <int MS>
+__global__ void __launch_bounds__(64) k_modes_persistent(const ModeProb *__restrict__ mp,
+                                                         const ModeWork *__restrict__ work,
+                                                         const int *__restrict__ n_work_dev, int *queue,
+                                                         ModesOut out) {
+    __shared__ ModesShared sh;
     const int nw = *n_work_dev;
     const int x = blockIdx.x & (NCPA_XCDS - 1);
     for (;;) {
@@ -556,11 +576,11 @@ __global__ void __launch_bounds__(1024) k_plan_build(const ProbDev *__restrict__
         ModeProb q;
         q.a = A + (size_t)p * Nz;
         q.w = W + (size_t)p * Nz;
-        q.b2 = b * b;
-        q.log2b = log2(b);
-        q.lo = v.lo;
-        q.hi = v.hi;
-        q.tol = 8.0 * DBL_EPSILON * (4.0 / h2 + fmax(fabs(v.hi), fabs(v.lo)));
+        q.h2 = h2;
+        q.inv_h2 = b;
+        q.lo = v.lo * h2;
+        q.hi = v.hi * h2;
+        q.tol = 8.0 * DBL_EPSILON * (4.0 + fmax(fabs(q.hi), fabs(q.lo)));   // 8 eps ||dz^2 T||
         q.inv_dz = 1.0 / dz;
         q.n = Nz;
         q.srow = probs[p].srow;
@@ -592,6 +612,7 @@ __global__ void __launch_bounds__(1024) k_plan_build(const ProbDev *__restrict__
         meta->total = s_tot;   // modes found, also when they are not computed (overflow)
         meta->overflow = s_flag;
         meta->max_nm = s_max;
+        meta->n_work_need = s_nw;
 #pragma unroll
         for (int x = 0; x < NCPA_XCDS; ++x) meta->queue[x] = 0;
     }
@@ -632,9 +653,8 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
     const int j = wk.j0 + lane;
     const bool active = j < P.n_hi;
     const size_t o = (size_t)P.out_off + (size_t)(active ? (j - P.n_lo) : 0);
-    const double s = active ? k2[o] : 0.5 * (P.lo + P.hi);
+    const double s = active ? k2[o] * P.h2 : 0.5 * (P.lo + P.hi);   // shift of dz^2 T
     const int n = P.n;
-    const double b2 = P.b2;
     const double *a = P.a;
     const size_t src = (size_t)ra.nrec * ra.ld + o;   // slot of the source row
     const int last_rec = ra.row0 + (ra.nrec - 1) * ra.stride;
@@ -652,7 +672,7 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
         sweep_up(
             a, s_ring, n - 1,
             [&](double ak, int k) {
-                const double t0 = fma(ak - s, t1, -b2 * t2);
+                const double t0 = fma(ak - s, t1, -t2);
                 t2 = t1;
                 t1 = t0;
                 rec[k & (NCPA_CHUNK - 1)][lane] = t1;
@@ -689,8 +709,8 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
             a, s_ring, n,
             [&](double ai, int i) {
                 const double am = ai - s;
-                const double p0 = fma(am, p1, -b2 * p2);
-                const double q0 = fma(am, q1, fma(-b2, q2, -p1));
+                const double p0 = fma(am, p1, -p2);
+                const double q0 = fma(am, q1, -(q2 + p1));
                 p2 = p1;
                 p1 = p0;
                 q2 = q1;
@@ -790,7 +810,7 @@ __global__ void __launch_bounds__(256) k_rows_ref(const double *__restrict__ TH,
     double us = 0.0;
     if (tv != 0.0 && pv != 0.0) {
         const double sg = ((d & 1) ? 1.0 : -1.0) * copysign(1.0, tv) * copysign(1.0, pv) * sq;
-        us = sg * exp2(log2abs_prod(tv, tev, pv, pev) + d * P.log2b - lq - 0.5 * l2r);
+        us = sg * exp2(log2abs_prod(tv, tev, pv, pev) - lq - 0.5 * l2r);
     }
     // psi(zs) from the reference row: accurate also for modes trapped aloft, where the
     // square root of the adjugate diagonal at the ground would sit at its noise floor
@@ -841,7 +861,7 @@ __global__ void __launch_bounds__(256) k_rows_combine(const double *__restrict__
         double u = 0.0;
         if (tv != 0.0 && pv != 0.0) {
             const double sg = ((d & 1) ? 1.0 : -1.0) * copysign(1.0, tv) * copysign(1.0, pv) * sq;
-            u = sg * exp2(log2abs_prod(tv, tev, pv, pev) + d * P.log2b - lq - 0.5 * l2r);
+            u = sg * exp2(log2abs_prod(tv, tev, pv, pev) - lq - 0.5 * l2r);
         }
         psi[ko] = flip * u;
     }
@@ -973,12 +993,21 @@ namespace ncpa {
 // k_modes with MS multisection passes (1..3, chosen by the autotuner; 2 by default)
 void launch_k_modes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, int n_work,
                     const int *n_work_dev, int *queue, const ModesOut &mo) {
+    if (queue) {
+        if (ms <= 1)
+            hipLaunchKernelGGL(k_modes_persistent<1>, grid, dim3(64), 0, st, mp, work, n_work_dev, queue, mo);
+        else if (ms == 2)
+            hipLaunchKernelGGL(k_modes_persistent<2>, grid, dim3(64), 0, st, mp, work, n_work_dev, queue, mo);
+        else
+            hipLaunchKernelGGL(k_modes_persistent<3>, grid, dim3(64), 0, st, mp, work, n_work_dev, queue, mo);
+        return;
+    }
     if (ms <= 1)
-        hipLaunchKernelGGL(k_modes<1>, grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, queue, mo);
+        hipLaunchKernelGGL(k_modes<1>, grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, mo);
     else if (ms == 2)
-        hipLaunchKernelGGL(k_modes<2>, grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, queue, mo);
+        hipLaunchKernelGGL(k_modes<2>, grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, mo);
     else
-        hipLaunchKernelGGL(k_modes<3>, grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, queue, mo);
+        hipLaunchKernelGGL(k_modes<3>, grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, mo);
 }
 
 void launch_plan_build(hipStream_t st, const ProbDev *probs, const IntervalOut *iv, int P, int Nz, double dz,
@@ -1058,12 +1087,12 @@ int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, con
         ModeProb &q = m->mp[p];
         q.a = m->d_A + size_t(p) * Nz;
         q.w = m->d_W + size_t(p) * Nz;
-        q.b2 = b * b;
-        q.log2b = std::log2(b);
-        q.lo = v.lo;
-        q.hi = v.hi;
-        // ||T|| <= 4/h^2 + max diag: the absolute floor of FP64 eigenvalues of T
-        const double normT = 4.0 / h2 + std::max(std::fabs(v.hi), std::fabs(v.lo));
+        q.h2 = h2;
+        q.inv_h2 = b;
+        q.lo = v.lo * h2;   // the eigenvalue kernels work on dz^2 T
+        q.hi = v.hi * h2;
+        // ||dz^2 T|| <= 4 + max diag: the absolute floor of FP64 eigenvalues of dz^2 T
+        const double normT = 4.0 + std::max(std::fabs(q.hi), std::fabs(q.lo));
         q.tol = 8.0 * DBL_EPSILON * normT;
         q.inv_dz = 1.0 / dz;
         q.n = Nz;
@@ -1428,11 +1457,11 @@ int ncpa_ess_eigen(ncpa_ctx_t *ctx, const double *diag, int Nz, double dz, doubl
         return set_error(ctx, NCPA_ERR_ARG, "ncpa_ess_eigen: bad arguments");
     NCPA_HIP(ctx, hipSetDevice(ctx->device));
     *nconv = 0;
-    // the operator of EigenEngine::doESSCalculation (EigenEngine.cpp:57-79)
+    // the operator of EigenEngine::doESSCalculation (EigenEngine.cpp:57-79), times dz^2
     const double h2 = dz * dz;
     std::vector<double> a(Nz);
-    for (int i = 0; i < Nz - 1; ++i) a[i] = -2.0 / h2 + diag[i];
-    a[Nz - 1] = -2.0 / h2 + diag[Nz - 2];
+    for (int i = 0; i < Nz - 1; ++i) a[i] = -2.0 + h2 * diag[i];
+    a[Nz - 1] = -2.0 + h2 * diag[Nz - 2];
     ncpa_modes *m = new ncpa_modes();
     m->ctx = ctx;
     m->P = 1;
@@ -1457,7 +1486,7 @@ int ncpa_ess_eigen(ncpa_ctx_t *ctx, const double *diag, int Nz, double dz, doubl
         (e = hipMemsetAsync(m->d_W, 0, sizeof(double) * Nz, ctx->stream)) != hipSuccess)
         return done(hip_error(ctx, e, "ncpa_ess_eigen upload"));
     const double lo = (*k_min) * (*k_min), hi = (*k_max) * (*k_max);   // EPSSetInterval(k_min^2, k_max^2)
-    hipLaunchKernelGGL(k_counts2, dim3(1), dim3(64), 0, ctx->stream, m->d_A, Nz, (1.0 / h2) * (1.0 / h2), lo, hi, d_cnt);
+    hipLaunchKernelGGL(k_counts2, dim3(1), dim3(64), 0, ctx->stream, m->d_A, Nz, lo * h2, hi * h2, d_cnt);
     int cnt[2] = {0, 0};
     if ((e = hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)

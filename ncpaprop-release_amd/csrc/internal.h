@@ -34,13 +34,14 @@ struct IntervalOut {
     int pad;
 };
 
+// Per problem of the eigenvalue kernels.  a: the rows of dz^2 T (unit off-diagonal); lo,
+// hi, tol and the shifts are in eigenvalues of dz^2 T; outputs are lambda = lambda' / dz^2.
 struct ModeProb {
     const double *a;
     const double *w;
-    double b2;
-    double log2b;
+    double h2, inv_h2;   // dz^2, 1/dz^2
     double lo, hi;
-    double tol;       // absolute convergence tolerance on lambda
+    double tol;       // absolute convergence tolerance on lambda' (eigenvalue of dz^2 T)
     double inv_dz;
     int n;
     int srow, rrow;
@@ -65,6 +66,7 @@ struct PlanMeta {
     int overflow;   // bit 0: a problem above NCPA_MAX_MODES; bit 1: batch above the plan's
                     // capacity.  Either way no mode is computed (n_work = 0).
     int max_nm;     // most modes of one problem
+    int n_work_need;   // work-list length the batch needs (also when it exceeds the capacity)
     int queue[NCPA_XCDS];   // per-XCD work counters of the persistent launch (reset here)
 };
 
@@ -74,6 +76,8 @@ void launch_solve_front(hipStream_t st, const ProbDev *probs, int P, int max_nod
 void launch_plan_build(hipStream_t st, const ProbDev *probs, const IntervalOut *iv, int P, int Nz, double dz,
                        const double *A, const double *W, int cap_modes, int cap_work, int n_sets, int n_freq,
                        ModeProb *mp, int *offsets, ModeWork *work, int *mbeg, int *mcnt, PlanMeta *meta);
+// queue != nullptr: the persistent form (grid = resident blocks); else one block per item,
+// n_work from the host or, when n_work_dev is given, from the device (grid = capacity)
 void launch_k_modes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, int n_work,
                     const int *n_work_dev, int *queue, const ModesOut &mo);
 int build_problems(ncpa_ctx *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems, int n_problems,

@@ -89,9 +89,15 @@ void enqueue_front(ncpa_bb_plan *pl, int cap_modes, int cap_work) {
                       pl->n_sets, pl->n_freq, pl->mp, pl->offsets, pl->work, pl->mbeg, pl->mcnt, pl->meta);
 }
 
+// blocks > 0: persistent form with that grid; 0: one block per work-list slot (grid =
+// capacity, blocks past the device count exit)
 void enqueue_modes(ncpa_bb_plan *pl, int ms, int blocks) {
-    launch_k_modes(ms, dim3((unsigned)blocks), pl->ctx->stream, pl->mp, pl->work, 0, &pl->meta->n_work,
-                   pl->meta->queue, modes_out(pl));
+    if (blocks > 0)
+        launch_k_modes(ms, dim3((unsigned)blocks), pl->ctx->stream, pl->mp, pl->work, 0, &pl->meta->n_work,
+                       pl->meta->queue, modes_out(pl));
+    else
+        launch_k_modes(ms, dim3((unsigned)pl->cap_work), pl->ctx->stream, pl->mp, pl->work, 0, &pl->meta->n_work,
+                       nullptr, modes_out(pl));
 }
 
 int enqueue_all(ncpa_bb_plan *pl) {
@@ -126,17 +132,17 @@ int overflow_error(ncpa_bb_plan *pl, const PlanMeta &m) {
                          std::to_string(pl->cap_modes));
 }
 
-// Times every (multisection passes, persistent grid) candidate on the plan's own batch
-// (front stages once, then k_plan_build + k_modes per candidate; k_plan_build resets the
-// queues) and keeps the fastest.
+// Times every (multisection passes, launch form) candidate on the plan's own batch (front
+// stages once, then k_plan_build + k_modes per candidate; k_plan_build resets the queues)
+// and keeps the fastest.  Launch forms: persistent grids of 4, 8 and 15 blocks per CU (one
+// wave per SIMD, two, the LDS limit) and one block per work-list slot (blocks = 0).
 int autotune(ncpa_bb_plan *pl, int n_cu) {
     hipStream_t st = pl->ctx->stream;
     hipEvent_t e0, e1;
     NCPA_HIP(pl->ctx, hipEventCreate(&e0));
     NCPA_HIP(pl->ctx, hipEventCreate(&e1));
-    // blocks per CU: one wave per SIMD, two, and the LDS limit (10 KB per block, 160 KB per CU)
     const int ms_cand[] = {1, 2, 3};
-    const int per_cu[] = {4, 8, 15};
+    const int per_cu[] = {0, 4, 8, 15};
     double best = 1e300;
     int rc = NCPA_OK;
     enqueue_front(pl, pl->cap_modes, pl->cap_work);
@@ -230,7 +236,10 @@ int ncpa_bb_plan_create(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_pro
     if (m.overflow & 1) return done(overflow_error(pl, m));
     pl->cap_modes = job->max_modes_total > 0 ? job->max_modes_total : m.total + m.total / 2 + 64;
     if (m.total > pl->cap_modes) return done(overflow_error(pl, m));
-    pl->cap_work = NCPA_XCDS * (pl->cap_modes / 64 + P) + NCPA_XCDS;
+    // work-list slots: the batch's need with the same 1.5x headroom (the one-block-per-slot
+    // launch has this many blocks; a later batch that needs more is reported as overflow)
+    pl->cap_work = job->max_modes_total > 0 ? NCPA_XCDS * (pl->cap_modes / 64 + P) + NCPA_XCDS
+                                            : m.n_work_need + m.n_work_need / 2 + 8 * NCPA_XCDS;
     const size_t cm = (size_t)pl->cap_modes;
     const int n_tr = pl->n_sets * pl->n_ranges;
     size_t scratch = 0;
@@ -267,7 +276,7 @@ int ncpa_bb_plan_create(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_pro
     if ((e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device)) != hipSuccess)
         return done(hip_error(ctx, e, "ncpa_bb_plan_create"));
     pl->ms = ctx->ms_passes;
-    pl->blocks = std::max(NCPA_XCDS, n_cu * 8);
+    pl->blocks = 0;
     if (job->tune) {
         if ((rc = autotune(pl, std::max(n_cu, 1)))) return done(rc);
         ctx->ms_passes = pl->ms;   // the eager solves of this context follow the tuned depth

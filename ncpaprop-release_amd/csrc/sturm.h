@@ -1,18 +1,22 @@
 // FP64 Sturm sweeps over the finite-difference Helmholtz tridiagonal.
 //
 // T = tridiag(b, a_i, b), constant off-diagonal b = 1/dz^2 (src/libmodes/EigenEngine.cpp:57-79).
-// A sweep runs the continuant (three-term) recurrence from the top row down to the ground:
-//     phi_n = 1, phi_{n+1} = 0,   phi_i = (a_i - s) phi_{i+1} - b^2 phi_{i+2}
-// so phi_i = det(T[i:n, i:n] - s) and phi_0 = det(T - s).  The LDL^T pivots are
+// The sweeps work on T' = dz^2 T = tridiag(1, a'_i, 1), a'_i = dz^2 a_i, whose eigenvalues
+// are dz^2 lambda and whose eigenvectors are those of T.  A sweep runs the continuant
+// (three-term) recurrence from the top row down to the ground:
+//     phi_n = 1, phi_{n+1} = 0,   phi_i = (a'_i - s) phi_{i+1} - phi_{i+2}
+// so phi_i = det(T'[i:n, i:n] - s) and phi_0 = det(T' - s).  The LDL^T pivots are
 // D_i = phi_i / phi_{i+1}; negative pivots count the eigenvalues below s (Sylvester).
-// No division sits on the dependency chain (one mul + one fma per row); every 16 rows the
-// state is rescaled by an exact power of two so nothing over- or underflows.
+// With the unit off-diagonal a row is one subtract and one fma (the negation is an operand
+// modifier); every 16 rows the state is rescaled by an exact power of two so nothing
+// over- or underflows.
 //
 // Derivative continuants carried with it:
 //     q_i = d phi_i / ds             -> Newton step on det:  -phi_0 / q_0
-//     e_i = d phi_i / d eps  for T + eps diag(w)   -> absorption u^T W u = -e_0 / q_0
-// and the adjugate of a tridiagonal gives the eigenvector products at an eigenvalue:
-//     u_s u_r = -(-b)^(r-s) theta_{s-1} phi_{r+1} / q_0      (s <= r)
+//     e_i = d phi_i / d eps  for T' + eps diag(w)  -> absorption u^T W u = -e_0 / q_0
+// and the adjugate of a tridiagonal with unit off-diagonal gives the eigenvector products
+// at an eigenvalue:
+//     u_s u_r = -(-1)^(r-s) theta_{s-1} phi_{r+1} / q_0      (s <= r)
 // with theta_k the forward continuant det(T[0:k+1, 0:k+1] - s) from the ground.  Each
 // factor is computed in its stable direction, so tiny components (modes trapped aloft)
 // come out tiny instead of at a noise floor.
@@ -268,14 +272,14 @@ __device__ __forceinline__ void sweep_up(const double *a, double *ring, int m, S
 }
 
 // count of eigenvalues below s (full-wave entry; s may differ per lane)
-__device__ __forceinline__ int sturm_count(const double *a, double *ring, int n, double b2, double s) {
+__device__ __forceinline__ int sturm_count(const double *a, double *ring, int n, double s) {
     double p1 = 1.0, p2 = 0.0;
     int cnt = 0;
     unsigned bits = 0u;   // bit 0: sign of the newest continuant; starts with phi_n = +1
     sweep_down(
         a, ring, n,
         [&](double ai, int) {
-            const double p0 = fma(ai - s, p1, -b2 * p2);
+            const double p0 = fma(ai - s, p1, -p2);
             bits = push_sign(bits, p0);
             p2 = p1;
             p1 = p0;
@@ -296,7 +300,7 @@ struct NewtonOut {
 };
 
 // count plus det and d det/ds (Newton step -p0/q0)
-__device__ __forceinline__ NewtonOut sturm_newton(const double *a, double *ring, int n, double b2, double s) {
+__device__ __forceinline__ NewtonOut sturm_newton(const double *a, double *ring, int n, double s) {
     double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
     int cnt = 0;
     unsigned bits = 0u;
@@ -304,8 +308,8 @@ __device__ __forceinline__ NewtonOut sturm_newton(const double *a, double *ring,
         a, ring, n,
         [&](double ai, int) {
             const double am = ai - s;
-            const double p0 = fma(am, p1, -b2 * p2);
-            const double q0 = fma(am, q1, fma(-b2, q2, -p1));
+            const double p0 = fma(am, p1, -p2);
+            const double q0 = fma(am, q1, -(q2 + p1));
             bits = push_sign(bits, p0);
             p2 = p1;
             p1 = p0;
@@ -341,8 +345,8 @@ struct FinalOut {
 };
 
 // backward sweep with absorption derivative and the records phi_{s+1}, phi_{r+1}
-__device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w, double *ring2, int n, double b2,
-                                                double s, int srow, int rrow) {
+__device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w, double *ring2, int n, double s,
+                                                int srow, int rrow) {
     double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0, e1 = 0.0, e2 = 0.0;
     int ex = 0;
     FinalOut o;
@@ -350,9 +354,9 @@ __device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w
     o.phi_r1 = SVal{1.0, 0};
     auto step = [&](double ai, double wi, int) {
         const double am = ai - s;
-        const double p0 = fma(am, p1, -b2 * p2);
-        const double q0 = fma(am, q1, fma(-b2, q2, -p1));
-        const double e0 = fma(am, e1, fma(-b2, e2, wi * p1));
+        const double p0 = fma(am, p1, -p2);
+        const double q0 = fma(am, q1, -(q2 + p1));
+        const double e0 = fma(am, e1, fma(wi, p1, -e2));
         p2 = p1;
         p1 = p0;
         q2 = q1;
@@ -388,8 +392,8 @@ __device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w
 }
 
 // forward continuants theta_{s-1}, theta_{r-1} (theta_{-1} = 1); rows 0 .. max(s,r)-1
-__device__ __forceinline__ void theta_forward(const double *a, double *ring, double b2, double s, int srow, int rrow,
-                                              SVal &th_s, SVal &th_r) {
+__device__ __forceinline__ void theta_forward(const double *a, double *ring, double s, int srow, int rrow, SVal &th_s,
+                                              SVal &th_r) {
     th_s = SVal{1.0, 0};
     th_r = SVal{1.0, 0};
     const int top = srow > rrow ? srow : rrow;
@@ -398,7 +402,7 @@ __device__ __forceinline__ void theta_forward(const double *a, double *ring, dou
     sweep_up(
         a, ring, top,
         [&](double ak, int k) {
-            const double t0 = fma(ak - s, t1, -b2 * t2);
+            const double t0 = fma(ak - s, t1, -t2);
             t2 = t1;
             t1 = t0;
             if (k == srow - 1) th_s = SVal{t1, ex};
@@ -412,14 +416,12 @@ __device__ __forceinline__ void theta_forward(const double *a, double *ring, dou
         });
 }
 
-// u_lo u_hi = -(-1)^(hi-lo) b^(hi-lo) theta_{lo-1} phi_{hi+1} / q_0
-__device__ __forceinline__ double adjugate_product(SVal th, SVal ph, double q0, int qex, int dist,
-                                                   double log2b) {
+// u_lo u_hi = -(-1)^(hi-lo) theta_{lo-1} phi_{hi+1} / q_0
+__device__ __forceinline__ double adjugate_product(SVal th, SVal ph, double q0, int qex, int dist) {
     if (th.v == 0.0 || ph.v == 0.0 || q0 == 0.0) return 0.0;
     const double sgn = ((dist & 1) ? 1.0 : -1.0) * copysign(1.0, th.v) * copysign(1.0, ph.v) *
                        copysign(1.0, q0);
-    const double l2 = log2(fabs(th.v)) + th.e + log2(fabs(ph.v)) + ph.e - log2(fabs(q0)) - qex +
-                      dist * log2b;
+    const double l2 = log2(fabs(th.v)) + th.e + log2(fabs(ph.v)) + ph.e - log2(fabs(q0)) - qex;
     return sgn * exp2(l2);
 }
 
