@@ -376,22 +376,39 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
         if (F16 && i < nr_pad) sc = ldexp(sc, SB[(size_t)t.p * nr_pad + i]);
         s_sc[threadIdx.x] = sc;
     }
-    // the thread's ranges are i0 + w + 8q: e^{i k r} exact at q = 0, then advanced by the
-    // exact factor e^{i k 8 dr} (a complex multiply instead of a sincos and an exp)
-    double zr = 0.0, zi = 0.0, mr = 0.0, mi = 0.0;
-    if (live) {
-        const double r0 = (i0 + (threadIdx.x >> 5) + 1) * dr;
-        double sn, cs;
-        sincos(f2_reduce(kr * r0), &sn, &cs);
-        const double a0 = exp(-ki * r0);
-        zr = a0 * cs;
-        zi = a0 * sn;
-        sincos(f2_reduce(kr * (8.0 * dr)), &sn, &cs);
-        const double a8 = exp(-ki * (8.0 * dr));
-        mr = a8 * cs;
-        mi = a8 * sn;
+    // The thread's ranges are i0 + w + 8q (w = tid / 32).  Threads 0-31 form, for their mode,
+    // e^{i k r} exact at the tile's first range (FP64 phase reduced mod 2 pi, sincos, exp)
+    // and the exact factor e^{i k dr}; its powers give the 8 starting phasors (w = 0..7) and
+    // the step e^{i k 8 dr}, shared through LDS: one sincos/exp pair per mode and tile.
+    __shared__ double s_z[8][32][2], s_m8[32][2];
+    if (threadIdx.x < 32) {
+        double ar = 0.0, ai = 0.0, fr = 0.0, fi = 0.0;
+        if (live) {
+            const double r0 = (i0 + 1) * dr;
+            double sn, cs;
+            sincos(f2_reduce(kr * r0), &sn, &cs);
+            const double a0 = exp(-ki * r0);
+            ar = a0 * cs;
+            ai = a0 * sn;
+            sincos(f2_reduce(kr * dr), &sn, &cs);
+            const double a1 = exp(-ki * dr);
+            fr = a1 * cs;
+            fi = a1 * sn;
+        }
+        double pr = 1.0, pi = 0.0;   // e^{i k w dr}
+        for (int w = 0; w < 8; ++w) {
+            s_z[w][mloc][0] = ar * pr - ai * pi;
+            s_z[w][mloc][1] = ar * pi + ai * pr;
+            const double t = pr * fr - pi * fi;
+            pi = pr * fi + pi * fr;
+            pr = t;
+        }
+        s_m8[mloc][0] = pr;
+        s_m8[mloc][1] = pi;
     }
     __syncthreads();
+    double zr = s_z[threadIdx.x >> 5][mloc][0], zi = s_z[threadIdx.x >> 5][mloc][1];
+    const double mr = s_m8[mloc][0], mi = s_m8[mloc][1];
     for (int q = 0; q < 8; ++q) {
         const int il = (threadIdx.x >> 5) + 8 * q;   // range within the tile, 0..63
         const int i = i0 + il;
