@@ -26,10 +26,18 @@ __host__ __device__ inline double sgn1(double v) { return v < 0.0 ? -1.0 : 1.0; 
 // YF(k) returns the node value y_k.  Outside [z0, z_{n-1}] the end value is returned,
 // as AtmosphericProperty1D::get does (src/atmosphere/AtmosphericProperty1D.cpp:116-126).
 template <class YF>
+__host__ __device__ inline double steffen_eval_at(const double *z, int n, YF yf, double x, int k);
+
+template <class YF>
 __host__ __device__ inline double steffen_eval(const double *z, int n, YF yf, double x) {
     if (x > z[n - 1]) return yf(n - 1);
     if (x < z[0]) return yf(0);
-    const int k = bsearch_interval(z, n, x);
+    return steffen_eval_at(z, n, yf, x, bsearch_interval(z, n, x));
+}
+
+// the same, on interval k = bsearch_interval(z, n, x) found by the caller (x inside the nodes)
+template <class YF>
+__host__ __device__ inline double steffen_eval_at(const double *z, int n, YF yf, double x, int k) {
     const double hk = z[k + 1] - z[k];
     const double yk = yf(k), yk1 = yf(k + 1);
     const double sk = (yk1 - yk) / hk;

@@ -147,9 +147,28 @@ struct ArrNode {
     __device__ double operator()(int k) const { return y[k]; }
 };
 
+// c_eff at the profile nodes of problem p (the node-level _CEFF_ the reference splines,
+// ESSModeSolver.cpp:545): grid (ceil(ldn / 256), P)
+__global__ void k_ceff_nodes(const ProbDev *__restrict__ probs, int ldn, double *__restrict__ out) {
+    const int p = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const ProbDev pr = probs[p];
+    if (k >= pr.n_nodes) return;
+    const int n = pr.n_nodes;
+    out[size_t(p) * ldn + k] = CeffNode{pr.prof + PF_C0 * n, pr.prof + PF_WS * n, pr.prof + PF_WD * n, pr.az_rad}(k);
+}
+
+// spline value at z: the end value outside the nodes (AtmosphericProperty1D.cpp:116-126),
+// else Steffen on the interval k the caller found once for all properties
+__device__ __forceinline__ double spline_at(const double *zn, int n, const double *y, double z, int k) {
+    if (z > zn[n - 1]) return y[n - 1];
+    if (z < zn[0]) return y[0];
+    return steffen_eval_at(zn, n, ArrNode{y}, z, k);
+}
+
 __global__ void k_setup_grid(const ProbDev *__restrict__ probs, int Nz, double z_min, double dz,
-                             const double *__restrict__ alpha_nodes, int ldn,
-                             double *__restrict__ A, double *__restrict__ W,
+                             const double *__restrict__ alpha_nodes, const double *__restrict__ ceff_nodes,
+                             int ldn, double *__restrict__ A, double *__restrict__ W,
                              double *__restrict__ CE) {
     const int p = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -157,10 +176,11 @@ __global__ void k_setup_grid(const ProbDev *__restrict__ probs, int Nz, double z
     const ProbDev pr = probs[p];
     const int n = pr.n_nodes;
     const double *zn = pr.prof + PF_Z * n;
-    CeffNode ce{pr.prof + PF_C0 * n, pr.prof + PF_WS * n, pr.prof + PF_WD * n, pr.az_rad};
+    const double *cn = ceff_nodes + size_t(p) * ldn;
     const double h2 = dz * dz;
     const double z = z_min + i * dz;
-    const double ceff = steffen_eval(zn, n, ce, z);
+    const int k = bsearch_interval(zn, n, z);   // one interval for every property of the row
+    const double ceff = spline_at(zn, n, cn, z, k);
     // row of dz^2 T: -2 + dz^2 diag, with diag = (omega/c_eff)^2 + the ground term on row 0
     // (ESSModeSolver.cpp:544-559, EigenEngine.cpp:57-79)
     double diag = h2 * ((pr.omega / ceff) * (pr.omega / ceff));
@@ -169,14 +189,14 @@ __global__ void k_setup_grid(const ProbDev *__restrict__ probs, int Nz, double z
     if (i == Nz - 1) {
         // last row takes diag[Nz-2] (stale value[1], EigenEngine.cpp:57-71)
         const double zp = z_min + (Nz - 2) * dz;
-        const double cp = steffen_eval(zn, n, ce, zp);
+        const double cp = spline_at(zn, n, cn, zp, bsearch_interval(zn, n, zp));
         double dp = h2 * ((pr.omega / cp) * (pr.omega / cp));
         if (Nz - 2 == 0) dp = 1.0 / (dz * pr.admittance + 1) + dp;
         a = -2.0 + dp;
     }
-    const double alpha = steffen_eval(zn, n, ArrNode{alpha_nodes + size_t(p) * ldn}, z);
-    const double Pz = steffen_eval(zn, n, ArrNode{pr.prof + PF_P * n}, z);
-    const double Rz = steffen_eval(zn, n, ArrNode{pr.prof + PF_RHO * n}, z);
+    const double alpha = spline_at(zn, n, alpha_nodes + size_t(p) * ldn, z, k);
+    const double Pz = spline_at(zn, n, pr.prof + PF_P * n, z, k);
+    const double Rz = spline_at(zn, n, pr.prof + PF_RHO * n, z, k);
     const double cT = sqrt(1.4 * Pz / Rz);
     const size_t o = size_t(p) * Nz + i;
     A[o] = a;
@@ -1017,13 +1037,20 @@ void launch_plan_build(hipStream_t st, const ProbDev *probs, const IntervalOut *
                        n_sets, n_freq, mp, offsets, work, mbeg, mcnt, meta);
 }
 
-void launch_solve_front(hipStream_t st, const ProbDev *probs, int P, int max_nodes, const ncpa_grid_t &grid,
-                        double *d_alpha, double *A, double *W, double *CE, IntervalOut *iv) {
+void launch_setup(hipStream_t st, const ProbDev *probs, int P, int max_nodes, const ncpa_grid_t &grid,
+                  double *d_nodes, double *A, double *W, double *CE) {
     dim3 ga((max_nodes + 255) / 256, P);
-    hipLaunchKernelGGL(k_alpha_nodes, ga, dim3(256), 0, st, probs, max_nodes, d_alpha);
+    double *ceff_nodes = d_nodes + (size_t)P * max_nodes;
+    hipLaunchKernelGGL(k_alpha_nodes, ga, dim3(256), 0, st, probs, max_nodes, d_nodes);
+    hipLaunchKernelGGL(k_ceff_nodes, ga, dim3(256), 0, st, probs, max_nodes, ceff_nodes);
     dim3 gs((grid.Nz + 255) / 256, P);
-    hipLaunchKernelGGL(k_setup_grid, gs, dim3(256), 0, st, probs, grid.Nz, grid.z_min, grid.dz, d_alpha, max_nodes,
-                       A, W, CE);
+    hipLaunchKernelGGL(k_setup_grid, gs, dim3(256), 0, st, probs, grid.Nz, grid.z_min, grid.dz, d_nodes, ceff_nodes,
+                       max_nodes, A, W, CE);
+}
+
+void launch_solve_front(hipStream_t st, const ProbDev *probs, int P, int max_nodes, const ncpa_grid_t &grid,
+                        double *d_nodes, double *A, double *W, double *CE, IntervalOut *iv) {
+    launch_setup(st, probs, P, max_nodes, grid, d_nodes, A, W, CE);
     hipLaunchKernelGGL(k_interval, dim3(P), dim3(256), 0, st, probs, grid.Nz, grid.dz, A, CE, iv);
 }
 }  // namespace ncpa
@@ -1265,7 +1292,7 @@ int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_
     m->P = P;
     m->grid = *grid;
     ProbDev *d_probs = pool_alloc<ProbDev>(ctx, P);
-    double *d_alpha = pool_alloc<double>(ctx, size_t(P) * max_nodes);
+    double *d_alpha = pool_alloc<double>(ctx, 2 * size_t(P) * max_nodes);   // alpha and c_eff nodes
     m->d_A = pool_alloc<double>(ctx, size_t(P) * Nz);
     m->d_W = pool_alloc<double>(ctx, size_t(P) * Nz);
     m->d_CE = pool_alloc<double>(ctx, size_t(P) * Nz);
@@ -1284,11 +1311,7 @@ int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_
     if (e != hipSuccess) return fail(hip_error(ctx, e, "copy problems"));
     {
         TimedLaunch tl(ctx, "setup");
-        dim3 ga((max_nodes + 255) / 256, P);
-        hipLaunchKernelGGL(k_alpha_nodes, ga, dim3(256), 0, ctx->stream, d_probs, max_nodes, d_alpha);
-        dim3 gs((Nz + 255) / 256, P);
-        hipLaunchKernelGGL(k_setup_grid, gs, dim3(256), 0, ctx->stream, d_probs, Nz, grid->z_min, dz,
-                           d_alpha, max_nodes, m->d_A, m->d_W, m->d_CE);
+        launch_setup(ctx->stream, d_probs, P, max_nodes, *grid, d_alpha, m->d_A, m->d_W, m->d_CE);
     }
     {
         TimedLaunch tl(ctx, "interval");

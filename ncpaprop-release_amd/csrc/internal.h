@@ -70,9 +70,11 @@ struct PlanMeta {
     int queue[NCPA_XCDS];   // per-XCD work counters of the persistent launch (reset here)
 };
 
-// ess.hip
+// ess.hip.  d_nodes: 2 P max_nodes doubles of scratch (attenuation and c_eff at the nodes)
+void launch_setup(hipStream_t st, const ProbDev *probs, int P, int max_nodes, const ncpa_grid_t &grid,
+                  double *d_nodes, double *A, double *W, double *CE);
 void launch_solve_front(hipStream_t st, const ProbDev *probs, int P, int max_nodes, const ncpa_grid_t &grid,
-                        double *d_alpha, double *A, double *W, double *CE, IntervalOut *iv);
+                        double *d_nodes, double *A, double *W, double *CE, IntervalOut *iv);
 void launch_plan_build(hipStream_t st, const ProbDev *probs, const IntervalOut *iv, int P, int Nz, double dz,
                        const double *A, const double *W, int cap_modes, int cap_work, int n_sets, int n_freq,
                        ModeProb *mp, int *offsets, ModeWork *work, int *mbeg, int *mcnt, PlanMeta *meta);

@@ -221,7 +221,7 @@ int ncpa_bb_plan_create(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_pro
         return code == NCPA_OK ? code : fail(code);
     };
     // per-problem buffers, then a count-only pass (capacity 0) to size the mode buffers
-    if ((rc = dev_alloc(pl, &pl->probs, P)) || (rc = dev_alloc(pl, &pl->alpha, (size_t)P * pl->max_nodes)) ||
+    if ((rc = dev_alloc(pl, &pl->probs, P)) || (rc = dev_alloc(pl, &pl->alpha, 2 * (size_t)P * pl->max_nodes)) ||
         (rc = dev_alloc(pl, &pl->A, (size_t)P * Nz)) || (rc = dev_alloc(pl, &pl->W, (size_t)P * Nz)) ||
         (rc = dev_alloc(pl, &pl->CE, (size_t)P * Nz)) || (rc = dev_alloc(pl, &pl->iv, P)) ||
         (rc = dev_alloc(pl, &pl->mp, P)) || (rc = dev_alloc(pl, &pl->offsets, (size_t)P + 1)) ||
