@@ -457,19 +457,19 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
                                                        const unsigned short *__restrict__ Bhat,
                                                        size_t split_stride_a, size_t split_stride_b,
                                                        float *__restrict__ C, size_t ldc, int M, int N,
-                                                       int n_tiles_n, const int *__restrict__ SA, int Mp,
+                                                       int tiles_m, int n_tiles_n, const int *__restrict__ SA, int Mp,
                                                        const int *__restrict__ SB, int nr_pad) {
     __shared__ __attribute__((aligned(16))) unsigned short As[S][TBM][TLD];
     __shared__ __attribute__((aligned(16))) unsigned short Bs[S][TBN][TLD];
     const SplitProb P = sp[blockIdx.z];
-    // Blocks are dealt to the 8 XCDs round-robin (id % 8).  All M tiles of one N tile are
-    // kept on one XCD, one after another, so its Bhat tile is fetched into one L2 only.
-    const int n_tiles_n8 = (n_tiles_n + 7) & ~7;
-    const int tiles_m = gridDim.x / n_tiles_n8;
+    // Blocks are dealt to the 8 XCDs round-robin (id % 8).  XCD x takes M-tile half x % 2 and
+    // every fourth N tile from x / 2; on the XCD a B tile is used by the half's M tiles one
+    // after another, so an L2 holds half of the problem's A and the B tiles in flight.
+    const int tmh = (tiles_m + 1) / 2;                     // M tiles per half
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int tm = slot % tiles_m;
-    const int tn = (slot / tiles_m) * 8 + xcd;
-    if (tn >= n_tiles_n) return;
+    const int tm = (xcd & 1) * tmh + slot % tmh;
+    const int tn = (slot / tmh) * 4 + (xcd >> 1);
+    if (tn >= n_tiles_n || tm >= tiles_m) return;
     const int m0 = tm * TBM, n0 = tn * TBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
@@ -636,9 +636,10 @@ int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *h
     {
         TimedLaunch tl(ctx, "field2d");
         const int tn = Np / TBN, tm = Mp / TBM;
-        dim3 g((unsigned)(tm * ((tn + 7) & ~7)), 1, (unsigned)P);
+        // 8 XCDs x ceil(tm / 2) M tiles x ceil(tn / 4) N tiles per XCD
+        dim3 g((unsigned)(8 * ((tm + 1) / 2) * ((tn + 3) / 4)), 1, (unsigned)P);
         hipLaunchKernelGGL((k_gemm_split<S, NP, F16>), g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb, out,
-                           (size_t)N, M, N, tn, SA, Mp, SB, nr_pad);
+                           (size_t)N, M, N, tm, tn, SA, Mp, SB, nr_pad);
     }
     e = hipGetLastError();
     cleanup();
