@@ -222,14 +222,22 @@ __host__ __device__ __forceinline__ size_t tiled_index(int row, int k, int Kp) {
 // One 16-bit part of a split: the part's bits and its value.  BF16 (8-bit significand,
 // FP32 range) or FP16 (11-bit significand; operands are pre-scaled by powers of two so
 // the largest magnitudes sit at 2^14).
+// The conversion goes through FP32: two hardware conversions, where a direct FP64 -> FP16
+// rounding is a long integer sequence on gfx950 (the empty asm keeps the compiler from
+// merging the two back into it).  The rare double rounding is harmless: the caller
+// subtracts the value actually stored, so the parts still sum to the input.
 template <bool F16>
 __device__ __forceinline__ unsigned short split_part(double v, double &back) {
     if (F16) {
-        const _Float16 h = (_Float16)v;
-        back = (double)h;
+        float f = (float)v;
+        asm volatile("" : "+v"(f));
+        const _Float16 h = (_Float16)f;
+        back = (double)(float)h;
         return __builtin_bit_cast(unsigned short, h);
     }
-    const __bf16 h = (__bf16)(float)v;
+    float f = (float)v;
+    asm volatile("" : "+v"(f));
+    const __bf16 h = (__bf16)f;
     back = (double)(float)h;
     return __builtin_bit_cast(unsigned short, h);
 }
