@@ -771,48 +771,50 @@ __device__ __forceinline__ double log2abs_prod(double t, int te, double p, int p
 
 // Per mode: the reference row r (largest u_r^2 = -theta_{r-1} phi_{r+1} / q_0 among the
 // recorded rows), log2 u_r^2, the sign flip that makes u(source row) >= 0, and psi(zs).
-// Block = 4 waves x 64 modes; wave w scans slots w, w+4, ... and the 4 partial maxima are
-// merged in LDS.
-__global__ void __launch_bounds__(256) k_rows_ref(const double *__restrict__ TH, const int *__restrict__ THE,
-                                                  const double *__restrict__ PH, const int *__restrict__ PHE,
-                                                  const double *__restrict__ Q0, const int *__restrict__ QEX,
-                                                  const int *__restrict__ mode_prob,
-                                                  const ModeProb *__restrict__ mp, RowsArgs ra, size_t n_modes,
-                                                  int *__restrict__ ref_slot, double *__restrict__ ref_l2,
-                                                  double *__restrict__ flip_out, double *__restrict__ psi_src) {
-    __shared__ double s_l2[4][64];
-    __shared__ int s_k[4][64];
+// Block = 16 waves x 64 modes; wave w scans slots w, w+16, ...  Candidates are compared by
+// the exponent and mantissa of |theta phi| 2^(te + pe) (no log per slot); the 16 partial
+// maxima are merged in LDS.
+#define NCPA_REF_WAVES 16
+__global__ void __launch_bounds__(64 * NCPA_REF_WAVES) k_rows_ref(
+        const double *__restrict__ TH, const int *__restrict__ THE, const double *__restrict__ PH,
+        const int *__restrict__ PHE, const double *__restrict__ Q0, const int *__restrict__ QEX,
+        const int *__restrict__ mode_prob, const ModeProb *__restrict__ mp, RowsArgs ra, size_t n_modes,
+        int *__restrict__ ref_slot, double *__restrict__ ref_l2, double *__restrict__ flip_out,
+        double *__restrict__ psi_src) {
+    __shared__ int s_e[NCPA_REF_WAVES][64], s_k[NCPA_REF_WAVES][64];
+    __shared__ double s_m[NCPA_REF_WAVES][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t o = blockIdx.x * (size_t)64 + lane;
     const bool live = o < n_modes;
-    double l2r = -INFINITY;
-    int r = -1;
+    int be = INT_MIN, r = -1;   // best exponent, slot
+    double bm = 0.0;            // best mantissa in [0.5, 1)
     if (live) {
-        const double q0 = Q0[o];
-        const double lq = log2abs_s(q0, QEX[o]);
-        const double sq = copysign(1.0, q0);
-        for (int k = wave; k < ra.nrec; k += 4) {
+        const double sq = copysign(1.0, Q0[o]);
+        for (int k = wave; k < ra.nrec; k += NCPA_REF_WAVES) {
             const double t = TH[(size_t)k * ra.ld + o], p = PH[(size_t)k * ra.ld + o];
             if (t == 0.0 || p == 0.0) continue;
             if (-copysign(1.0, t) * copysign(1.0, p) * sq <= 0.0) continue;
-            const double l2 = log2abs_prod(t, THE[(size_t)k * ra.ld + o], p, PHE[(size_t)k * ra.ld + o]) - lq;
-            if (l2 > l2r) { l2r = l2; r = k; }
+            int e;
+            const double m = frexp(fabs(t * p), &e);
+            e += THE[(size_t)k * ra.ld + o] + PHE[(size_t)k * ra.ld + o];
+            if (e > be || (e == be && m > bm)) { be = e; bm = m; r = k; }
         }
     }
-    s_l2[wave][lane] = l2r;
+    s_e[wave][lane] = be;
+    s_m[wave][lane] = bm;
     s_k[wave][lane] = r;
     __syncthreads();
     if (wave != 0 || !live) return;
-    for (int w = 1; w < 4; ++w) {   // ties go to the lowest slot, as a sequential scan would
-        const double l2 = s_l2[w][lane];
-        const int k = s_k[w][lane];
-        if (k >= 0 && (l2 > l2r || (l2 == l2r && k < r))) { l2r = l2; r = k; }
+    for (int w = 1; w < NCPA_REF_WAVES; ++w) {   // ties go to the lowest slot, as a sequential scan would
+        const int e = s_e[w][lane], k = s_k[w][lane];
+        const double m = s_m[w][lane];
+        if (k >= 0 && (r < 0 || e > be || (e == be && (m > bm || (m == bm && k < r))))) { be = e; bm = m; r = k; }
     }
     const ModeProb P = mp[mode_prob[o]];
     const double inv_sqrt_dz = sqrt(P.inv_dz);
     ref_slot[o] = r;
-    ref_l2[o] = l2r;
     if (r < 0) {
+        ref_l2[o] = -INFINITY;
         flip_out[o] = 0.0;
         psi_src[o] = 0.0;
         return;
@@ -820,6 +822,8 @@ __global__ void __launch_bounds__(256) k_rows_ref(const double *__restrict__ TH,
     const double q0 = Q0[o];
     const double lq = log2abs_s(q0, QEX[o]);
     const double sq = copysign(1.0, q0);
+    const double l2r = log2(bm) + (double)be - lq;
+    ref_l2[o] = l2r;
     const int row_r = ra.row0 + r * ra.stride;
     const size_t ro = (size_t)r * ra.ld + o, so = (size_t)ra.nrec * ra.ld + o;
     // u(source row) u_r / u_r with the adjugate product of the sorted pair
@@ -1213,7 +1217,7 @@ int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, do
         hipLaunchKernelGGL(k_rows, dim3((unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
                            m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
         const unsigned mb = (unsigned)((ld + 63) / 64);
-        hipLaunchKernelGGL(k_rows_ref, dim3(mb), dim3(256), 0, ctx->stream, TH, THE, PH, PHE, Q0, QEX,
+        hipLaunchKernelGGL(k_rows_ref, dim3(mb), dim3(64 * NCPA_REF_WAVES), 0, ctx->stream, TH, THE, PH, PHE, Q0, QEX,
                            m->d_mode_prob, m->d_mp, ra, ld, RS, RL, FL, m->d_psis_rows);
         const unsigned sb = (unsigned)((nrec + 4 * NCPA_ROWS_PER_THREAD - 1) / (4 * NCPA_ROWS_PER_THREAD));
         hipLaunchKernelGGL(k_rows_combine, dim3(mb, sb), dim3(256), 0, ctx->stream, TH, THE, PH, PHE, Q0, QEX,
