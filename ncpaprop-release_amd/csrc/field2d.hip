@@ -212,7 +212,11 @@ struct SplitProb {
     int K, Kp;
 };
 
-constexpr int TBM = 128, TBN = 128, TKS = 32, TLD = TKS + 8;   // LDS row: 40 halves = 80 B
+// LDS rows of a k-step are 32 halves (64 B) with the four 16-byte groups of row r stored at
+// group g ^ ((r >> 2) & 3): a 16-lane write of 4 rows x 4 groups and a 16-lane fragment read
+// of 16 rows x 1 group both touch 16 distinct 4-bank quads of the 64 banks.
+constexpr int TBM = 128, TBN = 128, TKS = 32, TLD = TKS;
+__device__ __forceinline__ int lds_group(int row, int g) { return 8 * (g ^ ((row >> 2) & 3)); }
 
 // element offset of (row, k) in the tiled [row/128][k/32][row%128][k%32] layout
 __host__ __device__ __forceinline__ size_t tiled_index(int row, int k, int Kp) {
@@ -511,8 +515,8 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
             for (int h = 0; h < 2; ++h) {
                 const int v = tid + 256 * h;
                 const int row = v >> 2, part = v & 3;
-                *(u4 *)&As[s][row][part * 8] = ra[s][h];
-                *(u4 *)&Bs[s][row][part * 8] = rb[s][h];
+                *(u4 *)&As[s][row][lds_group(row, part)] = ra[s][h];
+                *(u4 *)&Bs[s][row][lds_group(row, part)] = rb[s][h];
             }
     };
     fetch(0);
@@ -521,7 +525,8 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
         stash();
         __syncthreads();
         if (k0 + TKS < Kp) fetch(k0 + TKS);
-        const int kq = 8 * (lane >> 4);
+        // rows wm + 16a + (lane & 15): the swizzle of k-group lane >> 4 depends on lane only
+        const int kq = lds_group(lane & 15, lane >> 4);
         u4 af[S][4], bfr[S][4];
 #pragma unroll
         for (int s = 0; s < S; ++s)
