@@ -14,12 +14,15 @@ The steps are timed twice: on one device stream (the stage split and the rooflin
 from this region, each kernel alone on the GPU) and with independent batches in flight on
 --streams device streams (default 3, one host thread each), which gives the headline value.
 
+--workload precision: the 2-D field at 2 Hz, 1500 x 10000, in every contraction precision
+against FP64 (time, speed-up, max/RMS dB error).
+
 --workload broadband: one step = a batched ModBB job: dispersion solves at 1024 frequencies
 (f_min = f_step = 2/1024 Hz) x 8 azimuths on the seed-0 profile, then transfer functions at
 16 receiver ranges (200-1000 km) and pulse synthesis (NFFT = max(4096, 4 Nfreq) = 4100, the
 reference's rule) with a Gaussian-windowed 0.5 Hz source given as a spectrum.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload field|broadband|serving]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload field|broadband|serving|precision]
                     [--streams S] [--no-cpu]
 
 Prints one JSON line.  cpu_baseline: the FP64 oracle (scipy LAPACK bisection + inverse
@@ -50,13 +53,16 @@ MAXRANGE_M = 1000e3
 FIELD_ROWS, FIELD_STRIDE = 1500, 13        # 1500 depths, 0 - 146 km AGL
 FLOPS_PER_STURM_STEP = 3.0                 # one row of one sweep
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
-PREC_NAMES = {0: "f64", 1: "f32", 2: "bf16x3", 3: "bf16x6", 4: "fp16x3", 5: "fp16x4"}
-PREC_PEAK = {0: "f64", 1: "f32", 2: "bf16", 3: "bf16", 4: "bf16", 5: "bf16"}   # f16 MFMA rate = bf16
-# split-precision kernel (parts, products, FP16) per precision
-PREC_KERNEL = {2: "k_gemm_split<2, 3, false>", 3: "k_gemm_split<3, 6, false>", 4: "k_gemm_split<2, 3, true>",
-               5: "k_gemm_split<2, 4, true>"}
-# candidate field precisions, fastest first (3 products < 4 products < 6 products < FP32 MFMA)
-CANDIDATES = [2, 4, 5, 3, 1]
+PREC_NAMES = {0: "f64", 1: "f32", 2: "bf16x3", 3: "bf16x6", 4: "fp16x3", 5: "fp16x4", 6: "bf16x6f",
+              7: "fp16x4f"}
+PREC_PEAK = {0: "f64", 1: "f32", 2: "bf16", 3: "bf16", 4: "bf16", 5: "bf16", 6: "bf16", 7: "bf16"}   # f16 = bf16
+# split-precision kernel (parts, products, FP16, FP64 fold) per precision
+PREC_KERNEL = {2: "k_gemm_split<2, 3, false, false>", 3: "k_gemm_split<3, 6, false, false>",
+               4: "k_gemm_split<2, 3, true, false>", 5: "k_gemm_split<2, 4, true, false>",
+               6: "k_gemm_split<3, 6, false, true>", 7: "k_gemm_split<2, 4, true, true>"}
+PRODUCTS = {0: 1, 1: 1, 2: 3, 3: 6, 4: 3, 5: 4, 6: 6, 7: 4}
+# candidate field precisions, fastest first
+CANDIDATES = [2, 4, 5, 7, 3, 6, 1]
 TL_GATE_DB = 0.1
 
 BB_NF = 1024
@@ -179,24 +185,27 @@ def cpu_cores():
 # ------------------------------------------------------------------------------------------
 # GPU legs
 # ------------------------------------------------------------------------------------------
-def field_error_db(ctx, grid, problems, prec):
-    """Max |TL_prec - TL_fp64| [dB] over TL > -140 dB for a few problems (untimed check)."""
-    import numpy as np
+def field_gate(ctx, grid, problems):
+    """Max / RMS |TL_prec - TL_fp64| [dB] over TL > -140 dB at every point of every problem of
+    the step, for every candidate precision (compared on the device, untimed); returns the
+    errors and the first candidate within TL_GATE_DB (0 = FP64 if none is)."""
     m = ctx.solve(grid, problems)
     rows = m.rows_device(0, FIELD_STRIDE, FIELD_ROWS)
     dr = MAXRANGE_M / NR
     o64 = m.field2d_device(rows, FIELD_ROWS, dr, NR, 0)
-    oc = m.field2d_device(rows, FIELD_ROWS, dr, NR, prec)
-    a = o64.numpy().reshape(-1, 2)
-    b = oc.numpy().astype(np.float64).reshape(-1, 2)
-    for x in (o64, oc, rows):
+    n = len(problems) * FIELD_ROWS * NR
+    errs, chosen = {}, 0
+    for prec in CANDIDATES:
+        oc = m.field2d_device(rows, FIELD_ROWS, dr, NR, prec)
+        mx, rms, cnt = ctx.field_compare(o64, oc, prec, n)
+        oc.free()
+        errs[prec] = (mx, rms, cnt)
+        if not chosen and mx <= TL_GATE_DB:
+            chosen = prec
+    for x in (o64, rows):
         x.free()
     m.free()
-    tl0 = 10.0 * np.log10(np.maximum(a[:, 0] ** 2 + a[:, 1] ** 2, 1e-300))
-    tl1 = 10.0 * np.log10(np.maximum(b[:, 0] ** 2 + b[:, 1] ** 2, 1e-300))
-    mask = tl0 > -140.0
-    d = np.abs(tl1 - tl0)[mask]
-    return float(d.max()), float(np.sqrt(np.mean(d ** 2)))
+    return errs, chosen
 
 
 def run_field(lanes, args, prec):
@@ -321,14 +330,8 @@ def field_main(args, cpu):
     grid = ctx.grid(pids[0], Nz=NZ)
     problems = [ncpamodes.problem(pid, FREQ, az) for pid in pids for az in AZIMUTHS]
     P = len(problems)
-    # precision gate on two solves (different profiles and azimuths), before any timing
-    check = [problems[2], problems[8 * 5 + 6]]
-    errs, chosen = {}, 0
-    for prec in CANDIDATES:
-        errs[prec] = field_error_db(ctx, grid, check, prec)
-        if errs[prec][0] <= TL_GATE_DB:
-            chosen = prec
-            break
+    # precision gate over every point of all 64 solves, before any timing
+    errs, chosen = field_gate(ctx, grid, problems)
     lanes = [(ctx, problems, grid)]
     for _ in range(args.streams - 1):
         c2 = ncpamodes.Context(0)
@@ -393,7 +396,8 @@ def field_main(args, cpu):
             "device_streams": multi["streams"],
             "parallelism": "single GPU",
         },
-        "tl_check_db": {PREC_NAMES[p]: {"max": e[0], "rms": e[1]} for p, e in errs.items()},
+        "tl_check_db": {PREC_NAMES[p]: {"max": e[0], "rms": e[1], "points": e[2]} for p, e in errs.items()},
+        "tl_check_scope": "all %d solves of the step, every field point with TL_fp64 > -140 dB" % P,
         "fp64_plain": {"value": P * args.steps / plain["elapsed"], "unit": "solves/s",
                        "ms_per_step": plain["elapsed"] / args.steps * 1e3},
         "single_stream": {"value": P * args.steps / res["elapsed"], "unit": "solves/s",
@@ -504,13 +508,81 @@ def broadband_main(args, cpu):
     return line
 
 
+def precision_main(args):
+    """Config 3: the 2-D field at 2 Hz on 1500 depths x 10000 ranges, every contraction precision
+    against the FP64 contraction of the same modes: time per launch (HIP events), speed-up over
+    FP64, max / RMS |TL - TL_fp64| over TL > -140 dB, and the algorithmic and MFMA-work rates."""
+    import numpy as np
+    import ncpamodes
+    tmp = tempfile.mkdtemp(prefix="ncpa_bench_")
+    paths = write_profiles(tmp, [0, 1])
+    ctx = ncpamodes.Context(0)
+    pids = [ctx.read_profile(p)[0] for p in paths]
+    grid = ctx.grid(pids[0], Nz=NZ)
+    problems = [ncpamodes.problem(pids[0], 2.0, 90.0), ncpamodes.problem(pids[1], 2.0, 225.0)]
+    m = ctx.solve(grid, problems)
+    rows = m.rows_device(0, FIELD_STRIDE, FIELD_ROWS)
+    dr = MAXRANGE_M / NR
+    P = len(problems)
+    times, res = {}, {}
+    flops = 4.0 * FIELD_ROWS * m.total * NR
+    n = P * FIELD_ROWS * NR
+    o64 = m.field2d_device(rows, FIELD_ROWS, dr, NR, 0)
+    for prec in (0, 1, 6, 3, 7, 5, 4, 2):
+        out = ctx.empty(P * FIELD_ROWS * 2 * NR, np.float64 if prec == 0 else np.float32)
+        for _ in range(max(args.warmup, 1)):
+            m.field2d_device(rows, FIELD_ROWS, dr, NR, prec, out=out)
+        ctx.synchronize()
+        ctx.set_timing(True)
+        ctx.reset_timing()
+        for _ in range(args.steps):
+            m.field2d_device(rows, FIELD_ROWS, dr, NR, prec, out=out)
+        ctx.synchronize()
+        ms, n_l = ctx.kernel_time("field2d")
+        ctx.set_timing(False)
+        t = ms / max(n_l, 1)
+        times[prec] = t
+        mx, rms, _ = ctx.field_compare(o64, out, prec, n)
+        out.free()
+        res[PREC_NAMES[prec]] = {
+            "ms": t, "speedup_vs_fp64": times[0] / t, "max_db": mx, "rms_db": rms,
+            "algorithmic_tflops": flops / (t / 1e3) / 1e12,
+            "mfma_work_tflops": flops * PRODUCTS[prec] / (t / 1e3) / 1e12,
+            "peak_tflops": PEAK_TFLOPS[PREC_PEAK[prec]]}
+    total = m.total
+    o64.free()
+    rows.free()
+    m.free()
+    ctx.close()
+    best = min((p for p in times if p != 0 and res[PREC_NAMES[p]]["max_db"] <= TL_GATE_DB), key=lambda p: times[p])
+    t = times[best]
+    return {
+        "metric": METRIC + " (mixed-precision 2-D field)",
+        "value": P / (t / 1e3), "unit": "2-D fields/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+%s" % PREC_NAMES[best], "data": "synthetic",
+        "config": {"workload": "2-D field at 2 Hz, 1500 depths x 10000 ranges (150 km x 1000 km), Nz=20000, "
+                               "2 problems (seeds 0, 1; azimuths 90, 225), %d modes; one step = one contraction "
+                               "launch of both fields in the fastest precision within 0.1 dB of FP64" % total,
+                   "modes": int(total), "parallelism": "single GPU"},
+        "precisions": res,
+        "roofline": {"kernel": "field contraction (%s)" % PREC_NAMES[best], "bound": "mfma",
+                     "achieved": res[PREC_NAMES[best]]["algorithmic_tflops"], "peak": PEAK_TFLOPS[PREC_PEAK[best]],
+                     "unit": "TFLOP/s", "frac": res[PREC_NAMES[best]]["algorithmic_tflops"] / PEAK_TFLOPS[PREC_PEAK[best]],
+                     "traffic": None,
+                     "algorithmic": "4 x depths x modes x ranges real flops per problem (real x complex)"},
+        "cpu_baseline": None,
+    }
+
+
 def serving_main(args):
     """Config 4: Poisson request streams through the continuous-batching server."""
     import numpy as np
     from ncpamodes import serve
     n_profiles = 64
     streams = [(10.0, 150), (100.0, 1000), (500.0, 2000)]
-    srv = serve.Server(n_workers=3, precision=3, max_batch=64)
+    srv = serve.Server(n_workers=3, precision=6, max_batch=64)
     srv.warm(range(n_profiles))
     warm = serve.make_requests(64, 1000.0, seed=99, n_profiles=n_profiles)
     srv.serve(warm, realtime=False)
@@ -547,7 +619,7 @@ def serving_main(args):
         "unit": "solves/s",
         "n_gpus": 1, "steps": len(cap_reqs), "warmup": len(warm),
         "ms_per_step": 1e3 / capacity,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+bf16x6",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+bf16x6f",
         "data": "synthetic",
         "config": {"workload": "serving: Poisson request streams; per request a seeded profile (jet amplitudes "
                                "+-40%, heights +-5 km, 64-profile pool resident on the device), f log-uniform "
@@ -566,7 +638,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["field", "broadband", "serving"], default="field")
+    ap.add_argument("--workload", choices=["field", "broadband", "serving", "precision"], default="field")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--streams", type=int, default=3,
                     help="field workload: independent batches in flight on this many device streams")
@@ -582,6 +654,9 @@ def main():
               "(profile, frequency, azimuth) solves would shard across GPUs with no communication"
               % args.gpus, file=sys.stderr)
         return 2
+    if args.workload == "precision":
+        print(json.dumps(precision_main(args)), flush=True)
+        return 0
     if args.workload == "serving":
         print(json.dumps(serving_main(args)), flush=True)
         return 0

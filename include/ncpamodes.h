@@ -190,6 +190,8 @@ int ncpa_tl1d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, double dr, int n_r, int loss
 #define NCPA_PREC_BF16X6 3   /* split-BF16, 3 parts, 6 products (FP32-class accuracy), FP32 accumulate */
 #define NCPA_PREC_FP16X3 4   /* split-FP16, 2 parts, 3 products (~2^-22), power-of-two row/range scaling, FP32 accumulate */
 #define NCPA_PREC_FP16X4 5   /* split-FP16, 2 parts, 4 products (exact part products), scaled, FP32 accumulate */
+#define NCPA_PREC_BF16X6F 6  /* NCPA_PREC_BF16X6, each 32-mode step's FP32 sum folded into FP64 accumulators */
+#define NCPA_PREC_FP16X4F 7  /* NCPA_PREC_FP16X4, each 32-mode step's FP32 sum folded into FP64 accumulators */
 
 /* 2-D field of every problem (ModeSolver::getTLoss2D, ModeSolver.cpp:524-544) at the rows of
  * dev_psi_rows (from ncpa_modes_rows, nrows x total) and ranges r_i = (i+1) dr, i < n_r:
@@ -198,6 +200,16 @@ int ncpa_tl1d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, double dr, int n_r, int loss
  * Re(k_pert).  The phase k r is formed in FP64 and reduced mod 2 pi in every precision. */
 int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, const double *dev_psi_rows, int nrows,
                  double dr, int n_r, int lossless, int precision, void *dev_out);
+
+/* Transmission-loss agreement of two fields of n_points complex values (device buffers laid
+ * out as ncpa_field2d writes them): dev_ref complex128, dev_test complex128 if test_precision
+ * is NCPA_PREC_FP64 and complex64 otherwise.  Over the points with 10 log10 |p_ref|^2 >
+ * floor_db: *max_db = max |TL_test - TL_ref|, *rms_db its root mean square (optional),
+ * *n_counted the number of such points (optional).  The reference has no counterpart: it is
+ * the check that a reduced-precision contraction stays within its stated dB error (TL as
+ * written by ModeSolver::getTLoss2D, ModeSolver.cpp:524-544). */
+int ncpa_field_compare(ncpa_ctx_t *ctx, const double *dev_ref, const void *dev_test, int test_precision,
+                       size_t n_points, double floor_db, double *max_db, double *rms_db, size_t *n_counted);
 
 /* Drop-in for EigenEngine::doESSCalculation (EigenEngine.h:13-14): host diag[Nz]
  * (the (omega/c_eff)^2 trace with the ground term), all eigenpairs with k^2 in

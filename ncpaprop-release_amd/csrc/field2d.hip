@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -58,7 +59,8 @@ __device__ __forceinline__ void f2_csqrt(double x, double y, double &re, double 
     }
 }
 
-enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3, PREC_FP16X3 = 4, PREC_FP16X4 = 5 };
+enum { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3, PREC_FP16X3 = 4, PREC_FP16X4 = 5,
+       PREC_BF16X6F = 6, PREC_FP16X4F = 7 };
 
 // B rows for every mode: 1-D grid of total * ceil(nr / (256 * EMAT_R)) blocks; a thread
 // forms EMAT_R consecutive ranges, the phasor exact at the first and advanced by the exact
@@ -463,7 +465,12 @@ __device__ __forceinline__ f4 mfma16(u4 a, u4 b, f4 c) {
 
 // S parts per operand, NP products (smallest first): S = 2 -> 3 (hi*hi + hi*lo + lo*hi) or
 // 4 (+ lo*lo); S = 3 -> 6 (all products with i + j <= 2).  With F16, C = acc 2^-(SA + SB).
-template <int S, int NP, bool F16>
+// FOLD: each k-step's MFMA sums start from zero and are added to FP64 accumulators, so only
+// the 32 modes of one k-step are summed in FP32 (the FP32 running sum over all modes is what
+// limits the plain forms at interference nulls, where the partial sums exceed the result by
+// orders of magnitude).  The FP64 accumulators take the registers of half the columns: block
+// tile 128 x 64 (waves of 64 x 32) instead of 128 x 128 (waves of 64 x 64).
+template <int S, int NP, bool F16, bool FOLD>
 __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restrict__ sp,
                                                        const unsigned short *__restrict__ Ahat,
                                                        const unsigned short *__restrict__ Bhat,
@@ -471,8 +478,11 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
                                                        float *__restrict__ C, size_t ldc, int M, int N,
                                                        int tiles_m, int n_tiles_n, const int *__restrict__ SA, int Mp,
                                                        const int *__restrict__ SB, int nr_pad) {
+    constexpr int NB = FOLD ? 2 : 4;     // 16-column MFMA tiles per wave
+    constexpr int BN = 32 * NB;          // block tile columns
+    constexpr int VB = BN / 64;          // B vectors per thread, part and k-step
     __shared__ __attribute__((aligned(16))) unsigned short As[S][TBM][TLD];
-    __shared__ __attribute__((aligned(16))) unsigned short Bs[S][TBN][TLD];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[S][BN][TLD];
     const SplitProb P = sp[blockIdx.z];
     // Blocks are dealt to the 8 XCDs round-robin (id % 8).  XCD x takes M-tile half x % 2 and
     // every fourth N tile from x / 2; on the XCD a B tile is used by the half's M tiles one
@@ -482,42 +492,59 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
     const int tm = (xcd & 1) * tmh + slot % tmh;
     const int tn = (slot / tmh) * 4 + (xcd >> 1);
     if (tn >= n_tiles_n || tm >= tiles_m) return;
-    const int m0 = tm * TBM, n0 = tn * TBN;
+    const int m0 = tm * TBM, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    f4 acc[4][4];
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * (BN / 2);
+    f4 acc[4][NB];
+    double acc64[FOLD ? 4 : 1][FOLD ? NB : 1][FOLD ? 4 : 1];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
+        for (int b = 0; b < NB; ++b) acc[a][b] = f4{0, 0, 0, 0};
+    if constexpr (FOLD) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc64[a][b][v] = 0.0;
+    }
     const unsigned short *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
     const int Kp = P.Kp;
-    // each thread moves 2 x 16 B of A and 2 x 16 B of B per split and k-step:
-    // tile rows 128 x 4 vectors = 512 vectors per operand and split.  Native vector type:
-    // arrays of the HIP uint4 struct were placed in scratch memory (a round trip through
-    // HBM every k-step); ext_vector_type arrays stay in registers.
-    u4 ra[S][2], rb[S][2];
+    // per part and k-step a tile row is 4 vectors of 16 B: A 128 x 4 = 2 per thread, B BN x 4
+    // = VB per thread; the next k-step is fetched into registers while the current one runs
+    // from LDS.  Native vector types: arrays of the HIP uint4 struct were placed in scratch
+    // memory (a round trip through HBM every k-step); ext_vector_type arrays stay in registers.
+    u4 ra[S][2], rb[S][VB];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+        for (int s = 0; s < S; ++s) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int v = tid + 256 * h;
-                const int row = v >> 2, part = v & 3;
-                ra[s][h] = *(const u4 *)&Ab[s * split_stride_a + tiled_index(m0 + row, k0 + part * 8, Kp)];
-                rb[s][h] = *(const u4 *)&Bb[s * split_stride_b + tiled_index(n0 + row, k0 + part * 8, Kp)];
+                ra[s][h] = *(const u4 *)&Ab[s * split_stride_a + tiled_index(m0 + (v >> 2), k0 + (v & 3) * 8, Kp)];
             }
+#pragma unroll
+            for (int h = 0; h < VB; ++h) {
+                const int v = tid + 256 * h;
+                rb[s][h] = *(const u4 *)&Bb[s * split_stride_b + tiled_index(n0 + (v >> 2), k0 + (v & 3) * 8, Kp)];
+            }
+        }
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+        for (int s = 0; s < S; ++s) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int v = tid + 256 * h;
-                const int row = v >> 2, part = v & 3;
-                *(u4 *)&As[s][row][lds_group(row, part)] = ra[s][h];
-                *(u4 *)&Bs[s][row][lds_group(row, part)] = rb[s][h];
+                *(u4 *)&As[s][v >> 2][lds_group(v >> 2, v & 3)] = ra[s][h];
             }
+#pragma unroll
+            for (int h = 0; h < VB; ++h) {
+                const int v = tid + 256 * h;
+                *(u4 *)&Bs[s][v >> 2][lds_group(v >> 2, v & 3)] = rb[s][h];
+            }
+        }
     };
     fetch(0);
     for (int k0 = 0; k0 < Kp; k0 += TKS) {
@@ -527,18 +554,19 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
         if (k0 + TKS < Kp) fetch(k0 + TKS);
         // rows wm + 16a + (lane & 15): the swizzle of k-group lane >> 4 depends on lane only
         const int kq = lds_group(lane & 15, lane >> 4);
-        u4 af[S][4], bfr[S][4];
+        u4 af[S][4], bfr[S][NB];
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+        for (int s = 0; s < S; ++s) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                af[s][a] = *(const u4 *)&As[s][wm + a * 16 + (lane & 15)][kq];
-                bfr[s][a] = *(const u4 *)&Bs[s][wn + a * 16 + (lane & 15)][kq];
-            }
+            for (int a = 0; a < 4; ++a) af[s][a] = *(const u4 *)&As[s][wm + a * 16 + (lane & 15)][kq];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) bfr[s][b] = *(const u4 *)&Bs[s][wn + b * 16 + (lane & 15)][kq];
+        }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
+            for (int b = 0; b < NB; ++b) {
+                if constexpr (FOLD) acc[a][b] = f4{0, 0, 0, 0};
                 // smallest products first
                 if constexpr (S == 3) {
                     acc[a][b] = mfma16<F16>(af[2][a], bfr[0][b], acc[a][b]);
@@ -549,25 +577,95 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
                 acc[a][b] = mfma16<F16>(af[1][a], bfr[0][b], acc[a][b]);
                 acc[a][b] = mfma16<F16>(af[0][a], bfr[1][b], acc[a][b]);
                 acc[a][b] = mfma16<F16>(af[0][a], bfr[0][b], acc[a][b]);
+                if constexpr (FOLD) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc64[a][b][v] += (double)acc[a][b][v];
+                }
             }
     }
     float *Cp = C + P.c_off;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < NB; ++b) {
             const int c = n0 + wn + b * 16 + (lane & 15);
             const int sb = (F16 && c < N) ? SB[(size_t)blockIdx.z * nr_pad + (c >> 1)] : 0;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int r = m0 + wm + a * 16 + 4 * (lane >> 4) + v;
                 if (r < M && c < N) {
-                    float x = acc[a][b][v];
-                    if (F16) x = ldexpf(x, -(SA[(size_t)blockIdx.z * Mp + r] + sb));
+                    const int e = F16 ? -(SA[(size_t)blockIdx.z * Mp + r] + sb) : 0;
+                    float x;
+                    if constexpr (FOLD) x = (float)ldexp(acc64[a][b][v], e);
+                    else x = F16 ? ldexpf(acc[a][b][v], e) : acc[a][b][v];
                     Cp[(size_t)r * ldc + c] = x;
                 }
             }
         }
+}
+
+// |TL_test - TL_ref| in dB over the points with TL_ref above a floor (10 log10 |p|^2 > floor_db;
+// |p_test|^2 clamped at 1e-300 as a zero field): per block the maximum, the sum of squares and
+// the count, folded into acc by one atomic each.  A non-negative double orders like its bits
+// as an unsigned 64-bit integer, so the maximum is an integer atomicMax.
+struct CompareAcc {
+    unsigned long long max_bits, count;
+    double sumsq;
+};
+
+__global__ void k_compare_init(CompareAcc *acc) {
+    if (threadIdx.x == 0) {
+        acc->max_bits = 0;
+        acc->count = 0;
+        acc->sumsq = 0.0;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_tl_compare(const double *__restrict__ ref, const T *__restrict__ test,
+                                                    size_t n, double floor_pow, CompareAcc *acc) {
+    double mx = 0.0, ss = 0.0;
+    unsigned long long cnt = 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double r0 = ref[2 * i], r1 = ref[2 * i + 1];
+        const double t0 = (double)test[2 * i], t1 = (double)test[2 * i + 1];
+        const double pr = r0 * r0 + r1 * r1;
+        if (pr > floor_pow) {
+            const double pt = fmax(t0 * t0 + t1 * t1, 1e-300);
+            const double d = fabs(10.0 * log10(pt / pr));
+            mx = fmax(mx, d);
+            ss += d * d;
+            ++cnt;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        ss += __shfl_xor(ss, o);
+        cnt += __shfl_xor(cnt, o);
+    }
+    __shared__ double s_mx[4], s_ss[4];
+    __shared__ unsigned long long s_cnt[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_mx[wave] = mx;
+        s_ss[wave] = ss;
+        s_cnt[wave] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            mx = fmax(mx, s_mx[w]);
+            ss += s_ss[w];
+            cnt += s_cnt[w];
+        }
+        if (cnt) {
+            atomicMax(&acc->max_bits, (unsigned long long)__double_as_longlong(mx));
+            atomicAdd(&acc->sumsq, ss);
+            atomicAdd(&acc->count, cnt);
+        }
+    }
 }
 
 }  // namespace ncpa
@@ -582,7 +680,7 @@ int ncpa_modes_internal(ncpa_modes_t *m, int *P, int *total, const int **h_offse
 
 namespace {
 
-template <int S, int NP, bool F16>
+template <int S, int NP, bool F16, bool FOLD = false>
 int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *hcnt, const int *d_off,
                   const double *d_k2, const double *d_abs, const double *d_psis, const double *psi, int M,
                   double dr, int n_r, int lossless, float *out) {
@@ -648,10 +746,11 @@ int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *h
     }
     {
         TimedLaunch tl(ctx, "field2d");
-        const int tn = Np / TBN, tm = Mp / TBM;
+        // Np is a multiple of 128 (the E-staging tiles), so of either block width
+        const int tn = Np / (FOLD ? 64 : 128), tm = Mp / TBM;
         // 8 XCDs x ceil(tm / 2) M tiles x ceil(tn / 4) N tiles per XCD
         dim3 g((unsigned)(8 * ((tm + 1) / 2) * ((tn + 3) / 4)), 1, (unsigned)P);
-        hipLaunchKernelGGL((k_gemm_split<S, NP, F16>), g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb, out,
+        hipLaunchKernelGGL((k_gemm_split<S, NP, F16, FOLD>), g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb, out,
                            (size_t)N, M, N, tm, tn, SA, Mp, SB, nr_pad);
     }
     e = hipGetLastError();
@@ -667,7 +766,7 @@ extern "C" {
 int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, int nrows, double dr, int n_r,
                  int lossless, int precision, void *dev_out) {
     if (!ctx || !m || !dev_psi_rows || nrows <= 0 || n_r <= 0 || !(dr > 0.0) || !dev_out ||
-        precision < 0 || precision > PREC_FP16X4)
+        precision < 0 || precision > PREC_FP16X4F)
         return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d: bad arguments");
     int P = 0, total = 0;
     const int *hoff = nullptr, *hcnt = nullptr;
@@ -688,6 +787,12 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, i
         if (precision == PREC_FP16X3)
             return field2d_split<2, 3, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows,
                                              nrows, dr, n_r, lossless, (float *)dev_out);
+        if (precision == PREC_BF16X6F)
+            return field2d_split<3, 6, false, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis,
+                                                    dev_psi_rows, nrows, dr, n_r, lossless, (float *)dev_out);
+        if (precision == PREC_FP16X4F)
+            return field2d_split<2, 4, true, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis,
+                                                   dev_psi_rows, nrows, dr, n_r, lossless, (float *)dev_out);
         return field2d_split<2, 4, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, nrows,
                                          dr, n_r, lossless, (float *)dev_out);
     }
@@ -739,6 +844,39 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, i
     e = hipGetLastError();
     cleanup();
     if (e != hipSuccess) return hip_error(ctx, e, "ncpa_field2d launch");
+    return NCPA_OK;
+}
+
+int ncpa_field_compare(ncpa_ctx_t *ctx, const double *dev_ref, const void *dev_test, int test_precision,
+                       size_t n_points, double floor_db, double *max_db, double *rms_db, size_t *n_counted) {
+    if (!ctx || !dev_ref || !dev_test || test_precision < 0 || test_precision > PREC_FP16X4F || !max_db)
+        return set_error(ctx, NCPA_ERR_ARG, "ncpa_field_compare: bad arguments");
+    *max_db = 0.0;
+    if (rms_db) *rms_db = 0.0;
+    if (n_counted) *n_counted = 0;
+    if (n_points == 0) return NCPA_OK;
+    CompareAcc *acc = pool_alloc<CompareAcc>(ctx, 1);
+    if (!acc) return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_field_compare: allocation failed");
+    const double floor_pow = std::pow(10.0, floor_db / 10.0);
+    const unsigned blocks = (unsigned)std::min<size_t>((n_points + 1023) / 1024, 8192);
+    hipLaunchKernelGGL(k_compare_init, dim3(1), dim3(64), 0, ctx->stream, acc);
+    if (test_precision == PREC_FP64)
+        hipLaunchKernelGGL(k_tl_compare<double>, dim3(blocks), dim3(256), 0, ctx->stream, dev_ref,
+                           (const double *)dev_test, n_points, floor_pow, acc);
+    else
+        hipLaunchKernelGGL(k_tl_compare<float>, dim3(blocks), dim3(256), 0, ctx->stream, dev_ref,
+                           (const float *)dev_test, n_points, floor_pow, acc);
+    CompareAcc h{};
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, acc, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    ctx->pool.put(acc);
+    if (e != hipSuccess) return hip_error(ctx, e, "ncpa_field_compare");
+    double mx;
+    std::memcpy(&mx, &h.max_bits, sizeof(mx));
+    *max_db = mx;
+    if (rms_db) *rms_db = h.count ? std::sqrt(h.sumsq / (double)h.count) : 0.0;
+    if (n_counted) *n_counted = (size_t)h.count;
     return NCPA_OK;
 }
 

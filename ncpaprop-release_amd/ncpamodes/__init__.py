@@ -47,6 +47,14 @@ class Context:
     def synchronize(self):
         self.check(self.lib.ncpa_ctx_synchronize(self.h))
 
+    def field_compare(self, ref, test, test_precision, n_points, floor_db=-140.0):
+        """(max, rms) |TL_test - TL_ref| [dB] of two device fields of n_points complex values
+        over TL_ref > floor_db, and the number of such points (computed on the device)."""
+        mx, rms, cnt = C.c_double(), C.c_double(), C.c_size_t()
+        self.check(self.lib.ncpa_field_compare(self.h, C.c_void_p(ref.ptr), C.c_void_p(test.ptr), test_precision,
+                                               n_points, floor_db, C.byref(mx), C.byref(rms), C.byref(cnt)))
+        return mx.value, rms.value, cnt.value
+
     @property
     def stream(self):
         return self.lib.ncpa_ctx_stream(self.h)

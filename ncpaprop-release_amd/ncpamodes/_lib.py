@@ -23,6 +23,8 @@ NCPA_PREC_BF16X3 = 2
 NCPA_PREC_BF16X6 = 3
 NCPA_PREC_FP16X3 = 4
 NCPA_PREC_FP16X4 = 5
+NCPA_PREC_BF16X6F = 6
+NCPA_PREC_FP16X4F = 7
 
 NCPA_F_WKB = 1
 NCPA_F_WVNUM_FILTER = 2
@@ -38,7 +40,8 @@ EXPORTS = [
     "ncpa_profile_read", "ncpa_profile_nodes", "ncpa_profile_admittance",
     "ncpa_grid_make", "ncpa_profile_sample",
     "ncpa_ess_solve", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
-    "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d", "ncpa_ess_eigen",
+    "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d",
+    "ncpa_field_compare", "ncpa_ess_eigen",
     "ncpa_fft_batch", "ncpa_kpert", "ncpa_bb_transfer", "ncpa_bb_pulse", "ncpa_bb_source",
     "ncpa_bb_plan_create", "ncpa_bb_plan_set_problems", "ncpa_bb_plan_run", "ncpa_bb_plan_result",
     "ncpa_bb_plan_tuning", "ncpa_bb_plan_destroy",
@@ -121,6 +124,8 @@ def load():
         "ncpa_modes_ceff": (C.c_int, [vp, C.c_int, dp]),
         "ncpa_tl1d": (C.c_int, [vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp]),
         "ncpa_field2d": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, vp]),
+        "ncpa_field_compare": (C.c_int, [vp, vp, vp, C.c_int, C.c_size_t, C.c_double, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_size_t)]),
         "ncpa_ess_eigen": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int,
                                      ip, dp, dp, C.c_int]),
         "ncpa_fft_batch": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int]),

@@ -144,7 +144,7 @@ class Worker:
 class Server:
     """Continuous-batching dispatcher over n_workers device contexts."""
 
-    def __init__(self, n_workers=2, Nz=20000, precision=3, hbm_budget_gb=32.0, max_batch=64, ctx_factory=None):
+    def __init__(self, n_workers=2, Nz=20000, precision=6, hbm_budget_gb=32.0, max_batch=64, ctx_factory=None):
         import ncpamodes
         factory = ctx_factory or (lambda: ncpamodes.Context(0))
         self.workers = [Worker(i, Nz, precision, factory) for i in range(n_workers)]

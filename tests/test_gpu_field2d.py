@@ -1,5 +1,5 @@
 """GPU parity: 2-D field (eigenvector rows + MFMA contraction) in FP64, FP32, split BF16 (3 and 6
-products) and split FP16 (3 and 4 products) vs the oracle."""
+products) and split FP16 (3 and 4 products), plain and FP64-folded, vs the oracle."""
 import numpy as np
 import pytest
 
@@ -23,7 +23,8 @@ def field_case(gpu_ctx, synth_profile):
     return m, stride, nrows, nr, dr, ref.T   # ref [rows, ranges]
 
 
-@pytest.mark.parametrize("prec,tol_db", [(0, 1e-2), (1, 0.1), (2, 0.1), (3, 0.1), (4, 0.1), (5, 0.1)])
+@pytest.mark.parametrize("prec,tol_db", [(0, 1e-2), (1, 0.1), (2, 0.1), (3, 0.1), (4, 0.1), (5, 0.1), (6, 0.1),
+                                         (7, 0.1)])
 def test_field2d(field_case, prec, tol_db):
     m, stride, nrows, nr, dr, ref = field_case
     print("case field2d precision=%d" % prec, flush=True)
