@@ -57,9 +57,9 @@ PREC_NAMES = {0: "f64", 1: "f32", 2: "bf16x3", 3: "bf16x6", 4: "fp16x3", 5: "fp1
               7: "fp16x4f"}
 PREC_PEAK = {0: "f64", 1: "f32", 2: "bf16", 3: "bf16", 4: "bf16", 5: "bf16", 6: "bf16", 7: "bf16"}   # f16 = bf16
 # split-precision kernel (parts, products, FP16, FP64 fold) per precision
-PREC_KERNEL = {2: "k_gemm_split<2, 3, false, false>", 3: "k_gemm_split<3, 6, false, false>",
-               4: "k_gemm_split<2, 3, true, false>", 5: "k_gemm_split<2, 4, true, false>",
-               6: "k_gemm_split<3, 6, false, true>", 7: "k_gemm_split<2, 4, true, true>"}
+PREC_KERNEL = {2: "k_gemm_split<2, 3, false>", 3: "k_gemm_split<3, 6, false>",
+               4: "k_gemm_split<2, 3, true>", 5: "k_gemm_split<2, 4, true>",
+               6: "k_gemm_fold<3, 6, false>", 7: "k_gemm_fold<2, 4, true>"}
 PRODUCTS = {0: 1, 1: 1, 2: 3, 3: 6, 4: 3, 5: 4, 6: 6, 7: 4}
 # candidate field precisions, fastest first
 CANDIDATES = [2, 4, 5, 7, 3, 6, 1]

@@ -463,14 +463,48 @@ __device__ __forceinline__ f4 mfma16(u4 a, u4 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
 }
 
-// S parts per operand, NP products (smallest first): S = 2 -> 3 (hi*hi + hi*lo + lo*hi) or
-// 4 (+ lo*lo); S = 3 -> 6 (all products with i + j <= 2).  With F16, C = acc 2^-(SA + SB).
-// FOLD: each k-step's MFMA sums start from zero and are added to FP64 accumulators, so only
-// the 32 modes of one k-step are summed in FP32 (the FP32 running sum over all modes is what
-// limits the plain forms at interference nulls, where the partial sums exceed the result by
-// orders of magnitude).  The FP64 accumulators take the registers of half the columns: block
-// tile 128 x 64 (waves of 64 x 32) instead of 128 x 128 (waves of 64 x 64).
-template <int S, int NP, bool F16, bool FOLD>
+// Epilogue shared by the contractions: the wave's accumulators (FP32, or FP64 when folded)
+// to C, undoing the power-of-two scales of the FP16 forms: C = acc 2^-(SA[row] + SB[range]).
+template <bool F16, int NB, typename Acc>
+__device__ __forceinline__ void store_tile(float *__restrict__ Cp, size_t ldc, int M, int N, int row0, int col0,
+                                           int lane, const Acc &acc, const int *__restrict__ SAp,
+                                           const int *__restrict__ SBp) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int c = col0 + b * 16 + (lane & 15);
+            const int sb = (F16 && c < N) ? SBp[c >> 1] : 0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = row0 + a * 16 + 4 * (lane >> 4) + v;
+                if (r < M && c < N) {
+                    const int e = F16 ? -(SAp[r] + sb) : 0;
+                    Cp[(size_t)r * ldc + c] = F16 ? (float)ldexp((double)acc[a][b][v], e) : (float)acc[a][b][v];
+                }
+            }
+        }
+}
+
+// The NP part products of one 16 x 16 tile and k-step, smallest first: S = 2 -> 3 (hi*hi +
+// hi*lo + lo*hi) or 4 (+ lo*lo); S = 3 -> 6 (all products with i + j <= 2).
+template <int S, int NP, bool F16>
+__device__ __forceinline__ f4 part_products(const u4 (&af)[S], const u4 (&bf)[S], f4 acc) {
+    if constexpr (S == 3) {
+        acc = mfma16<F16>(af[2], bf[0], acc);
+        acc = mfma16<F16>(af[1], bf[1], acc);
+        acc = mfma16<F16>(af[0], bf[2], acc);
+    }
+    if constexpr (S == 2 && NP == 4) acc = mfma16<F16>(af[1], bf[1], acc);
+    acc = mfma16<F16>(af[1], bf[0], acc);
+    acc = mfma16<F16>(af[0], bf[1], acc);
+    return mfma16<F16>(af[0], bf[0], acc);
+}
+
+// Plain split contraction: FP32 accumulation over all modes.  Block tile 128 x 128, 4 waves
+// of 64 x 64; operands staged global -> registers -> LDS, the next k-step fetched into
+// registers while the current one runs from LDS.
+template <int S, int NP, bool F16>
 __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restrict__ sp,
                                                        const unsigned short *__restrict__ Ahat,
                                                        const unsigned short *__restrict__ Bhat,
@@ -478,11 +512,8 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
                                                        float *__restrict__ C, size_t ldc, int M, int N,
                                                        int tiles_m, int n_tiles_n, const int *__restrict__ SA, int Mp,
                                                        const int *__restrict__ SB, int nr_pad) {
-    constexpr int NB = FOLD ? 2 : 4;     // 16-column MFMA tiles per wave
-    constexpr int BN = 32 * NB;          // block tile columns
-    constexpr int VB = BN / 64;          // B vectors per thread, part and k-step
     __shared__ __attribute__((aligned(16))) unsigned short As[S][TBM][TLD];
-    __shared__ __attribute__((aligned(16))) unsigned short Bs[S][BN][TLD];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[S][TBN][TLD];
     const SplitProb P = sp[blockIdx.z];
     // Blocks are dealt to the 8 XCDs round-robin (id % 8).  XCD x takes M-tile half x % 2 and
     // every fourth N tile from x / 2; on the XCD a B tile is used by the half's M tiles one
@@ -492,59 +523,39 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
     const int tm = (xcd & 1) * tmh + slot % tmh;
     const int tn = (slot / tmh) * 4 + (xcd >> 1);
     if (tn >= n_tiles_n || tm >= tiles_m) return;
-    const int m0 = tm * TBM, n0 = tn * BN;
+    const int m0 = tm * TBM, n0 = tn * TBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * (BN / 2);
-    f4 acc[4][NB];
-    double acc64[FOLD ? 4 : 1][FOLD ? NB : 1][FOLD ? 4 : 1];
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = f4{0, 0, 0, 0};
-    if constexpr (FOLD) {
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) acc64[a][b][v] = 0.0;
-    }
+        for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
     const unsigned short *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
     const int Kp = P.Kp;
-    // per part and k-step a tile row is 4 vectors of 16 B: A 128 x 4 = 2 per thread, B BN x 4
-    // = VB per thread; the next k-step is fetched into registers while the current one runs
-    // from LDS.  Native vector types: arrays of the HIP uint4 struct were placed in scratch
-    // memory (a round trip through HBM every k-step); ext_vector_type arrays stay in registers.
-    u4 ra[S][2], rb[S][VB];
+    // per part and k-step a tile row is 4 vectors of 16 B: A and B 128 x 4 = 2 per thread.
+    // Native vector types: arrays of the HIP uint4 struct were placed in scratch memory (a
+    // round trip through HBM every k-step); ext_vector_type arrays stay in registers.
+    u4 ra[S][2], rb[S][2];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
+        for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int v = tid + 256 * h;
                 ra[s][h] = *(const u4 *)&Ab[s * split_stride_a + tiled_index(m0 + (v >> 2), k0 + (v & 3) * 8, Kp)];
-            }
-#pragma unroll
-            for (int h = 0; h < VB; ++h) {
-                const int v = tid + 256 * h;
                 rb[s][h] = *(const u4 *)&Bb[s * split_stride_b + tiled_index(n0 + (v >> 2), k0 + (v & 3) * 8, Kp)];
             }
-        }
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
+        for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int v = tid + 256 * h;
                 *(u4 *)&As[s][v >> 2][lds_group(v >> 2, v & 3)] = ra[s][h];
-            }
-#pragma unroll
-            for (int h = 0; h < VB; ++h) {
-                const int v = tid + 256 * h;
                 *(u4 *)&Bs[s][v >> 2][lds_group(v >> 2, v & 3)] = rb[s][h];
             }
-        }
     };
     fetch(0);
     for (int k0 = 0; k0 < Kp; k0 += TKS) {
@@ -554,54 +565,121 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
         if (k0 + TKS < Kp) fetch(k0 + TKS);
         // rows wm + 16a + (lane & 15): the swizzle of k-group lane >> 4 depends on lane only
         const int kq = lds_group(lane & 15, lane >> 4);
-        u4 af[S][4], bfr[S][NB];
+        u4 af[4][S], bfr[4][S];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                af[a][s] = *(const u4 *)&As[s][wm + a * 16 + (lane & 15)][kq];
+                bfr[a][s] = *(const u4 *)&Bs[s][wn + a * 16 + (lane & 15)][kq];
+            }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = part_products<S, NP, F16>(af[a], bfr[b], acc[a][b]);
+    }
+    store_tile<F16, 4>(C + P.c_off, ldc, M, N, m0 + wm, n0 + wn, lane, acc,
+                       F16 ? SA + (size_t)blockIdx.z * Mp : nullptr, F16 ? SB + (size_t)blockIdx.z * nr_pad : nullptr);
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// FP64-folded split contraction: each k-step's part products start from zero in FP32 and
+// are added to FP64 accumulators, so FP32 sums run over the 32 modes of one k-step only
+// (the FP32 running sum over all modes is what limits the plain forms at interference nulls,
+// where the partial sums exceed the result by orders of magnitude).  The FP64 accumulators
+// take the registers of half the columns: block tile 128 x 64, 4 waves of 64 x 32.
+// Operands go from global memory straight into LDS (global_load_lds_dwordx4: 16 B per lane,
+// one 1 KB wave instruction = 16 tile rows, LDS address = base + 16 lane) into two stages:
+// one barrier per k-step, the next k-step's loads in flight during the current MFMAs.  The
+// XOR swizzle of the 16-byte groups is applied on the global side (lane -> row, stored
+// group; the group loaded is the logical one), so the fragment reads are those of the
+// register-staged kernel.  The two stages are separate arrays so that the compiler can tell
+// the loads into one from the reads of the other (no wait on the loads in flight).
+template <int S, int NP, bool F16>
+__global__ void __launch_bounds__(256, 2) k_gemm_fold(const SplitProb *__restrict__ sp,
+                                                      const unsigned short *__restrict__ Ahat,
+                                                      const unsigned short *__restrict__ Bhat,
+                                                      size_t split_stride_a, size_t split_stride_b,
+                                                      float *__restrict__ C, size_t ldc, int M, int N,
+                                                      int tiles_m, int n_tiles_n, const int *__restrict__ SA, int Mp,
+                                                      const int *__restrict__ SB, int nr_pad) {
+    constexpr int BN = 64;
+    typedef unsigned short ATile[TBM][TKS];
+    typedef unsigned short BTile[BN][TKS];
+    __shared__ __attribute__((aligned(16))) ATile As0[S], As1[S];
+    __shared__ __attribute__((aligned(16))) BTile Bs0[S], Bs1[S];
+    const SplitProb P = sp[blockIdx.z];
+    // XCD mapping as in k_gemm_split
+    const int tmh = (tiles_m + 1) / 2;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int tm = (xcd & 1) * tmh + slot % tmh;
+    const int tn = (slot / tmh) * 4 + (xcd >> 1);
+    if (tn >= n_tiles_n || tm >= tiles_m) return;
+    const int m0 = tm * TBM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 32;
+    f4 acc[4][2];
+    double acc64[4][2][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc64[a][b][v] = 0.0;
+    const unsigned short *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
+    const int Kp = P.Kp;
+    const int crow = lane >> 2, gs = lane & 3;
+    // k-step k0 into one stage: A = 8 chunks of 16 rows (2 per wave), B = 4 chunks (1 per wave)
+    auto issue = [&](ATile *Ax, BTile *Bx, int k0) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) af[s][a] = *(const u4 *)&As[s][wm + a * 16 + (lane & 15)][kq];
+            for (int j = 0; j < 2; ++j) {
+                const int ch = 2 * wave + j, row = ch * 16 + crow;
+                const int g = gs ^ ((row >> 2) & 3);
+                __builtin_amdgcn_global_load_lds(
+                    (const void *)&Ab[s * split_stride_a + tiled_index(m0 + row, k0 + 8 * g, Kp)],
+                    (lds_void *)&Ax[s][ch * 16][0], 16, 0, 0);
+            }
+            const int row = wave * 16 + crow;
+            const int g = gs ^ ((row >> 2) & 3);
+            __builtin_amdgcn_global_load_lds(
+                (const void *)&Bb[s * split_stride_b + tiled_index(n0 + row, k0 + 8 * g, Kp)],
+                (lds_void *)&Bx[s][wave * 16][0], 16, 0, 0);
+        }
+    };
+    const int kq = lds_group(lane & 15, lane >> 4);
+    auto step = [&](ATile *Ac, BTile *Bc, ATile *An, BTile *Bn, int k0) {
+        // this wave's loads of the current stage (the only ones in flight), then every wave's
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        // the other stage was last read in the previous k-step, which every wave has left
+        if (k0 + TKS < Kp) issue(An, Bn, k0 + TKS);
+        u4 af[4][S], bfr[2][S];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) bfr[s][b] = *(const u4 *)&Bs[s][wn + b * 16 + (lane & 15)][kq];
+        for (int s = 0; s < S; ++s) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) af[a][s] = *(const u4 *)&Ac[s][wm + a * 16 + (lane & 15)][kq];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bfr[b][s] = *(const u4 *)&Bc[s][wn + b * 16 + (lane & 15)][kq];
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                if constexpr (FOLD) acc[a][b] = f4{0, 0, 0, 0};
-                // smallest products first
-                if constexpr (S == 3) {
-                    acc[a][b] = mfma16<F16>(af[2][a], bfr[0][b], acc[a][b]);
-                    acc[a][b] = mfma16<F16>(af[1][a], bfr[1][b], acc[a][b]);
-                    acc[a][b] = mfma16<F16>(af[0][a], bfr[2][b], acc[a][b]);
-                }
-                if constexpr (S == 2 && NP == 4) acc[a][b] = mfma16<F16>(af[1][a], bfr[1][b], acc[a][b]);
-                acc[a][b] = mfma16<F16>(af[1][a], bfr[0][b], acc[a][b]);
-                acc[a][b] = mfma16<F16>(af[0][a], bfr[1][b], acc[a][b]);
-                acc[a][b] = mfma16<F16>(af[0][a], bfr[0][b], acc[a][b]);
-                if constexpr (FOLD) {
+            for (int b = 0; b < 2; ++b) {
+                acc[a][b] = part_products<S, NP, F16>(af[a], bfr[b], f4{0, 0, 0, 0});
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) acc64[a][b][v] += (double)acc[a][b][v];
-                }
+                for (int v = 0; v < 4; ++v) acc64[a][b][v] += (double)acc[a][b][v];
             }
+    };
+    issue(As0, Bs0, 0);
+    for (int k0 = 0; k0 < Kp; k0 += 2 * TKS) {
+        step(As0, Bs0, As1, Bs1, k0);
+        if (k0 + TKS < Kp) step(As1, Bs1, As0, Bs0, k0 + TKS);
     }
-    float *Cp = C + P.c_off;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int c = n0 + wn + b * 16 + (lane & 15);
-            const int sb = (F16 && c < N) ? SB[(size_t)blockIdx.z * nr_pad + (c >> 1)] : 0;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = m0 + wm + a * 16 + 4 * (lane >> 4) + v;
-                if (r < M && c < N) {
-                    const int e = F16 ? -(SA[(size_t)blockIdx.z * Mp + r] + sb) : 0;
-                    float x;
-                    if constexpr (FOLD) x = (float)ldexp(acc64[a][b][v], e);
-                    else x = F16 ? ldexpf(acc[a][b][v], e) : acc[a][b][v];
-                    Cp[(size_t)r * ldc + c] = x;
-                }
-            }
-        }
+    store_tile<F16, 2>(C + P.c_off, ldc, M, N, m0 + wm, n0 + wn, lane, acc64,
+                       F16 ? SA + (size_t)blockIdx.z * Mp : nullptr, F16 ? SB + (size_t)blockIdx.z * nr_pad : nullptr);
 }
 
 // |TL_test - TL_ref| in dB over the points with TL_ref above a floor (10 log10 |p|^2 > floor_db;
@@ -747,11 +825,15 @@ int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *h
     {
         TimedLaunch tl(ctx, "field2d");
         // Np is a multiple of 128 (the E-staging tiles), so of either block width
-        const int tn = Np / (FOLD ? 64 : 128), tm = Mp / TBM;
+        const int tn = Np / (FOLD ? 64 : TBN), tm = Mp / TBM;
         // 8 XCDs x ceil(tm / 2) M tiles x ceil(tn / 4) N tiles per XCD
         dim3 g((unsigned)(8 * ((tm + 1) / 2) * ((tn + 3) / 4)), 1, (unsigned)P);
-        hipLaunchKernelGGL((k_gemm_split<S, NP, F16, FOLD>), g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb, out,
-                           (size_t)N, M, N, tm, tn, SA, Mp, SB, nr_pad);
+        if constexpr (FOLD)
+            hipLaunchKernelGGL((k_gemm_fold<S, NP, F16>), g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb, out,
+                               (size_t)N, M, N, tm, tn, SA, Mp, SB, nr_pad);
+        else
+            hipLaunchKernelGGL((k_gemm_split<S, NP, F16>), g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb,
+                               out, (size_t)N, M, N, tm, tn, SA, Mp, SB, nr_pad);
     }
     e = hipGetLastError();
     cleanup();
