@@ -8,11 +8,12 @@ profiles x 8 azimuths), each through the HIP library: grid -> k_min/k_max -> eve
 eigenvector rows at 1500 depths -> 2-D field 1500 depths x 10000 ranges.  Eigenvalues,
 phases (reduced mod 2 pi) and the 1-D TL are FP64; the field contraction runs in the fastest
 precision whose TL stays within 0.1 dB of the FP64 contraction wherever TL > -140 dB, which
-the run checks itself on two of the solves before timing (the FP64-plain rate is reported
-beside it).  Profiles are resident on the device before the timed region; outputs stay there.
+the run checks itself on every point of all 64 solves before timing (the FP64-plain rate is
+reported beside it).  Profiles are resident on the device before the timed region; outputs stay there.
 The steps are timed twice: on one device stream (the stage split and the roofline line come
-from this region, each kernel alone on the GPU) and with independent batches in flight on
---streams device streams (default 3, one host thread each), which gives the headline value.
+from this region, each kernel alone on the GPU) and as a two-stage pipeline over two device
+streams (step i's eigenpairs, 1-D TL, rows and field staging beside step i-1's contraction), which gives
+the headline value (--schedule streams: independent batches on --streams streams instead).
 
 --workload precision: the 2-D field at 2 Hz, 1500 x 10000, in every contraction precision
 against FP64 (time, speed-up, max/RMS dB error).
@@ -23,7 +24,7 @@ against FP64 (time, speed-up, max/RMS dB error).
 reference's rule) with a Gaussian-windowed 0.5 Hz source given as a spectrum.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload field|broadband|serving|precision]
-                    [--streams S] [--no-cpu]
+                    [--schedule pipeline|streams] [--streams S] [--no-cpu]
 
 Prints one JSON line.  cpu_baseline: the FP64 oracle (scipy LAPACK bisection + inverse
 iteration, numpy sums; one thread per process) on a bounded sample of the same workload, run
@@ -287,6 +288,64 @@ def run_field(lanes, args, prec):
     return res
 
 
+def run_field_pipelined(solver, contractor, problems, grid, args, prec, depth=2):
+    """Timed loop of the config-1 step as a two-stage pipeline over two device streams, one
+    host thread: the solver context runs step i's eigenpairs, 1-D TL, eigenvector rows and the
+    field's operand staging while the contractor context contracts step i-1's field, so the
+    contraction (MFMA) and the latency-bound eigenvalue sweeps (FP64 VALU) of consecutive
+    batches share the GPU.  Stream ordering by events: a contraction waits for its staging;
+    the solver reuses a row buffer and frees a step's modes and staged operands only after
+    the contraction that read them (the solver runs at most `depth` steps ahead).  Returns a
+    timing dict like run_field."""
+    import numpy as np
+    dr = MAXRANGE_M / NR
+    P = len(problems)
+    tl = [solver.empty(P * NR) for _ in range(3)]
+    out = contractor.empty(P * FIELD_ROWS * 2 * NR, np.float64 if prec == 0 else np.float32)
+    rows = [None] * (depth + 1)
+    inflight = []            # (modes, staged operands, event after the contraction)
+
+    def retire(keep):
+        while len(inflight) > keep:
+            m, stage, ev = inflight.pop(0)
+            solver.wait(ev)          # the solver's later work (buffer reuse) follows the contraction
+            stage.free()
+            m.free()
+            ev.free()
+
+    def step(i):
+        m = solver.solve(grid, problems)
+        m.tl1d_device(dr, NR, False, tl[0].ptr, tl[1].ptr, tl[2].ptr)
+        slot = i % (depth + 1)
+        if rows[slot] is None:
+            rows[slot] = solver.empty(FIELD_ROWS * max(m.total, 1))
+        solver.check(solver.lib.ncpa_modes_rows(solver.h, m.h, 0, FIELD_STRIDE, FIELD_ROWS,
+                                                ctypes.c_void_p(rows[slot].ptr)))
+        stage = m.field2d_stage(rows[slot], FIELD_ROWS, dr, NR, prec)
+        ready = solver.record()
+        contractor.wait(ready)
+        ready.free()
+        stage.contract(contractor, out)
+        inflight.append((m, stage, contractor.record()))
+        retire(depth)
+
+    for i in range(args.warmup):
+        step(i)
+    retire(0)
+    solver.synchronize()
+    contractor.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i)
+    retire(0)
+    solver.synchronize()
+    contractor.synchronize()
+    elapsed = time.perf_counter() - t0
+    for b in tl + [out] + [r for r in rows if r is not None]:
+        b.free()
+    return {"elapsed": elapsed, "streams": 2}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.md")
 
 
@@ -333,15 +392,22 @@ def field_main(args, cpu):
     # precision gate over every point of all 64 solves, before any timing
     errs, chosen = field_gate(ctx, grid, problems)
     lanes = [(ctx, problems, grid)]
-    for _ in range(args.streams - 1):
+    n_lanes = args.streams if args.schedule == "streams" else 1
+    for _ in range(n_lanes - 1):
         c2 = ncpamodes.Context(0)
         p2 = [c2.read_profile(p)[0] for p in paths]
         g2 = c2.grid(p2[0], Nz=NZ)
         lanes.append((c2, [ncpamodes.problem(pid, FREQ, az) for pid in p2 for az in AZIMUTHS], g2))
     # one stream: the stage split and the roofline (each kernel alone on the GPU); then the
-    # headline throughput with independent batches in flight on every stream
+    # headline throughput: the two-stage pipeline (default) or independent batches in flight
+    # on several streams
     res = run_field(lanes[:1], args, chosen)
-    multi = run_field(lanes, args, chosen) if len(lanes) > 1 else res
+    if args.schedule == "pipeline":
+        contractor = ncpamodes.Context(0)
+        multi = run_field_pipelined(ctx, contractor, problems, grid, args, chosen)
+        contractor.close()
+    else:
+        multi = run_field(lanes, args, chosen) if len(lanes) > 1 else res
     plain = run_field(lanes[:1], args, 0) if chosen != 0 else res
     for c, _, _ in lanes[1:]:
         c.close()
@@ -394,6 +460,10 @@ def field_main(args, cpu):
             "modes_per_step": res["total_modes"],
             "field_precision": PREC_NAMES[chosen],
             "device_streams": multi["streams"],
+            "schedule": ("two-stage pipeline: eigenpairs, 1-D TL, rows and field operand staging of step i "
+                         "on one stream while step i-1's field contraction runs on the other"
+                         if args.schedule == "pipeline"
+                         else "independent batches on %d streams" % multi["streams"]),
             "parallelism": "single GPU",
         },
         "tl_check_db": {PREC_NAMES[p]: {"max": e[0], "rms": e[1], "points": e[2]} for p, e in errs.items()},
@@ -640,8 +710,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["field", "broadband", "serving", "precision"], default="field")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", choices=["pipeline", "streams"], default="pipeline",
+                    help="field workload headline: two-stage pipeline over two streams, or "
+                         "independent batches on --streams streams")
     ap.add_argument("--streams", type=int, default=3,
-                    help="field workload: independent batches in flight on this many device streams")
+                    help="field workload with --schedule streams: batches in flight on this many streams")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run the plain copy kernel over MIB MiB first (byte-counter calibration)")
     ap.add_argument("--cpu-worker", help=argparse.SUPPRESS)

@@ -37,6 +37,7 @@ extern "C" {
 
 typedef struct ncpa_ctx ncpa_ctx_t;
 typedef struct ncpa_modes ncpa_modes_t;
+typedef struct ncpa_event ncpa_event_t;
 
 /* ---------------------------------------------------------------------------------- */
 /* context                                                                             */
@@ -58,6 +59,21 @@ void *ncpa_ctx_stream(ncpa_ctx_t *ctx);
 int ncpa_ctx_set_timing(ncpa_ctx_t *ctx, int enable);
 int ncpa_ctx_kernel_time(ncpa_ctx_t *ctx, const char *kernel, double *total_ms, int *launches);
 int ncpa_ctx_reset_timing(ncpa_ctx_t *ctx);
+
+/* Ordering between contexts (no counterpart in the reference, whose solver is one
+ * sequential loop; these let a caller pipeline the stages of consecutive batches over two
+ * contexts, e.g. eigenpairs of batch i+1 on one stream while the 2-D field of batch i is
+ * contracted on another).  ncpa_event_record marks the current end of ctx's stream;
+ * ncpa_event_wait makes work later queued on ctx wait for it (no host wait);
+ * ncpa_event_synchronize waits on the host.  A modes handle may be used on another context
+ * (e.g. ncpa_field2d) once that context waits for an event recorded after the work that
+ * produced it; its buffers belong to the context that solved it, so the caller keeps the
+ * handle until the other context's work on it has completed (an event waited for by the
+ * owning context, or synchronised on the host) before ncpa_modes_free. */
+int ncpa_event_record(ncpa_ctx_t *ctx, ncpa_event_t **ev);
+int ncpa_event_wait(ncpa_ctx_t *ctx, ncpa_event_t *ev);
+int ncpa_event_synchronize(ncpa_event_t *ev);
+void ncpa_event_destroy(ncpa_event_t *ev);
 
 /* Device memory owned by the caller (allocated on the context's device).  Copies are
  * ordered on the context stream; download and upload return once the host buffer may be
@@ -200,6 +216,22 @@ int ncpa_tl1d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, double dr, int n_r, int loss
  * Re(k_pert).  The phase k r is formed in FP64 and reduced mod 2 pi in every precision. */
 int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, const double *dev_psi_rows, int nrows,
                  double dr, int n_r, int lossless, int precision, void *dev_out);
+
+/* ncpa_field2d in two calls, so that the operand staging (phasors, splits, scales) and the
+ * contraction can run on different contexts (no counterpart in the reference, whose
+ * getTLoss2D is one loop): ncpa_field2d_stage queues the staging on ctx and returns a handle
+ * to the staged operands (buffers of ctx's pool); ncpa_field2d_contract queues the
+ * contraction on any context that has waited for an event recorded after the staging
+ * (ncpa_event_*), writing dev_out as ncpa_field2d does; ncpa_field_stage_free returns the
+ * buffers to the staging context, which must not run before the contraction has completed
+ * (wait for an event recorded after it, or synchronise).  The eigenvector rows are read by
+ * both calls (FP64 contraction: by the contraction only) and must stay valid until the
+ * contraction completes. */
+typedef struct ncpa_field_stage ncpa_field_stage_t;
+int ncpa_field2d_stage(ncpa_ctx_t *ctx, ncpa_modes_t *modes, const double *dev_psi_rows, int nrows,
+                       double dr, int n_r, int lossless, int precision, ncpa_field_stage_t **stage);
+int ncpa_field2d_contract(ncpa_ctx_t *ctx, ncpa_field_stage_t *stage, void *dev_out);
+void ncpa_field_stage_free(ncpa_field_stage_t *stage);
 
 /* Transmission-loss agreement of two fields of n_points complex values (device buffers laid
  * out as ncpa_field2d writes them): dev_ref complex128, dev_test complex128 if test_precision

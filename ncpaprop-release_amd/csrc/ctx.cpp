@@ -169,6 +169,44 @@ int ncpa_ctx_reset_timing(ncpa_ctx_t *ctx) {
     return NCPA_OK;
 }
 
+struct ncpa_event {
+    hipEvent_t e;
+};
+
+int ncpa_event_record(ncpa_ctx_t *ctx, ncpa_event_t **ev) {
+    if (!ctx || !ev) return NCPA_ERR_ARG;
+    *ev = nullptr;
+    NCPA_HIP(ctx, hipSetDevice(ctx->device));
+    ncpa_event *x = new (std::nothrow) ncpa_event{nullptr};
+    if (!x) return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_event_record: allocation failed");
+    hipError_t e = hipEventCreateWithFlags(&x->e, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(x->e, ctx->stream);
+    if (e != hipSuccess) {
+        if (x->e) (void)hipEventDestroy(x->e);
+        delete x;
+        return hip_error(ctx, e, "ncpa_event_record");
+    }
+    *ev = x;
+    return NCPA_OK;
+}
+
+int ncpa_event_wait(ncpa_ctx_t *ctx, ncpa_event_t *ev) {
+    if (!ctx || !ev) return set_error(ctx, NCPA_ERR_ARG, "ncpa_event_wait: bad arguments");
+    NCPA_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev->e, 0));
+    return NCPA_OK;
+}
+
+int ncpa_event_synchronize(ncpa_event_t *ev) {
+    if (!ev) return NCPA_ERR_ARG;
+    return hipEventSynchronize(ev->e) == hipSuccess ? NCPA_OK : NCPA_ERR_HIP;
+}
+
+void ncpa_event_destroy(ncpa_event_t *ev) {
+    if (!ev) return;
+    (void)hipEventDestroy(ev->e);
+    delete ev;
+}
+
 int ncpa_dev_alloc(ncpa_ctx_t *ctx, size_t bytes, void **dev_ptr) {
     if (!ctx || !dev_ptr) return NCPA_ERR_ARG;
     *dev_ptr = nullptr;

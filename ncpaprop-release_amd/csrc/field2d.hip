@@ -557,7 +557,7 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
                 *(u4 *)&Bs[s][v >> 2][lds_group(v >> 2, v & 3)] = rb[s][h];
             }
     };
-    fetch(0);
+    if (Kp > 0) fetch(0);   // a problem without modes has no operand block to read
     for (int k0 = 0; k0 < Kp; k0 += TKS) {
         __syncthreads();
         stash();
@@ -667,7 +667,7 @@ __global__ void __launch_bounds__(512, 1) k_gemm_fold(const SplitProb *__restric
                 for (int v = 0; v < 4; ++v) acc64[a][b][v] += (double)acc[a][b][v];
             }
     };
-    issue(As0, Bs0, 0);
+    if (Kp > 0) issue(As0, Bs0, 0);   // a problem without modes has no operand block to read
     for (int k0 = 0; k0 < Kp; k0 += 2 * TKS) {
         step(As0, Bs0, As1, Bs1, k0);
         if (k0 + TKS < Kp) step(As1, Bs1, As0, Bs0, k0 + TKS);
@@ -750,13 +750,47 @@ extern "C" int ncpa_modes_device(ncpa_modes_t *m, const int **off, const double 
 int ncpa_modes_internal(ncpa_modes_t *m, int *P, int *total, const int **h_offsets, const int **h_counts,
                         const double **d_psis);
 
+// Staged operands of one ncpa_field2d call: everything the contraction reads besides the
+// eigenvector rows, in device buffers of the staging context's pool.
+struct ncpa_field_stage {
+    ncpa_ctx *owner = nullptr;
+    int precision = 0, P = 0, M = 0, N = 0, total = 0;
+    // split precisions
+    int Mp = 0, Np = 0, nr_pad = 0;
+    size_t sa = 0, sb = 0;
+    SplitProb *d_sp = nullptr;
+    unsigned short *Ahat = nullptr, *Bhat = nullptr;
+    int *SA = nullptr, *SB = nullptr;
+    // FP64 / FP32
+    const double *psi_rows = nullptr;
+    GemmProb *d_gp = nullptr;
+    void *bh = nullptr;
+    float *ah = nullptr;
+    size_t lda = 0, ldb = 0;
+};
+
 namespace {
 
-template <int S, int NP, bool F16, bool FOLD = false>
-int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *hcnt, const int *d_off,
-                  const double *d_k2, const double *d_abs, const double *d_psis, const double *psi, int M,
-                  double dr, int n_r, int lossless, float *out) {
-    const int N = 2 * n_r;
+void stage_release(ncpa_field_stage *st) {
+    ncpa_ctx *c = st->owner;
+    c->pool.put(st->d_sp);
+    c->pool.put(st->Ahat);
+    c->pool.put(st->Bhat);
+    c->pool.put(st->SA);
+    c->pool.put(st->SB);
+    c->pool.put(st->d_gp);
+    c->pool.put(st->bh);
+    c->pool.put(st->ah);
+    delete st;
+}
+
+// Split operands: A rows and E columns cut into S 16-bit parts in the tiled layout (with
+// F16, the power-of-two row and range scales first).
+template <int S, bool F16>
+int stage_split(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int *hcnt, const int *d_off,
+                const double *d_k2, const double *d_abs, const double *d_psis, const double *psi, double dr,
+                int n_r, int lossless) {
+    const int P = st->P, M = st->M, N = st->N, total = st->total;
     const int Mp = (M + TBM - 1) / TBM * TBM, Np = (N + TBN - 1) / TBN * TBN;
     const int nr_pad = Np / 2;
     std::vector<SplitProb> sp(P);
@@ -769,33 +803,34 @@ int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *h
         sb += (size_t)Np * Kp;
         for (int k0 = 0; k0 < Kp; k0 += TKS) tiles.push_back({p, k0});
     }
-    SplitProb *d_sp = pool_alloc<SplitProb>(ctx, P);
+    st->Mp = Mp;
+    st->Np = Np;
+    st->nr_pad = nr_pad;
+    st->sa = sa;
+    st->sb = sb;
+    st->d_sp = pool_alloc<SplitProb>(ctx, P);
+    st->Ahat = pool_alloc<unsigned short>(ctx, std::max<size_t>(S * sa, 8));
+    st->Bhat = pool_alloc<unsigned short>(ctx, std::max<size_t>(S * sb, 8));
+    st->SA = F16 ? pool_alloc<int>(ctx, (size_t)P * Mp) : nullptr;
+    st->SB = F16 ? pool_alloc<int>(ctx, (size_t)P * nr_pad) : nullptr;
+    // needed only while staging
     ETile *d_tiles = pool_alloc<ETile>(ctx, std::max<size_t>(tiles.size(), 1));
-    unsigned short *Ahat = pool_alloc<unsigned short>(ctx, std::max<size_t>(S * sa, 8));
-    unsigned short *Bhat = pool_alloc<unsigned short>(ctx, std::max<size_t>(S * sb, 8));
-    int *SA = F16 ? pool_alloc<int>(ctx, (size_t)P * Mp) : nullptr;
-    int *SB = F16 ? pool_alloc<int>(ctx, (size_t)P * nr_pad) : nullptr;
     double *LG = F16 ? pool_alloc<double>(ctx, std::max(total, 1)) : nullptr;
     double *KI = F16 ? pool_alloc<double>(ctx, std::max(total, 1)) : nullptr;
-    auto cleanup = [&]() {
-        ctx->pool.put(d_sp);
+    auto put_temps = [&]() {
         ctx->pool.put(d_tiles);
-        ctx->pool.put(Ahat);
-        ctx->pool.put(Bhat);
-        ctx->pool.put(SA);
-        ctx->pool.put(SB);
         ctx->pool.put(LG);
         ctx->pool.put(KI);
     };
-    if (!d_sp || !d_tiles || !Ahat || !Bhat || (F16 && (!SA || !SB || !LG || !KI))) {
-        cleanup();
+    if (!st->d_sp || !st->Ahat || !st->Bhat || !d_tiles || (F16 && (!st->SA || !st->SB || !LG || !KI))) {
+        put_temps();
         return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_field2d: allocation failed");
     }
-    hipError_t e = hipMemcpyAsync(d_sp, sp.data(), sizeof(SplitProb) * P, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = hipMemcpyAsync(st->d_sp, sp.data(), sizeof(SplitProb) * P, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && !tiles.empty())
         e = hipMemcpyAsync(d_tiles, tiles.data(), sizeof(ETile) * tiles.size(), hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) {
-        cleanup();
+        put_temps();
         return hip_error(ctx, e, "ncpa_field2d copy");
     }
     const std::complex<double> I(0.0, 1.0);
@@ -803,95 +838,42 @@ int field2d_split(ncpa_ctx *ctx, int P, int total, const int *hoff, const int *h
     if (!tiles.empty()) {
         TimedLaunch tl(ctx, "emat");
         if (F16) {
-            hipLaunchKernelGGL(k_rowscale, dim3((Mp + 3) / 4, P), dim3(256), 0, ctx->stream, psi, (size_t)total, d_sp,
-                               d_off, M, Mp, SA);
+            hipLaunchKernelGGL(k_rowscale, dim3((Mp + 3) / 4, P), dim3(256), 0, ctx->stream, psi, (size_t)total,
+                               st->d_sp, d_off, M, Mp, st->SA);
             hipLaunchKernelGGL(k_mode_env, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis,
                                (size_t)total, lossless, ev.real(), ev.imag(), LG, KI);
-            hipLaunchKernelGGL(k_colscale, dim3((nr_pad + 255) / 256, P), dim3(256), 0, ctx->stream, d_sp, d_off, LG,
-                               KI, dr, n_r, nr_pad, SB);
+            hipLaunchKernelGGL(k_colscale, dim3((nr_pad + 255) / 256, P), dim3(256), 0, ctx->stream, st->d_sp, d_off,
+                               LG, KI, dr, n_r, nr_pad, st->SB);
         }
-        hipLaunchKernelGGL((k_asplit<S, F16>), dim3(64, P), dim3(256), 0, ctx->stream, psi, (size_t)total, d_sp,
-                           d_off, M, Mp, sa, SA, Ahat);
+        hipLaunchKernelGGL((k_asplit<S, F16>), dim3(64, P), dim3(256), 0, ctx->stream, psi, (size_t)total, st->d_sp,
+                           d_off, M, Mp, sa, st->SA, st->Ahat);
         hipLaunchKernelGGL((k_esplit<S, F16>), dim3((n_r + 63) / 64, (unsigned)tiles.size()), dim3(256), 0,
-                           ctx->stream, d_tiles, d_sp, d_off, d_k2, d_abs, d_psis, dr, n_r, lossless, ev.real(),
-                           ev.imag(), sb, SB, nr_pad, Bhat);
-    }
-    {
-        TimedLaunch tl(ctx, "field2d");
-        const int tn = Np / TBN, tm = Mp / TBM;
-        // 8 XCDs x ceil(tm / 2) M tiles x ceil(tn / 4) N tiles per XCD
-        dim3 g((unsigned)(8 * ((tm + 1) / 2) * ((tn + 3) / 4)), 1, (unsigned)P);
-        if constexpr (FOLD)
-            hipLaunchKernelGGL((k_gemm_fold<S, NP, F16>), g, dim3(512), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb, out,
-                               (size_t)N, M, N, tm, tn, SA, Mp, SB, nr_pad);
-        else
-            hipLaunchKernelGGL((k_gemm_split<S, NP, F16>), g, dim3(256), 0, ctx->stream, d_sp, Ahat, Bhat, sa, sb,
-                               out, (size_t)N, M, N, tm, tn, SA, Mp, SB, nr_pad);
+                           ctx->stream, d_tiles, st->d_sp, d_off, d_k2, d_abs, d_psis, dr, n_r, lossless, ev.real(),
+                           ev.imag(), sb, st->SB, nr_pad, st->Bhat);
     }
     e = hipGetLastError();
-    cleanup();
-    if (e != hipSuccess) return hip_error(ctx, e, "ncpa_field2d split launch");
+    put_temps();   // stream-ordered: later users of these blocks follow the staging kernels
+    if (e != hipSuccess) return hip_error(ctx, e, "ncpa_field2d staging launch");
     return NCPA_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, int nrows, double dr, int n_r,
-                 int lossless, int precision, void *dev_out) {
-    if (!ctx || !m || !dev_psi_rows || nrows <= 0 || n_r <= 0 || !(dr > 0.0) || !dev_out ||
-        precision < 0 || precision > PREC_FP16X4F)
-        return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d: bad arguments");
-    int P = 0, total = 0;
-    const int *hoff = nullptr, *hcnt = nullptr;
-    const double *d_psis = nullptr, *d_k2 = nullptr, *d_abs = nullptr;
-    ncpa_modes_internal(m, &P, &total, &hoff, &hcnt, &d_psis);
-    ncpa_modes_device(m, nullptr, &d_k2, &d_abs, nullptr);
-    if (total > 0 && !d_psis)
-        return set_error(ctx, NCPA_ERR_STATE, "ncpa_field2d: call ncpa_modes_rows first (it supplies psi(zs))");
-    if (precision >= PREC_BF16X3) {
-        const int *d_off = nullptr;
-        ncpa_modes_device(m, &d_off, nullptr, nullptr, nullptr);
-        if (precision == PREC_BF16X3)
-            return field2d_split<2, 3, false>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows,
-                                              nrows, dr, n_r, lossless, (float *)dev_out);
-        if (precision == PREC_BF16X6)
-            return field2d_split<3, 6, false>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows,
-                                              nrows, dr, n_r, lossless, (float *)dev_out);
-        if (precision == PREC_FP16X3)
-            return field2d_split<2, 3, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows,
-                                             nrows, dr, n_r, lossless, (float *)dev_out);
-        if (precision == PREC_BF16X6F)
-            return field2d_split<3, 6, false, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis,
-                                                    dev_psi_rows, nrows, dr, n_r, lossless, (float *)dev_out);
-        if (precision == PREC_FP16X4F)
-            return field2d_split<2, 4, true, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis,
-                                                   dev_psi_rows, nrows, dr, n_r, lossless, (float *)dev_out);
-        return field2d_split<2, 4, true>(ctx, P, total, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, nrows,
-                                         dr, n_r, lossless, (float *)dev_out);
-    }
-    const size_t ldb = 2 * (size_t)n_r, ldc = ldb, lda = (size_t)std::max(total, 1);
+// FP64 / FP32: the E matrix in the output precision (and A in FP32 for FP32).
+int stage_plain(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int *hcnt, const double *d_k2,
+                const double *d_abs, const double *d_psis, const double *psi, double dr, int n_r, int lossless) {
+    const int P = st->P, nrows = st->M, total = st->total, precision = st->precision;
+    st->ldb = 2 * (size_t)n_r;
+    st->lda = (size_t)std::max(total, 1);
+    st->psi_rows = psi;
     std::vector<GemmProb> gp(P);
-    for (int p = 0; p < P; ++p) gp[p] = {(size_t)hoff[p], (size_t)hoff[p], (size_t)p * nrows * ldc, hcnt[p]};
-    GemmProb *d_gp = pool_alloc<GemmProb>(ctx, P);
+    for (int p = 0; p < P; ++p) gp[p] = {(size_t)hoff[p], (size_t)hoff[p], (size_t)p * nrows * st->ldb, hcnt[p]};
+    st->d_gp = pool_alloc<GemmProb>(ctx, P);
     const size_t esz = precision == PREC_FP64 ? 8 : 4;
-    void *bh = ctx->pool.get(std::max<size_t>(total, 1) * ldb * esz);
-    float *ah = precision == PREC_FP64 ? nullptr : pool_alloc<float>(ctx, (size_t)nrows * lda);
-    auto cleanup = [&]() {
-        ctx->pool.put(d_gp);
-        ctx->pool.put(bh);
-        ctx->pool.put(ah);
-    };
-    if (!d_gp || !bh || (precision != PREC_FP64 && !ah)) {
-        cleanup();
+    st->bh = ctx->pool.get(std::max<size_t>(total, 1) * st->ldb * esz);
+    st->ah = precision == PREC_FP64 ? nullptr : pool_alloc<float>(ctx, (size_t)nrows * st->lda);
+    if (!st->d_gp || !st->bh || (precision != PREC_FP64 && !st->ah))
         return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_field2d: allocation failed");
-    }
-    hipError_t e = hipMemcpyAsync(d_gp, gp.data(), sizeof(GemmProb) * P, hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) {
-        cleanup();
-        return hip_error(ctx, e, "ncpa_field2d copy");
-    }
+    hipError_t e = hipMemcpyAsync(st->d_gp, gp.data(), sizeof(GemmProb) * P, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_error(ctx, e, "ncpa_field2d copy");
     const std::complex<double> I(0.0, 1.0);
     const std::complex<double> ev = 4 * PI * I * std::exp(-I * PI * 0.25) / std::sqrt(8.0 * PI);
     if (total > 0) {
@@ -899,27 +881,127 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, i
         dim3 ge((unsigned)(((n_r + 256 * EMAT_R - 1) / (256 * EMAT_R)) * (size_t)total));
         if (precision == PREC_FP64)
             hipLaunchKernelGGL(k_emat<PREC_FP64>, ge, dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis, dr, n_r,
-                               lossless, ev.real(), ev.imag(), bh);
+                               lossless, ev.real(), ev.imag(), st->bh);
         else
             hipLaunchKernelGGL(k_emat<PREC_FP32>, ge, dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis, dr, n_r,
-                               lossless, ev.real(), ev.imag(), bh);
+                               lossless, ev.real(), ev.imag(), st->bh);
         if (precision == PREC_FP32)
-            hipLaunchKernelGGL(k_amat, dim3(1024), dim3(256), 0, ctx->stream, dev_psi_rows, (size_t)nrows * lda, ah);
-    }
-    {
-        TimedLaunch tl(ctx, "field2d");
-        dim3 gg((unsigned)((ldb + BN - 1) / BN), (unsigned)((nrows + BM - 1) / BM), (unsigned)P);
-        if (precision == PREC_FP64)
-            hipLaunchKernelGGL((k_gemm_x4<double, 16>), gg, dim3(256), 0, ctx->stream, d_gp, dev_psi_rows, lda,
-                               (const double *)bh, ldb, (double *)dev_out, ldc, nrows, (int)ldb);
-        else
-            hipLaunchKernelGGL((k_gemm_x4<float, 16>), gg, dim3(256), 0, ctx->stream, d_gp, (const float *)ah, lda,
-                               (const float *)bh, ldb, (float *)dev_out, ldc, nrows, (int)ldb);
+            hipLaunchKernelGGL(k_amat, dim3(1024), dim3(256), 0, ctx->stream, psi, (size_t)nrows * st->lda, st->ah);
     }
     e = hipGetLastError();
-    cleanup();
+    if (e != hipSuccess) return hip_error(ctx, e, "ncpa_field2d staging launch");
+    return NCPA_OK;
+}
+
+template <int S, int NP, bool F16, bool FOLD>
+void launch_split(ncpa_ctx *ctx, const ncpa_field_stage *st, float *out) {
+    const int tn = st->Np / TBN, tm = st->Mp / TBM;
+    // 8 XCDs x ceil(tm / 2) M tiles x ceil(tn / 4) N tiles per XCD
+    dim3 g((unsigned)(8 * ((tm + 1) / 2) * ((tn + 3) / 4)), 1, (unsigned)st->P);
+    if constexpr (FOLD)
+        hipLaunchKernelGGL((k_gemm_fold<S, NP, F16>), g, dim3(512), 0, ctx->stream, st->d_sp, st->Ahat, st->Bhat,
+                           st->sa, st->sb, out, (size_t)st->N, st->M, st->N, tm, tn, st->SA, st->Mp, st->SB,
+                           st->nr_pad);
+    else
+        hipLaunchKernelGGL((k_gemm_split<S, NP, F16>), g, dim3(256), 0, ctx->stream, st->d_sp, st->Ahat, st->Bhat,
+                           st->sa, st->sb, out, (size_t)st->N, st->M, st->N, tm, tn, st->SA, st->Mp, st->SB,
+                           st->nr_pad);
+}
+
+int contract(ncpa_ctx *ctx, const ncpa_field_stage *st, void *dev_out) {
+    TimedLaunch tl(ctx, "field2d");
+    float *out = (float *)dev_out;
+    switch (st->precision) {
+    case PREC_BF16X3: launch_split<2, 3, false, false>(ctx, st, out); break;
+    case PREC_BF16X6: launch_split<3, 6, false, false>(ctx, st, out); break;
+    case PREC_FP16X3: launch_split<2, 3, true, false>(ctx, st, out); break;
+    case PREC_FP16X4: launch_split<2, 4, true, false>(ctx, st, out); break;
+    case PREC_BF16X6F: launch_split<3, 6, false, true>(ctx, st, out); break;
+    case PREC_FP16X4F: launch_split<2, 4, true, true>(ctx, st, out); break;
+    default: {
+        const size_t ldc = st->ldb;
+        dim3 gg((unsigned)((st->ldb + BN - 1) / BN), (unsigned)((st->M + BM - 1) / BM), (unsigned)st->P);
+        if (st->precision == PREC_FP64)
+            hipLaunchKernelGGL((k_gemm_x4<double, 16>), gg, dim3(256), 0, ctx->stream, st->d_gp, st->psi_rows,
+                               st->lda, (const double *)st->bh, st->ldb, (double *)dev_out, ldc, st->M, (int)st->ldb);
+        else
+            hipLaunchKernelGGL((k_gemm_x4<float, 16>), gg, dim3(256), 0, ctx->stream, st->d_gp, (const float *)st->ah,
+                               st->lda, (const float *)st->bh, st->ldb, (float *)dev_out, ldc, st->M, (int)st->ldb);
+    }
+    }
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(ctx, e, "ncpa_field2d launch");
     return NCPA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ncpa_field2d_stage(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, int nrows, double dr, int n_r,
+                       int lossless, int precision, ncpa_field_stage_t **stage) {
+    if (stage) *stage = nullptr;
+    if (!ctx || !m || !dev_psi_rows || nrows <= 0 || n_r <= 0 || !(dr > 0.0) || !stage || precision < 0 ||
+        precision > PREC_FP16X4F)
+        return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d: bad arguments");
+    int P = 0, total = 0;
+    const int *hoff = nullptr, *hcnt = nullptr, *d_off = nullptr;
+    const double *d_psis = nullptr, *d_k2 = nullptr, *d_abs = nullptr;
+    ncpa_modes_internal(m, &P, &total, &hoff, &hcnt, &d_psis);
+    ncpa_modes_device(m, &d_off, &d_k2, &d_abs, nullptr);
+    if (total > 0 && !d_psis)
+        return set_error(ctx, NCPA_ERR_STATE, "ncpa_field2d: call ncpa_modes_rows first (it supplies psi(zs))");
+    ncpa_field_stage *st = new (std::nothrow) ncpa_field_stage();
+    if (!st) return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_field2d: allocation failed");
+    st->owner = ctx;
+    st->precision = precision;
+    st->P = P;
+    st->M = nrows;
+    st->N = 2 * n_r;
+    st->total = total;
+    int rc = NCPA_OK;
+    switch (precision) {
+    case PREC_BF16X3:
+        rc = stage_split<2, false>(ctx, st, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, dr, n_r, lossless);
+        break;
+    case PREC_BF16X6:
+    case PREC_BF16X6F:
+        rc = stage_split<3, false>(ctx, st, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, dr, n_r, lossless);
+        break;
+    case PREC_FP16X3:
+    case PREC_FP16X4:
+    case PREC_FP16X4F:
+        rc = stage_split<2, true>(ctx, st, hoff, hcnt, d_off, d_k2, d_abs, d_psis, dev_psi_rows, dr, n_r, lossless);
+        break;
+    default: rc = stage_plain(ctx, st, hoff, hcnt, d_k2, d_abs, d_psis, dev_psi_rows, dr, n_r, lossless);
+    }
+    if (rc != NCPA_OK) {
+        stage_release(st);
+        return rc;
+    }
+    *stage = st;
+    return NCPA_OK;
+}
+
+int ncpa_field2d_contract(ncpa_ctx_t *ctx, ncpa_field_stage_t *stage, void *dev_out) {
+    if (!ctx || !stage || !dev_out) return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d_contract: bad arguments");
+    NCPA_HIP(ctx, hipSetDevice(ctx->device));
+    return contract(ctx, stage, dev_out);
+}
+
+void ncpa_field_stage_free(ncpa_field_stage_t *stage) {
+    if (stage) stage_release(stage);
+}
+
+int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, int nrows, double dr, int n_r,
+                 int lossless, int precision, void *dev_out) {
+    if (!dev_out) return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d: bad arguments");
+    ncpa_field_stage *st = nullptr;
+    int rc = ncpa_field2d_stage(ctx, m, dev_psi_rows, nrows, dr, n_r, lossless, precision, &st);
+    if (rc != NCPA_OK) return rc;
+    rc = contract(ctx, st, dev_out);
+    stage_release(st);   // stream-ordered on ctx: the blocks are reused only after the contraction
+    return rc;
 }
 
 int ncpa_field_compare(ncpa_ctx_t *ctx, const double *dev_ref, const void *dev_test, int test_precision,

@@ -47,6 +47,16 @@ class Context:
     def synchronize(self):
         self.check(self.lib.ncpa_ctx_synchronize(self.h))
 
+    def record(self):
+        """An Event at the current end of this context's stream."""
+        h = C.c_void_p()
+        self.check(self.lib.ncpa_event_record(self.h, C.byref(h)))
+        return Event(self.lib, h)
+
+    def wait(self, event):
+        """Work queued on this context after the call waits for `event` (no host wait)."""
+        self.check(self.lib.ncpa_event_wait(self.h, event.h))
+
     def field_compare(self, ref, test, test_precision, n_points, floor_db=-140.0):
         """(max, rms) |TL_test - TL_ref| [dB] of two device fields of n_points complex values
         over TL_ref > floor_db, and the number of such points (computed on the device)."""
@@ -122,6 +132,48 @@ def ess_eigen(ctx, diag, dz, k_min, k_max, max_modes=4000):
                                      max_modes, C.byref(nconv), _dp(k2), _dp(v), max_modes))
     n = nconv.value
     return k2[:n], v[:, :n]
+
+
+class Event:
+    """A point in one context's stream (ncpa_event_*): other contexts wait for it with
+    Context.wait, the host with synchronize()."""
+
+    def __init__(self, lib, h):
+        self.lib, self.h = lib, h
+
+    def synchronize(self):
+        if self.lib.ncpa_event_synchronize(self.h) != 0:
+            raise NcpaError(_lib.NCPA_ERR_HIP, "ncpa_event_synchronize failed")
+
+    def free(self):
+        if self.h:
+            self.lib.ncpa_event_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class FieldStage:
+    """Staged operands of one 2-D field (ncpa_field2d_stage)."""
+
+    def __init__(self, lib, h, n_out, precision):
+        self.lib, self.h, self.n_out, self.precision = lib, h, n_out, precision
+
+    def contract(self, ctx, out=None):
+        """Queue the contraction on `ctx` into `out` (allocated on ctx when None)."""
+        if out is None:
+            out = ctx.empty(self.n_out, np.float64 if self.precision == 0 else np.float32)
+        ctx.check(self.lib.ncpa_field2d_contract(ctx.h, self.h, C.c_void_p(out.ptr)))
+        return out
+
+    def free(self):
+        if self.h:
+            self.lib.ncpa_field_stage_free(self.h)
+            self.h = None
 
 
 class DeviceArray:
@@ -232,15 +284,26 @@ class Modes:
                                                     C.c_void_p(buf.ptr)))
         return buf
 
-    def field2d_device(self, psi_rows, nrows, dr, n_r, precision=0, lossless=False, out=None):
-        """2-D field into a device buffer [P, nrows, 2 n_r] (float64 for FP64 else float32)."""
+    def field2d_device(self, psi_rows, nrows, dr, n_r, precision=0, lossless=False, out=None, ctx=None):
+        """2-D field into a device buffer [P, nrows, 2 n_r] (float64 for FP64 else float32),
+        launched on `ctx` (default: the context that solved; another one must first wait for
+        an event recorded after ncpa_modes_rows)."""
+        c = self.ctx if ctx is None else ctx
         dt = np.float64 if precision == 0 else np.float32
         if out is None:
-            out = self.ctx.empty(self.P * nrows * 2 * n_r, dt)
-        self.ctx.check(self.ctx.lib.ncpa_field2d(self.ctx.h, self.h, C.c_void_p(psi_rows.ptr), nrows, dr,
-                                                 n_r, 1 if lossless else 0, precision,
-                                                 C.c_void_p(out.ptr)))
+            out = c.empty(self.P * nrows * 2 * n_r, dt)
+        c.check(c.lib.ncpa_field2d(c.h, self.h, C.c_void_p(psi_rows.ptr), nrows, dr, n_r,
+                                   1 if lossless else 0, precision, C.c_void_p(out.ptr)))
         return out
+
+    def field2d_stage(self, psi_rows, nrows, dr, n_r, precision=0, lossless=False):
+        """Queue the operand staging of the 2-D field on the solving context; the returned
+        FieldStage is contracted with FieldStage.contract (on any context that waited for an
+        event recorded after this call) and freed once that contraction has completed."""
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.ncpa_field2d_stage(self.ctx.h, self.h, C.c_void_p(psi_rows.ptr), nrows, dr,
+                                                       n_r, 1 if lossless else 0, precision, C.byref(h)))
+        return FieldStage(self.ctx.lib, h, self.P * nrows * 2 * n_r, precision)
 
     def field2d(self, row0, stride, nrows, dr, n_r, precision=0, lossless=False):
         """Host copy of the complex field [P, nrows, n_r]."""

@@ -35,13 +35,14 @@ NCPA_SRC = {"impulse": 0, "pulse1": 1, "pulse2": 2, "spectrum": 3, "waveform": 4
 EXPORTS = [
     "ncpa_ctx_create", "ncpa_ctx_destroy", "ncpa_ctx_error", "ncpa_ctx_synchronize",
     "ncpa_ctx_stream", "ncpa_ctx_set_timing", "ncpa_ctx_kernel_time", "ncpa_ctx_reset_timing",
+    "ncpa_event_record", "ncpa_event_wait", "ncpa_event_synchronize", "ncpa_event_destroy",
     "ncpa_dev_alloc", "ncpa_dev_free", "ncpa_dev_upload", "ncpa_dev_download", "ncpa_dev_copy",
     "ncpa_profile_upload", "ncpa_profile_set_attenuation", "ncpa_profile_clear",
     "ncpa_profile_read", "ncpa_profile_nodes", "ncpa_profile_admittance",
     "ncpa_grid_make", "ncpa_profile_sample",
     "ncpa_ess_solve", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
     "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d",
-    "ncpa_field_compare", "ncpa_ess_eigen",
+    "ncpa_field2d_stage", "ncpa_field2d_contract", "ncpa_field_stage_free", "ncpa_field_compare", "ncpa_ess_eigen",
     "ncpa_fft_batch", "ncpa_kpert", "ncpa_bb_transfer", "ncpa_bb_pulse", "ncpa_bb_source",
     "ncpa_bb_plan_create", "ncpa_bb_plan_set_problems", "ncpa_bb_plan_run", "ncpa_bb_plan_result",
     "ncpa_bb_plan_tuning", "ncpa_bb_plan_destroy",
@@ -100,6 +101,10 @@ def load():
         "ncpa_ctx_set_timing": (C.c_int, [vp, C.c_int]),
         "ncpa_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
         "ncpa_ctx_reset_timing": (C.c_int, [vp]),
+        "ncpa_event_record": (C.c_int, [vp, P(vp)]),
+        "ncpa_event_wait": (C.c_int, [vp, vp]),
+        "ncpa_event_synchronize": (C.c_int, [vp]),
+        "ncpa_event_destroy": (None, [vp]),
         "ncpa_dev_alloc": (C.c_int, [vp, C.c_size_t, P(vp)]),
         "ncpa_dev_free": (C.c_int, [vp, vp]),
         "ncpa_dev_upload": (C.c_int, [vp, vp, vp, C.c_size_t]),
@@ -124,6 +129,9 @@ def load():
         "ncpa_modes_ceff": (C.c_int, [vp, C.c_int, dp]),
         "ncpa_tl1d": (C.c_int, [vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp]),
         "ncpa_field2d": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, vp]),
+        "ncpa_field2d_stage": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, P(vp)]),
+        "ncpa_field2d_contract": (C.c_int, [vp, vp, vp]),
+        "ncpa_field_stage_free": (None, [vp]),
         "ncpa_field_compare": (C.c_int, [vp, vp, vp, C.c_int, C.c_size_t, C.c_double, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.POINTER(C.c_size_t)]),
         "ncpa_ess_eigen": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int,
