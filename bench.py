@@ -209,6 +209,17 @@ def field_gate(ctx, grid, problems):
     return errs, chosen
 
 
+def issued_mfma_flops(prec, counts):
+    """MFMA flops one split contraction issues: 2 x (rows padded to 128) x (2 ranges padded to
+    128) x (sum of modes padded to 32) x products; None for the FP64 / FP32 forms."""
+    if prec not in PREC_KERNEL:
+        return None
+    mp = -(-FIELD_ROWS // 128) * 128
+    np_ = -(-2 * NR // 128) * 128
+    kp = sum(-(-int(k) // 32) * 32 for k in counts)
+    return 2.0 * mp * np_ * kp * PRODUCTS[prec]
+
+
 def run_field(lanes, args, prec):
     """Timed loop of the config-1 step at one field precision; returns a timing dict.
 
@@ -279,7 +290,8 @@ def run_field(lanes, args, prec):
     iters = np.abs(last.iterations()).astype(np.float64)
     res = {"elapsed": elapsed, "fams": fams, "total_modes": int(last.total),
            "sturm_steps": float((iters + 1.0).sum()) * NZ,
-           "field_flops": 4.0 * FIELD_ROWS * last.total * NR, "streams": L}
+           "field_flops": 4.0 * FIELD_ROWS * last.total * NR, "streams": L,
+           "field_mfma_flops": issued_mfma_flops(prec, last.counts)}
     last.free()
     for st in lane_state:
         for b in st["tl"] + [st.get("rows"), st.get("out")]:
@@ -329,14 +341,17 @@ def run_field_pipelined(solver, contractor, problems, grid, args, prec, depth=2)
         inflight.append((m, stage, contractor.record()))
         retire(depth)
 
-    for i in range(args.warmup):
+    # untimed warm-up: at least depth + 1 steps, so that every rotating row buffer and every
+    # in-flight set of staged operands has been allocated before the timed region
+    n_warm = max(args.warmup, depth + 1)
+    for i in range(n_warm):
         step(i)
     retire(0)
     solver.synchronize()
     contractor.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step(args.warmup + i)
+        step(n_warm + i)
     retire(0)
     solver.synchronize()
     contractor.synchronize()
@@ -427,6 +442,13 @@ def field_main(args, cpu):
                 "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": measured_traffic(kname),
                 "algorithmic": "4 x depths x modes x ranges real flops per problem (real x complex)"}
+        if res.get("field_mfma_flops"):
+            issued = res["field_mfma_flops"] / avg_s / 1e12
+            roof["issued_mfma"] = {
+                "tflops": issued, "frac": issued / peak,
+                "note": "MFMA work the kernel issues: %d part products per algorithmic product, "
+                        "padded shapes; SQ_INSTS_VALU_MFMA_MOPS x 512 agrees (profiles/"
+                        "field_mfma_counters.md)" % PRODUCTS[chosen]}
     else:
         avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
         achieved = FLOPS_PER_STURM_STEP * res["sturm_steps"] / avg_s / 1e12

@@ -588,27 +588,27 @@ typedef __attribute__((address_space(3))) void lds_void;
 // are added to FP64 accumulators, so FP32 sums run over the 32 modes of one k-step only
 // (the FP32 running sum over all modes is what limits the plain forms at interference nulls,
 // where the partial sums exceed the result by orders of magnitude).  The FP64 accumulators
-// take the registers of half the columns, so a wave covers 64 x 32 and a 128 x 128 block
-// takes 8 waves (2 x 4).  Operands go from global memory straight into LDS
-// (global_load_lds_dwordx4: 16 B per lane, one 1 KB wave instruction = 16 tile rows, LDS
-// address = base + 16 lane) into two stages: one barrier per k-step, the next k-step's loads
-// in flight during the current MFMAs.  The XOR swizzle of the 16-byte groups is applied on the
-// global side (lane -> row, stored group; the group loaded is the logical one), so the
-// fragment reads are those of k_gemm_split.  The two stages are separate arrays so that the
-// compiler can tell the loads into one from the reads of the other (no wait on the loads in
-// flight).  96 KB of LDS and one block per CU leave room beside it for the latency-bound
-// eigenvalue waves of another stream (a 128 x 64 block of 4 waves at 72 KB, two per CU, ran
-// alone 3 % faster but left room for one such wave per CU).
+// take the registers of half the columns: block tile 128 x 64, 4 waves of 64 x 32.
+// Operands go from global memory straight into LDS (global_load_lds_dwordx4: 16 B per lane,
+// one 1 KB wave instruction = 16 tile rows, LDS address = base + 16 lane) into two stages:
+// one barrier per k-step, the next k-step's loads in flight during the current MFMAs.  The
+// XOR swizzle of the 16-byte groups is applied on the global side (lane -> row, stored
+// group; the group loaded is the logical one), so the fragment reads are those of the
+// register-staged kernel.  The two stages are separate arrays so that the compiler can tell
+// the loads into one from the reads of the other (no wait on the loads in flight).
 template <int S, int NP, bool F16>
-__global__ void __launch_bounds__(512, 1) k_gemm_fold(const SplitProb *__restrict__ sp,
+__global__ void __launch_bounds__(256, 2) k_gemm_fold(const SplitProb *__restrict__ sp,
                                                       const unsigned short *__restrict__ Ahat,
                                                       const unsigned short *__restrict__ Bhat,
                                                       size_t split_stride_a, size_t split_stride_b,
                                                       float *__restrict__ C, size_t ldc, int M, int N,
                                                       int tiles_m, int n_tiles_n, const int *__restrict__ SA, int Mp,
                                                       const int *__restrict__ SB, int nr_pad) {
-    typedef unsigned short Tile[TBM][TKS];     // 128 rows (A) or columns (B) x 32 modes
-    __shared__ __attribute__((aligned(16))) Tile As0[S], As1[S], Bs0[S], Bs1[S];
+    constexpr int BN = 64;
+    typedef unsigned short ATile[TBM][TKS];
+    typedef unsigned short BTile[BN][TKS];
+    __shared__ __attribute__((aligned(16))) ATile As0[S], As1[S];
+    __shared__ __attribute__((aligned(16))) BTile Bs0[S], Bs1[S];
     const SplitProb P = sp[blockIdx.z];
     // XCD mapping as in k_gemm_split
     const int tmh = (tiles_m + 1) / 2;
@@ -616,9 +616,9 @@ __global__ void __launch_bounds__(512, 1) k_gemm_fold(const SplitProb *__restric
     const int tm = (xcd & 1) * tmh + slot % tmh;
     const int tn = (slot / tmh) * 4 + (xcd >> 1);
     if (tn >= n_tiles_n || tm >= tiles_m) return;
-    const int m0 = tm * TBM, n0 = tn * TBN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;   // 8 waves
-    const int wm = (wave >> 2) * 64, wn = (wave & 3) * 32;
+    const int m0 = tm * TBM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 32;
     f4 acc[4][2];
     double acc64[4][2][4];
 #pragma unroll
@@ -629,22 +629,28 @@ __global__ void __launch_bounds__(512, 1) k_gemm_fold(const SplitProb *__restric
             for (int v = 0; v < 4; ++v) acc64[a][b][v] = 0.0;
     const unsigned short *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
     const int Kp = P.Kp;
-    // k-step k0 into one stage: per part, wave w loads rows 16w .. 16w + 15 of A and of B
-    const int row = wave * 16 + (lane >> 2);
-    const int g = (lane & 3) ^ ((row >> 2) & 3);
-    auto issue = [&](Tile *Ax, Tile *Bx, int k0) {
+    const int crow = lane >> 2, gs = lane & 3;
+    // k-step k0 into one stage: A = 8 chunks of 16 rows (2 per wave), B = 4 chunks (1 per wave)
+    auto issue = [&](ATile *Ax, BTile *Bx, int k0) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            __builtin_amdgcn_global_load_lds(
-                (const void *)&Ab[s * split_stride_a + tiled_index(m0 + row, k0 + 8 * g, Kp)],
-                (lds_void *)&Ax[s][wave * 16][0], 16, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int ch = 2 * wave + j, row = ch * 16 + crow;
+                const int g = gs ^ ((row >> 2) & 3);
+                __builtin_amdgcn_global_load_lds(
+                    (const void *)&Ab[s * split_stride_a + tiled_index(m0 + row, k0 + 8 * g, Kp)],
+                    (lds_void *)&Ax[s][ch * 16][0], 16, 0, 0);
+            }
+            const int row = wave * 16 + crow;
+            const int g = gs ^ ((row >> 2) & 3);
             __builtin_amdgcn_global_load_lds(
                 (const void *)&Bb[s * split_stride_b + tiled_index(n0 + row, k0 + 8 * g, Kp)],
                 (lds_void *)&Bx[s][wave * 16][0], 16, 0, 0);
         }
     };
     const int kq = lds_group(lane & 15, lane >> 4);
-    auto step = [&](Tile *Ac, Tile *Bc, Tile *An, Tile *Bn, int k0) {
+    auto step = [&](ATile *Ac, BTile *Bc, ATile *An, BTile *Bn, int k0) {
         // this wave's loads of the current stage (the only ones in flight), then every wave's
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
@@ -895,11 +901,12 @@ int stage_plain(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
 
 template <int S, int NP, bool F16, bool FOLD>
 void launch_split(ncpa_ctx *ctx, const ncpa_field_stage *st, float *out) {
-    const int tn = st->Np / TBN, tm = st->Mp / TBM;
+    // Np is a multiple of 128 (the E-staging tiles), so of either block width
+    const int tn = st->Np / (FOLD ? 64 : TBN), tm = st->Mp / TBM;
     // 8 XCDs x ceil(tm / 2) M tiles x ceil(tn / 4) N tiles per XCD
     dim3 g((unsigned)(8 * ((tm + 1) / 2) * ((tn + 3) / 4)), 1, (unsigned)st->P);
     if constexpr (FOLD)
-        hipLaunchKernelGGL((k_gemm_fold<S, NP, F16>), g, dim3(512), 0, ctx->stream, st->d_sp, st->Ahat, st->Bhat,
+        hipLaunchKernelGGL((k_gemm_fold<S, NP, F16>), g, dim3(256), 0, ctx->stream, st->d_sp, st->Ahat, st->Bhat,
                            st->sa, st->sb, out, (size_t)st->N, st->M, st->N, tm, tn, st->SA, st->Mp, st->SB,
                            st->nr_pad);
     else
