@@ -33,3 +33,22 @@ def test_no_device_context_fails_loudly(monkeypatch):
     except ncpamodes.NcpaError:
         return
     ctx.close()   # a real device is present (GPU box): nothing to check here
+
+
+def test_null_arguments_return_codes():
+    """Entry points given null handles return an argument error (no device access, no crash)."""
+    from ncpamodes import _lib
+    lib = _lib.load()
+    vp = ctypes.c_void_p
+    out = vp()
+    assert lib.ncpa_event_record(None, ctypes.byref(out)) == _lib.NCPA_ERR_ARG
+    assert lib.ncpa_event_wait(None, None) == _lib.NCPA_ERR_ARG
+    assert lib.ncpa_event_synchronize(None) == _lib.NCPA_ERR_ARG
+    lib.ncpa_event_destroy(None)
+    lib.ncpa_field_stage_free(None)
+    assert lib.ncpa_field2d_contract(None, None, None) == _lib.NCPA_ERR_ARG
+    assert lib.ncpa_field2d_stage(None, None, None, 10, 1.0, 10, 0, 6, ctypes.byref(out)) == _lib.NCPA_ERR_ARG
+    assert out.value is None
+    mx, rms, cnt = ctypes.c_double(), ctypes.c_double(), ctypes.c_size_t()
+    assert lib.ncpa_field_compare(None, None, None, 6, 10, -140.0, ctypes.byref(mx), ctypes.byref(rms),
+                                  ctypes.byref(cnt)) == _lib.NCPA_ERR_ARG
