@@ -118,3 +118,33 @@ def test_problem_without_modes_last(gpu_ctx, synth_profile, prec):
     m1.free()
     assert np.all(F[1] == 0)
     assert np.array_equal(F[0], F1[0])
+
+
+@pytest.mark.parametrize("prec", [0, 1, 4, 5, 6, 7])
+def test_stage_contract_equals_field2d(gpu_ctx, synth_profile, prec):
+    """ncpa_field2d_stage on one context + ncpa_field2d_contract on another after an event gives
+    the same field, bit for bit, as the one-call ncpa_field2d (every precision family: FP64 and
+    FP32 staging, the scaled FP16 splits, the folded forms)."""
+    import ncpamodes
+    print("case stage/contract precision %d" % prec, flush=True)
+    pid, _ = gpu_ctx.read_profile(synth_profile)
+    g = gpu_ctx.grid(pid)
+    m = gpu_ctx.solve(g, _probs(ncpamodes, pid, [70.0, 160.0]))
+    rows = m.rows_device(0, g.Nz // 500, ROWS)
+    ref = m.field2d_device(rows, ROWS, DR, NR, prec)
+    other = ncpamodes.Context(0)
+    st = m.field2d_stage(rows, ROWS, DR, NR, prec)
+    ev = gpu_ctx.record()
+    other.wait(ev)
+    ev.free()
+    got = st.contract(other)
+    done = other.record()
+    done.synchronize()
+    gpu_ctx.wait(done)
+    done.free()
+    st.free()
+    assert np.array_equal(got.numpy(), ref.numpy())
+    for x in (got, ref, rows):
+        x.free()
+    m.free()
+    other.close()
