@@ -252,6 +252,31 @@ int ncpa_ess_eigen(ncpa_ctx_t *ctx, const double *diag, int Nz, double dz, doubl
                    const double *k_min, const double *k_max, int max_modes, int *nconv,
                    double *k2, double *v, int ldv);
 
+/* ---- wide-angle, high-Mach modes (WMod, ModBB --method wmod) ---------------------- */
+
+/* Solve a batch with the wide-angle solver: WModeSolver::getModalTrace +
+ * EigenEngine::doWideAngleCalculation + doSelect + doPerturb (src/libmodes/WModeSolver.cpp:
+ * 322-380, 491-629; EigenEngine.cpp:174-395; ModeSolver.cpp:263-284).  `grid` is the
+ * setParams grid of ncpa_grid_make (profile samples every (maxheight - z_min)/(Nz - 1));
+ * the matrix, the ground term, the WKB integral, the perturbation and the source/receiver
+ * rows use dz = (maxheight - z_min)/Nz as WModeSolver::solve does (:297), and the result's
+ * grid carries that dz.  Every real eigenvalue k of the quadratic problem in [k_min, k_max]
+ * is returned (ascending), psi normalised as the reference's first half of a unit-norm
+ * linearised eigenvector: sum psi^2 dz = 1/(1 + k^2).  The modes serve ncpa_modes_*,
+ * ncpa_tl1d, ncpa_modes_rows and ncpa_field2d exactly as those of ncpa_ess_solve. */
+int ncpa_wmod_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems,
+                    int n_problems, ncpa_modes_t **out);
+
+/* Drop-in for EigenEngine::doWideAngleCalculation + WModeSolver::doSelect
+ * (EigenEngine.cpp:174-395, WModeSolver.cpp:491-506): kd, md, cd as getModalTrace fills them
+ * (kd[0] including the ground term).  Returns every real eigenvalue of
+ * (k^2 diag(md) + k diag(cd) + D) v = 0 in [k_min, k_max], kH[nconv] ascending, and
+ * v[j*ldv + mode] = x_j/sqrt(dz) for the unit-norm linearised eigenvector x = (v, k v).
+ * tol is accepted for signature compatibility.  NCPA_ERR_MAX_MODES above max_modes. */
+int ncpa_wmod_eigen(ncpa_ctx_t *ctx, const double *kd, const double *md, const double *cd, int Nz,
+                    double dz, double tol, double k_min, double k_max, int max_modes, int *nconv,
+                    double *kH, double *v, int ldv);
+
 /* ---- broadband pulse synthesis (ModBB --propagation) ------------------------------ */
 
 /* Batched complex FFT of `batch` contiguous transforms of n points (interleaved re/im

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libncpamodes.so")
-SOURCES = ["csrc/ctx.cpp", "csrc/ess.hip", "csrc/field2d.hip", "csrc/broadband.hip", "csrc/plan.cpp",
+SOURCES = ["csrc/ctx.cpp", "csrc/ess.hip", "csrc/wide.hip", "csrc/field2d.hip", "csrc/broadband.hip", "csrc/plan.cpp",
            "host/atmosphere.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -28,6 +28,7 @@
 #include "ctx.h"
 #include "sturm.h"
 #include "internal.h"
+#include "modes.h"
 
 namespace ncpa {
 
@@ -208,37 +209,6 @@ __global__ void k_setup_grid(const ProbDev *__restrict__ probs, int Nz, double z
 // ------------------------------------------------------------------------------------
 // interval: one block (256 threads) per problem
 // ------------------------------------------------------------------------------------
-__device__ double block_min(double v, double *sh) {
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double r = sh[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = fmin(r, sh[k]);
-    __syncthreads();
-    return r;
-}
-__device__ double block_sum(double v, double *sh) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double r = sh[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r += sh[k];
-    __syncthreads();
-    return r;
-}
-__device__ int block_min_int(int v, int *sh) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    int r = sh[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = min(r, sh[k]);
-    __syncthreads();
-    return r;
-}
-
 __global__ void __launch_bounds__(256) k_interval(const ProbDev *__restrict__ probs, int Nz, double dz,
                                                   const double *__restrict__ A,
                                                   const double *__restrict__ CE,
@@ -249,51 +219,9 @@ __global__ void __launch_bounds__(256) k_interval(const ProbDev *__restrict__ pr
     __shared__ double s_ring[NCPA_RING];
     const int p = blockIdx.x;
     const ProbDev pr = probs[p];
-    const double *ce = CE + size_t(p) * Nz;
-    const double om = pr.omega;
-    double m = DBL_MAX;
-    for (int i = threadIdx.x; i < Nz; i += blockDim.x) m = fmin(m, ce[i]);
-    const double ceffmin = fmin(block_min(m, shd), ce[0]);
-    double k_max;
-    int used_wkb = 0;
-    if ((pr.flags & NCPA_F_WKB) && fabs(pr.zs) < 1.0e-3 && fabs(pr.zr) < 1.0e-3) {
-        // WKB cut-off for ground-to-ground propagation (ESSModeSolver.cpp:563-602)
-        used_wkb = 1;
-        const double kmf = om / ceffmin;
-        const double kg = om / ce[0];
-        const double kmf2 = kmf * kmf, kg2 = kg * kg;
-        const double dkk = (kmf2 - kg2) / 100.0;
-        double kk = kg2;
-        if (dkk > 1.0e-10) {
-            // the reference steps ~100 times; the cap only bounds a degenerate dkk
-            int steps = 0;
-            for (kk = kg2; kk < kmf2 && steps < 1000; kk = kk + dkk, ++steps) {
-                int first = Nz - 1;
-                for (int i = threadIdx.x; i < Nz; i += blockDim.x) {
-                    const double ke = om / ce[i];
-                    if (fabs(kk - ke * ke) <= dkk) { first = min(first, i); break; }
-                }
-                first = block_min_int(first, shi);
-                double s = 0.0;
-                for (int i = threadIdx.x; i <= first; i += blockDim.x) {
-                    const double ke = om / ce[i];
-                    s += dz * sqrt(fabs(kk - ke * ke));
-                }
-                const double integral = block_sum(s, shd);
-                if (integral >= 10.0) break;
-            }
-        }
-        k_max = sqrt(kk);
-    } else {
-        k_max = om / ceffmin;
-    }
-    const int top = Nz - Nz / 10;
-    double k_min = om / ce[min(top + 1, Nz - 1)];   // Nz < 20 would read past the grid
-    if (used_wkb && k_max < k_min) k_max = om / ceffmin;
-    if (pr.flags & NCPA_F_WVNUM_FILTER) {
-        k_min = om / pr.c_max;   // 2*PI*freq/c_max (ESSModeSolver.cpp:366-369)
-        k_max = om / pr.c_min;
-    }
+    double k_min, k_max;
+    int used_wkb;
+    interval_bounds(pr, Nz, dz, CE + size_t(p) * Nz, shd, shi, true, k_min, k_max, used_wkb);
     const double lo = k_min * k_min, hi = k_max * k_max;
     if (threadIdx.x == 0) { s_lohi[0] = lo; s_lohi[1] = hi; }
     __syncthreads();
@@ -323,167 +251,49 @@ __global__ void k_counts2(const double *__restrict__ a, int n, double lo, double
 }
 
 
-#define NCPA_MAXIT 200
-
-// One lane per eigenvalue index j (ascending), 64 consecutive indices of one problem per
-// block.  Phase 0 narrows every bracket by multisection (see below); phase 1 bisects with count-only sweeps until [lo, hi) holds only eigenvalue j;
-// phase 2 runs Newton on det(T - s) (count kept as a safeguard) until the step is below
-// the working-precision tolerance; phase 3 is one sweep at the converged eigenvalue that
-// yields the absorption and the adjugate eigenvector products.  iters_out: sweeps used,
-// negative when the bracket collapsed before Newton converged (cluster at working precision).
-struct ModesShared {
-    double ms_s[64];
-    int ms_c[64];
-    double hb_lo[64], hb_hi[64];   // brackets of the unconverged eigenvalues
-    int h_list[64];                // their lanes, in lane order
-    double ring[2 * NCPA_RING];
-};
-
-template <int MS>
-__device__ __forceinline__ void modes_item(const ModeProb &P, const ModeWork wk, ModesShared &sh_,
-                                           const ModesOut &out) {
-    double *ms_s = sh_.ms_s, *hb_lo = sh_.hb_lo, *hb_hi = sh_.hb_hi, *s_ring = sh_.ring;
-    int *ms_c = sh_.ms_c, *h_list = sh_.h_list;
-    const int lane = (int)threadIdx.x;
-    const int j = wk.j0 + lane;
-    const bool active = j < P.n_hi;
-    double lo = P.lo, hi = P.hi;
-    int cl = P.n_lo, ch = P.n_hi;
-    // phase 0: multisection.  The 64 lanes evaluate the count at 64 distinct shifts spread
-    // over the bracket of the block's eigenvalues; every lane then takes the tightest
-    // bracket of its own eigenvalue from the 64 counts.  Two passes isolate most eigenvalues,
-    // work that bisection spends ~log2(n) sweeps on per lane.
-    const int last = min(P.n_hi - 1, wk.j0 + 63) - wk.j0;   // last active lane
-    double blo = lo, bhi = hi;
-    for (int pass = 0; pass < MS; ++pass) {
-        const double sh = blo + (double)(lane + 1) * ((bhi - blo) * (1.0 / 65.0));
-        const int c = sturm_count(P.a, s_ring, P.n, sh);
-        ms_s[lane] = sh;
-        ms_c[lane] = c;
-        __syncthreads();
-        for (int l = 0; l < 64; ++l) {           // counts are non-decreasing in the shift
-            const int cc = ms_c[l];
-            const double ss = ms_s[l];
-            if (cc <= j) {
-                if (ss > lo) { lo = ss; cl = cc; }
-            } else if (ss < hi) {
-                hi = ss;
-                ch = cc;
-            }
-        }
-        __syncthreads();
-        blo = __shfl(lo, 0);
-        bhi = __shfl(hi, last);
+// dz^2 T - s with T the effective-sound-speed operator: one diagonal array
+struct EssPencil {
+    static constexpr bool early_accept = true;
+    const ModeProb &p;
+    __device__ __forceinline__ int count(double *ring, double s) const { return sturm_count(p.a, ring, p.n, s); }
+    __device__ __forceinline__ NewtonOut newton(double *ring, double s) const { return sturm_newton(p.a, ring, p.n, s); }
+    // rows pass: shift of a mode with eigenvalue k2, row values a_i - s, q_0 unscaled
+    __device__ __forceinline__ double shift(double k2) const { return k2 * p.h2; }
+    __device__ __forceinline__ double qscale(double) const { return 1.0; }
+    template <class Step, class ChunkEnd>
+    __device__ __forceinline__ void up(double *ring, int m, double s, Step step, ChunkEnd ce) const {
+        sweep_up(p.a, ring, m, [&](double ak, int k) { step(ak - s, k); }, ce);
     }
-    double sig = 0.5 * (lo + hi);
-    bool done = !active;
-    int my_iters = MS;
-    bool collapsed = false;
-    // phases 1-2.  Lanes whose eigenvalue has converged (and lanes past the problem's last
-    // index) do not idle: each iteration they are dealt round-robin to the unconverged
-    // eigenvalues of the block and evaluate extra shifts spread over those brackets; every
-    // unconverged lane then tightens its bracket from all 64 (shift, count) pairs, so the
-    // stragglers that set the block's duration converge several bits per sweep.
-    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    double prev_step = INFINITY;   // the last Newton step taken (INFINITY after a bisection)
-    for (int it = 0; it < NCPA_MAXIT; ++it) {
-        const unsigned long long um = __ballot(!done);
-        if (um == 0ull) break;
-        const int nu = __popcll(um);
-        if (!done) {
-            hb_lo[lane] = lo;
-            hb_hi[lane] = hi;
-            h_list[__popcll(um & below)] = lane;
-        }
-        __syncthreads();
-        double sh = sig;
-        if (done) {
-            const int h = __popcll(~um & below);        // rank among the helpers
-            const int idx = h % nu, kk = h / nu;
-            const int nh = (64 - nu - idx + nu - 1) / nu;  // helpers dealt to eigenvalue idx
-            const int u = h_list[idx];
-            const double a = hb_lo[u], b = hb_hi[u];
-            sh = a + (double)(kk + 1) * ((b - a) / (double)(nh + 1));
-        }
-        const bool newton = !done && (ch - cl == 1);
-        int cnt;
-        double p0 = 0.0, q0 = 0.0;
-        if (__any(newton)) {
-            const NewtonOut o = sturm_newton(P.a, s_ring, P.n, sh);
-            cnt = o.cnt;
-            p0 = o.p0;
-            q0 = o.q0;
-        } else {
-            cnt = sturm_count(P.a, s_ring, P.n, sh);
-        }
-        ms_s[lane] = sh;
-        ms_c[lane] = cnt;
-        __syncthreads();
-        if (!done) {
-            for (int l = 0; l < 64; ++l) {           // counts are non-decreasing in the shift
-                const int cc = ms_c[l];
-                const double ss = ms_s[l];
-                if (cc <= j) {
-                    if (ss > lo) { lo = ss; cl = cc; }
-                } else if (ss < hi) {
-                    hi = ss;
-                    ch = cc;
-                }
-            }
-            sig = 0.5 * (lo + hi);
-            double step = INFINITY;
-            if (newton && q0 != 0.0 && isfinite(p0) && isfinite(q0)) {
-                const double dlt = -p0 / q0;
-                const double cand = sh + dlt;
-                const double adl = fabs(dlt);
-                // Converged when the step is at the tolerance, or when two successive Newton
-                // steps show the quadratic regime (ratio below 1e-3) and predict the next
-                // step, |dlt|^3 / |dlt_prev|^2, at the tolerance: the sweep that would only
-                // confirm it is skipped.
-                const double rat = adl / prev_step;   // 0 after a bisection: no estimate yet
-                if (adl <= P.tol || (prev_step < INFINITY && rat <= 1e-3 && adl * rat * rat <= P.tol)) {
-                    sig = cand;
-                    done = true;
-                    my_iters = MS + it + 1;
-                } else if (cand > lo && cand < hi) {
-                    sig = cand;
-                    step = adl;
-                }
-            }
-            prev_step = step;
-            if (!done && (hi - lo) <= P.tol) {
-                sig = 0.5 * (lo + hi);
-                done = true;
-                collapsed = (ch - cl) != 1;
-                my_iters = -(MS + it + 1);
-            }
-        }
-        __syncthreads();
+    template <class Step, class ChunkEnd>
+    __device__ __forceinline__ void down(double *ring, int m, double s, Step step, ChunkEnd ce) const {
+        sweep_down(p.a, ring, m, [&](double ai, int i) { step(ai - s, i); }, ce);
     }
-    if (!done) my_iters = -NCPA_MAXIT - 1;
     // phase 3: sensitivities at the converged eigenvalue (every lane, one sweep)
-    const FinalOut f = sturm_final(P.a, P.w, s_ring, P.n, sig, P.srow, P.rrow);
-    SVal th_s, th_r;
-    theta_forward(P.a, s_ring, sig, P.srow, P.rrow, th_s, th_r);
-    if (!active) return;   // (whole-wave work above is done; nothing of this lane is left)
-    const double mult = collapsed ? (double)(ch - cl) : 1.0;
-    const double us2 = adjugate_product(th_s, f.phi_s1, f.q0, f.ex, 0) / mult;
-    const double ur2 = adjugate_product(th_r, f.phi_r1, f.q0, f.ex, 0) / mult;
-    double usr;
-    if (P.srow <= P.rrow)
-        usr = adjugate_product(th_s, f.phi_r1, f.q0, f.ex, P.rrow - P.srow) / mult;
-    else
-        usr = adjugate_product(th_r, f.phi_s1, f.q0, f.ex, P.srow - P.rrow) / mult;
-    const int o = P.out_off + (j - P.n_lo);
-    out.k2[o] = sig * P.inv_h2;   // lambda = lambda' / dz^2
-    out.absn[o] = (f.q0 != 0.0) ? -f.e0 / f.q0 : 0.0;
-    const double ps = sqrt(fmax(us2, 0.0) * P.inv_dz);
-    const double pr = sqrt(fmax(ur2, 0.0) * P.inv_dz);
-    out.psis[o] = ps;
-    out.psir[o] = (usr < 0.0) ? -pr : pr;
-    out.psisr[o] = usr * P.inv_dz;
-    out.iters[o] = my_iters;
-}
+    __device__ __forceinline__ void finish(const ModeProb &P, double *s_ring, double sig, bool collapsed, int nclu,
+                                           int j, bool active, int my_iters, const ModesOut &out) const {
+        const FinalOut f = sturm_final(P.a, P.w, s_ring, P.n, sig, P.srow, P.rrow);
+        SVal th_s, th_r;
+        theta_forward(P.a, s_ring, sig, P.srow, P.rrow, th_s, th_r);
+        if (!active) return;   // (whole-wave work above is done; nothing of this lane is left)
+        const double mult = collapsed ? (double)nclu : 1.0;
+        const double us2 = adjugate_product(th_s, f.phi_s1, f.q0, f.ex, 0) / mult;
+        const double ur2 = adjugate_product(th_r, f.phi_r1, f.q0, f.ex, 0) / mult;
+        double usr;
+        if (P.srow <= P.rrow)
+            usr = adjugate_product(th_s, f.phi_r1, f.q0, f.ex, P.rrow - P.srow) / mult;
+        else
+            usr = adjugate_product(th_r, f.phi_s1, f.q0, f.ex, P.srow - P.rrow) / mult;
+        const int o = P.out_off + (j - P.n_lo);
+        out.k2[o] = sig * P.inv_h2;   // lambda = lambda' / dz^2
+        out.absn[o] = (f.q0 != 0.0) ? -f.e0 / f.q0 : 0.0;
+        const double ps = sqrt(fmax(us2, 0.0) * P.inv_dz);
+        const double pr = sqrt(fmax(ur2, 0.0) * P.inv_dz);
+        out.psis[o] = ps;
+        out.psir[o] = (usr < 0.0) ? -pr : pr;
+        out.psisr[o] = usr * P.inv_dz;
+        out.iters[o] = my_iters;
+    }
+};
 
 // Two launch forms of the eigenvalue kernel, both over the XCD-interleaved work list
 // (item 8t + x belongs to XCD x; workgroups go to the XCDs round-robin, block b -> b % 8):
@@ -501,7 +311,10 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     const int nw = n_work_dev ? *n_work_dev : n_work;
     if (blockIdx.x >= (unsigned)nw) return;
     const ModeWork wk = work[blockIdx.x];
-    if (wk.prob >= 0) modes_item<MS>(mp[wk.prob], wk, sh, out);
+    if (wk.prob >= 0) {
+        const ModeProb &P = mp[wk.prob];
+        modes_item<MS>(P, EssPencil{P}, wk, sh, out);
+    }
 }
 
 template <int MS>
@@ -519,7 +332,10 @@ __global__ void __launch_bounds__(64) k_modes_persistent(const ModeProb *__restr
         const int w = NCPA_XCDS * t + x;
         if (w >= nw) break;
         const ModeWork wk = work[w];
-        if (wk.prob >= 0) modes_item<MS>(mp[wk.prob], wk, sh, out);
+        if (wk.prob >= 0) {
+            const ModeProb &P = mp[wk.prob];
+            modes_item<MS>(P, EssPencil{P}, wk, sh, out);
+        }
         __syncthreads();
     }
 }
@@ -642,23 +458,7 @@ __global__ void __launch_bounds__(1024) k_plan_build(const ProbDev *__restrict__
 // ------------------------------------------------------------------------------------
 // eigenvectors at a strided set of rows (plus the source row)
 // ------------------------------------------------------------------------------------
-// Slots 0..nrec-1 are rows row0 + k*stride, slot nrec the source row.  For each mode (lane)
-// k_rows records theta_{i-1} (forward continuant from the ground) and phi_{i+1} (backward
-// continuant from the top) at every recorded row i, with their power-of-two exponents, and
-// q_0 = d det/ds at the eigenvalue.  k_rows_combine then forms
-//     u_i^2 = -theta_{i-1} phi_{i+1} / q_0                 (adjugate diagonal)
-// picks the row of largest |u| as reference r and writes
-//     u_i = [-(-b)^d theta_{lo-1} phi_{hi+1} / q_0] / u_r,  (lo, hi) = sorted(i, r)
-// so every factor comes from the direction in which its recurrence is stable.  Signs are
-// fixed so that u(source row) >= 0, the convention psi(zs) >= 0 of the mode products.
-struct RowsArgs {
-    int row0, stride, nrec;
-    size_t ld;      // scratch / output leading dimension (total modes)
-};
-
-// The sweeps keep their row loops free of branches: every row's continuant goes to a
-// 16-row LDS buffer (one column per lane) and the recorded rows of a chunk are copied out
-// once the chunk is done, before its rescale.
+// Eigenvector rows of the effective-sound-speed modes (rows_item in modes.h).
 __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
                                              const double *__restrict__ k2, RowsArgs ra,
                                              double *__restrict__ TH, int *__restrict__ THE,
@@ -669,99 +469,8 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
-    const int lane = (int)threadIdx.x;
-    const int j = wk.j0 + lane;
-    const bool active = j < P.n_hi;
-    const size_t o = (size_t)P.out_off + (size_t)(active ? (j - P.n_lo) : 0);
-    const double s = active ? k2[o] * P.h2 : 0.5 * (P.lo + P.hi);   // shift of dz^2 T
-    const int n = P.n;
-    const double *a = P.a;
-    const size_t src = (size_t)ra.nrec * ra.ld + o;   // slot of the source row
-    const int last_rec = ra.row0 + (ra.nrec - 1) * ra.stride;
-    // forward: theta_k for k = 0 .. n-2, recorded at row k+1
-    {
-        double t1 = 1.0, t2 = 0.0;
-        int ex = 0;
-        int slot = 0, next = ra.row0;   // next recorded row and its slot
-        if (next == 0) {
-            if (active) { TH[o] = 1.0; THE[o] = 0; }
-            slot = 1;
-            next = ra.nrec > 1 ? ra.row0 + ra.stride : INT_MAX;
-        }
-        if (P.srow == 0 && active) { TH[src] = 1.0; THE[src] = 0; }
-        sweep_up(
-            a, s_ring, n - 1,
-            [&](double ak, int k) {
-                const double t0 = fma(ak - s, t1, -t2);
-                t2 = t1;
-                t1 = t0;
-                rec[k & (NCPA_CHUNK - 1)][lane] = t1;
-            },
-            [&](int lo, int m) {
-                // rows lo+1 .. lo+m now hold theta_{row-1}
-                for (; next <= lo + m; ++slot, next = slot < ra.nrec ? next + ra.stride : INT_MAX)
-                    if (active) {
-                        TH[(size_t)slot * ra.ld + o] = rec[(next - 1) & (NCPA_CHUNK - 1)][lane];
-                        THE[(size_t)slot * ra.ld + o] = ex;
-                    }
-                if (P.srow >= lo + 1 && P.srow <= lo + m && active) {
-                    TH[src] = rec[(P.srow - 1) & (NCPA_CHUNK - 1)][lane];
-                    THE[src] = ex;
-                }
-                const int kk = pow2_exponent(t1);
-                t1 = ldexp(t1, -kk);
-                t2 = ldexp(t2, -kk);
-                ex += kk;
-            });
-    }
-    // backward: phi_i for i = n-1 .. 0, recorded at row i-1; q for the normalisation
-    {
-        double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
-        int ex = 0;
-        int slot = ra.nrec - 1, next = last_rec;
-        if (next == n - 1) {
-            if (active) { PH[(size_t)slot * ra.ld + o] = 1.0; PHE[(size_t)slot * ra.ld + o] = 0; }
-            --slot;
-            next = slot >= 0 ? next - ra.stride : INT_MIN;
-        }
-        if (P.srow == n - 1 && active) { PH[src] = 1.0; PHE[src] = 0; }
-        sweep_down(
-            a, s_ring, n,
-            [&](double ai, int i) {
-                const double am = ai - s;
-                const double p0 = fma(am, p1, -p2);
-                const double q0 = fma(am, q1, -(q2 + p1));
-                p2 = p1;
-                p1 = p0;
-                q2 = q1;
-                q1 = q0;
-                rec[i & (NCPA_CHUNK - 1)][lane] = p1;
-            },
-            [&](int lo, int m) {
-                // rows lo-1 .. lo+m-2 now hold phi_{row+1}
-                for (; next >= lo - 1; --slot, next = slot >= 0 ? next - ra.stride : INT_MIN)
-                    if (active) {
-                        PH[(size_t)slot * ra.ld + o] = rec[(next + 1) & (NCPA_CHUNK - 1)][lane];
-                        PHE[(size_t)slot * ra.ld + o] = ex;
-                    }
-                if (P.srow >= lo - 1 && P.srow <= lo + m - 2 && active) {
-                    PH[src] = rec[(P.srow + 1) & (NCPA_CHUNK - 1)][lane];
-                    PHE[src] = ex;
-                }
-                const int k = pow2_exponent(p1);
-                p1 = ldexp(p1, -k);
-                p2 = ldexp(p2, -k);
-                q1 = ldexp(q1, -k);
-                q2 = ldexp(q2, -k);
-                ex += k;
-            });
-        if (active) {
-            Q0[o] = q1;
-            QEX[o] = ex;
-        }
-    }
+    rows_item(P, EssPencil{P}, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec);
 }
-
 __device__ __forceinline__ double log2abs_s(double v, int e) { return log2(fabs(v)) + (double)e; }
 
 // log2 |t p| + te + pe for factors of moderate size (each rescaled every 16 rows)
@@ -1041,12 +750,16 @@ void launch_plan_build(hipStream_t st, const ProbDev *probs, const IntervalOut *
                        n_sets, n_freq, mp, offsets, work, mbeg, mcnt, meta);
 }
 
+void launch_node_tables(hipStream_t st, const ProbDev *probs, int P, int ldn, double *d_nodes) {
+    dim3 ga((ldn + 255) / 256, P);
+    hipLaunchKernelGGL(k_alpha_nodes, ga, dim3(256), 0, st, probs, ldn, d_nodes);
+    hipLaunchKernelGGL(k_ceff_nodes, ga, dim3(256), 0, st, probs, ldn, d_nodes + (size_t)P * ldn);
+}
+
 void launch_setup(hipStream_t st, const ProbDev *probs, int P, int max_nodes, const ncpa_grid_t &grid,
                   double *d_nodes, double *A, double *W, double *CE) {
-    dim3 ga((max_nodes + 255) / 256, P);
     double *ceff_nodes = d_nodes + (size_t)P * max_nodes;
-    hipLaunchKernelGGL(k_alpha_nodes, ga, dim3(256), 0, st, probs, max_nodes, d_nodes);
-    hipLaunchKernelGGL(k_ceff_nodes, ga, dim3(256), 0, st, probs, max_nodes, ceff_nodes);
+    launch_node_tables(st, probs, P, max_nodes, d_nodes);
     dim3 gs((grid.Nz + 255) / 256, P);
     hipLaunchKernelGGL(k_setup_grid, gs, dim3(256), 0, st, probs, grid.Nz, grid.z_min, grid.dz, d_nodes, ceff_nodes,
                        max_nodes, A, W, CE);
@@ -1060,26 +773,7 @@ void launch_solve_front(hipStream_t st, const ProbDev *probs, int P, int max_nod
 }  // namespace ncpa
 using namespace ncpa;
 
-struct ncpa_modes {
-    ncpa_ctx *ctx = nullptr;
-    int P = 0;
-    int total = 0;
-    ncpa_grid_t grid{};
-    std::vector<int> counts, offsets;
-    std::vector<IntervalOut> iv;
-    std::vector<ModeProb> mp;                  // host copy of the per-problem records
-    int *d_offsets = nullptr;
-    double *d_k2 = nullptr, *d_abs = nullptr, *d_psis = nullptr, *d_psir = nullptr, *d_psisr = nullptr;
-    int *d_iters = nullptr;
-    double *d_A = nullptr, *d_W = nullptr, *d_CE = nullptr;   // per-problem grid samples
-    double *d_psis_rows = nullptr;             // psi(zs) from the last eigenvector-rows pass
-    ModeProb *d_mp = nullptr;
-    ModeWork *d_work = nullptr;
-    int *d_mode_prob = nullptr;
-    size_t n_work = 0;
-};
-
-namespace {
+namespace ncpa {
 
 int check_grid(ncpa_ctx *ctx, const ncpa_grid_t *g) {
     if (!g || g->Nz < 10 || !(g->dz > 0.0))
@@ -1091,7 +785,7 @@ void release_modes(ncpa_modes *m) {
     if (!m || !m->ctx) return;
     DevicePool &pl = m->ctx->pool;
     void *bufs[] = {m->d_offsets, m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters,
-                    m->d_A, m->d_W, m->d_CE, m->d_mp, m->d_work, m->d_mode_prob, m->d_psis_rows};
+                    m->d_A, m->d_W, m->d_CE, m->d_mp, m->d_work, m->d_mode_prob, m->d_psis_rows, m->d_BC};
     for (void *b : bufs) pl.put(b);
 }
 
@@ -1120,11 +814,19 @@ int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, con
         q.w = m->d_W + size_t(p) * Nz;
         q.h2 = h2;
         q.inv_h2 = b;
-        q.lo = v.lo * h2;   // the eigenvalue kernels work on dz^2 T
-        q.hi = v.hi * h2;
-        // ||dz^2 T|| <= 4 + max diag: the absolute floor of FP64 eigenvalues of dz^2 T
-        const double normT = 4.0 + std::max(std::fabs(q.hi), std::fabs(q.lo));
-        q.tol = 8.0 * DBL_EPSILON * normT;
+        if (m->wide) {
+            // shifts are wavenumbers k; |d mu/dk| of dz^2 T(k) is about 2 dz^2 k, so the FP64
+            // floor of a crossing, 8 eps ||dz^2 T|| / |d mu/dk| with ||dz^2 T|| <= 6, is below
+            q.lo = v.lo;
+            q.hi = v.hi;
+            q.tol = 48.0 * DBL_EPSILON / (h2 * std::max(v.lo, 0.25 * v.hi));
+        } else {
+            q.lo = v.lo * h2;   // the eigenvalue kernels work on dz^2 T
+            q.hi = v.hi * h2;
+            // ||dz^2 T|| <= 4 + max diag: the absolute floor of FP64 eigenvalues of dz^2 T
+            const double normT = 4.0 + std::max(std::fabs(q.hi), std::fabs(q.lo));
+            q.tol = 8.0 * DBL_EPSILON * normT;
+        }
         q.inv_dz = 1.0 / dz;
         q.n = Nz;
         q.srow = srow[p];
@@ -1179,8 +881,12 @@ int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, con
         NCPA_HIP(ctx, hipMemcpyAsync(m->d_mode_prob, mode_prob.data(), sizeof(int) * total, hipMemcpyHostToDevice, ctx->stream));
         TimedLaunch tl(ctx, "modes");
         const ModesOut mo{m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters};
-        launch_k_modes(ctx->ms_passes, dim3((unsigned)work.size()), ctx->stream, m->d_mp, m->d_work,
-                       (int)work.size(), nullptr, nullptr, mo);
+        if (m->wide)
+            launch_k_wmodes(ctx->ms_passes, dim3((unsigned)work.size()), ctx->stream, m->d_mp, m->d_work,
+                            m->d_BC, (size_t)P * Nz, mo);
+        else
+            launch_k_modes(ctx->ms_passes, dim3((unsigned)work.size()), ctx->stream, m->d_mp, m->d_work,
+                           (int)work.size(), nullptr, nullptr, mo);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(ctx, e, "k_modes launch");
@@ -1214,8 +920,12 @@ int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, do
     }
     {
         TimedLaunch tl(ctx, "rows");
-        hipLaunchKernelGGL(k_rows, dim3((unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
-                           m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
+        if (m->wide)
+            launch_k_wrows(dim3((unsigned)m->n_work), ctx->stream, m->d_mp, m->d_work, m->d_BC,
+                           (size_t)m->P * m->grid.Nz, m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
+        else
+            hipLaunchKernelGGL(k_rows, dim3((unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
+                               m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
         const unsigned mb = (unsigned)((ld + 63) / 64);
         hipLaunchKernelGGL(k_rows_ref, dim3(mb), dim3(64 * NCPA_REF_WAVES), 0, ctx->stream, TH, THE, PH, PHE, Q0, QEX,
                            m->d_mode_prob, m->d_mp, ra, ld, RS, RL, FL, m->d_psis_rows);
@@ -1229,7 +939,7 @@ int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, do
     return NCPA_OK;
 }
 
-}  // namespace
+}  // namespace ncpa
 
 namespace ncpa {
 // Device problem records of a batch (the per-problem part of ESSModeSolver::setParams and

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "ncpamodes.h"
 
 struct ncpa_ctx;
@@ -70,6 +72,8 @@ struct PlanMeta {
     int queue[NCPA_XCDS];   // per-XCD work counters of the persistent launch (reset here)
 };
 
+struct RowsArgs;
+
 // ess.hip.  d_nodes: 2 P max_nodes doubles of scratch (attenuation and c_eff at the nodes)
 void launch_setup(hipStream_t st, const ProbDev *probs, int P, int max_nodes, const ncpa_grid_t &grid,
                   double *d_nodes, double *A, double *W, double *CE);
@@ -84,6 +88,16 @@ void launch_k_modes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const
                     const int *n_work_dev, int *queue, const ModesOut &mo);
 int build_problems(ncpa_ctx *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems, int n_problems,
                    ProbDev *hp, int *max_nodes);
+// attenuation and c_eff at the profile nodes: d_nodes[0 .. P ldn) and [P ldn .. 2 P ldn)
+void launch_node_tables(hipStream_t st, const ProbDev *probs, int P, int ldn, double *d_nodes);
+
+// wide.hip: the wide-angle eigenvalue and eigenvector-rows kernels; BC holds the B rows of
+// every problem (plane elements) followed by the C rows
+void launch_k_wmodes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work,
+                     const double *BC, size_t plane, const ModesOut &mo);
+void launch_k_wrows(dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, const double *BC,
+                    size_t plane, const double *k2, const RowsArgs &ra, double *TH, int *THE, double *PH,
+                    int *PHE, double *Q0, int *QEX);
 
 // broadband.hip
 void bb_launch_kpert(hipStream_t st, int n_cap, const int *n_dev, const double *k2, const double *absn,
@@ -97,4 +111,35 @@ int bb_prepare_fft(ncpa_ctx *ctx, int n, int batch, size_t *scratch_doubles);
 int bb_launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, int sign, double *scratch);
 void bb_launch_out(hipStream_t st, size_t n, const double *spec, double df, double *wave);
 
+}  // namespace ncpa
+
+// Modes of a solved batch (device-resident), shared by the translation units.
+struct ncpa_modes {
+    ncpa_ctx *ctx = nullptr;
+    int P = 0;
+    int total = 0;
+    bool wide = false;                         // wide-angle (WModeSolver) modes
+    ncpa_grid_t grid{};                        // dz: the matrix spacing
+    std::vector<int> counts, offsets;
+    std::vector<ncpa::IntervalOut> iv;
+    std::vector<ncpa::ModeProb> mp;            // host copy of the per-problem records
+    int *d_offsets = nullptr;
+    double *d_k2 = nullptr, *d_abs = nullptr, *d_psis = nullptr, *d_psir = nullptr, *d_psisr = nullptr;
+    int *d_iters = nullptr;
+    double *d_A = nullptr, *d_W = nullptr, *d_CE = nullptr;   // per-problem grid samples
+    double *d_BC = nullptr;                    // wide-angle B and C rows
+    double *d_psis_rows = nullptr;             // psi(zs) from the last eigenvector-rows pass
+    ncpa::ModeProb *d_mp = nullptr;
+    ncpa::ModeWork *d_work = nullptr;
+    int *d_mode_prob = nullptr;
+    size_t n_work = 0;
+};
+
+namespace ncpa {
+int check_grid(ncpa_ctx *ctx, const ncpa_grid_t *g);
+void release_modes(ncpa_modes *m);
+// eigenvalue kernel over the intervals in m->iv (matrices already in m's buffers)
+int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, const std::vector<int> &rrow);
+// eigenvectors of every mode at rows row0 + k*stride, k < nrec -> dev_psi[k * total + o]
+int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, double *dev_psi);
 }  // namespace ncpa

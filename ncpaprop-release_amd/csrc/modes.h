@@ -1,0 +1,381 @@
+// Shared device code of the eigenvalue kernels: the block reductions, the modal interval,
+// the search of one 64-lane block over 64 consecutive eigenvalue indices and the
+// eigenvector-rows pass, written for any pencil (ess.hip: dz^2 T - s; wide.hip: dz^2 T(k)).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "internal.h"
+#include "sturm.h"
+
+namespace ncpa {
+
+// ------------------------------------------------------------------------------------
+// block reductions (256-thread blocks)
+// ------------------------------------------------------------------------------------
+__device__ inline double block_min(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = sh[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = fmin(r, sh[k]);
+    __syncthreads();
+    return r;
+}
+__device__ inline double block_sum(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = sh[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r += sh[k];
+    __syncthreads();
+    return r;
+}
+__device__ inline int block_min_int(int v, int *sh) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    int r = sh[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = min(r, sh[k]);
+    __syncthreads();
+    return r;
+}
+
+// k_min / k_max of one problem from c_eff on the grid rows (whole block): the WKB cut-off
+// for ground-to-ground propagation (ESSModeSolver.cpp:561-607, WModeSolver.cpp:574-609),
+// k_min = omega/c_eff(Hgt[top+1]), top = Nz - Nz/10 (ESSModeSolver.cpp:609-614,
+// WModeSolver.cpp:611-620), and the wavenumber filter (ESSModeSolver.cpp:366-369,
+// WModeSolver.cpp:356-360).  wkb_fallback: drop the cut-off when it falls below k_min, as
+// ESSModeSolver.cpp:616-621 does (WModeSolver has no such rule).  dz: the spacing of the
+// WKB integral.
+__device__ inline void interval_bounds(const ProbDev &pr, int Nz, double dz, const double *ce, double *shd,
+                                       int *shi, bool wkb_fallback, double &k_min_o, double &k_max_o,
+                                       int &used_wkb_o) {
+    const double om = pr.omega;
+    double m = DBL_MAX;
+    for (int i = threadIdx.x; i < Nz; i += blockDim.x) m = fmin(m, ce[i]);
+    const double ceffmin = fmin(block_min(m, shd), ce[0]);
+    double k_max;
+    int used_wkb = 0;
+    if ((pr.flags & NCPA_F_WKB) && fabs(pr.zs) < 1.0e-3 && fabs(pr.zr) < 1.0e-3) {
+        used_wkb = 1;
+        const double kmf = om / ceffmin;
+        const double kg = om / ce[0];
+        const double kmf2 = kmf * kmf, kg2 = kg * kg;
+        const double dkk = (kmf2 - kg2) / 100.0;
+        double kk = kg2;
+        if (dkk > 1.0e-10) {
+            // the reference steps ~100 times; the cap only bounds a degenerate dkk
+            int steps = 0;
+            for (kk = kg2; kk < kmf2 && steps < 1000; kk = kk + dkk, ++steps) {
+                int first = Nz - 1;
+                for (int i = threadIdx.x; i < Nz; i += blockDim.x) {
+                    const double ke = om / ce[i];
+                    if (fabs(kk - ke * ke) <= dkk) { first = min(first, i); break; }
+                }
+                first = block_min_int(first, shi);
+                double s = 0.0;
+                for (int i = threadIdx.x; i <= first; i += blockDim.x) {
+                    const double ke = om / ce[i];
+                    s += dz * sqrt(fabs(kk - ke * ke));
+                }
+                const double integral = block_sum(s, shd);
+                if (integral >= 10.0) break;
+            }
+        }
+        k_max = sqrt(kk);
+    } else {
+        k_max = om / ceffmin;
+    }
+    const int top = Nz - Nz / 10;
+    double k_min = om / ce[min(top + 1, Nz - 1)];   // Nz < 20 would read past the grid
+    if (wkb_fallback && used_wkb && k_max < k_min) k_max = om / ceffmin;
+    if (pr.flags & NCPA_F_WVNUM_FILTER) {
+        k_min = om / pr.c_max;   // 2*PI*freq/c_max
+        k_max = om / pr.c_min;
+    }
+    k_min_o = k_min;
+    k_max_o = k_max;
+    used_wkb_o = used_wkb;
+}
+
+// ------------------------------------------------------------------------------------
+// eigenvalues + sensitivities: one 64-lane block per (problem, 64 consecutive indices)
+// ------------------------------------------------------------------------------------
+#define NCPA_MAXIT 200
+
+// One lane per eigenvalue index j (ascending), 64 consecutive indices of one problem per
+// block.  Phase 0 narrows every bracket by multisection (see below); phase 1 bisects with count-only sweeps until [lo, hi) holds only eigenvalue j;
+// phase 2 runs Newton on det(T - s) (count kept as a safeguard) until the step is below
+// the working-precision tolerance; phase 3 is one sweep at the converged eigenvalue that
+// yields the absorption and the adjugate eigenvector products.  iters_out: sweeps used,
+// negative when the bracket collapsed before Newton converged (cluster at working precision).
+//
+// The search is written once for any pencil whose count of eigenvalues below the shift
+// is non-decreasing in the shift: `pen` supplies count(ring, s), newton(ring, s) (count,
+// det and d det/ds) and finish(...) (phase 3 and the outputs).  EssPencil is dz^2 T - s;
+// the wide-angle pencil (wide.h) is dz^2 T(k) with the shift k inside the diagonal.
+template <int NR>
+struct ModesSharedN {
+    double ms_s[64];
+    int ms_c[64];
+    double hb_lo[64], hb_hi[64];   // brackets of the unconverged eigenvalues
+    int h_list[64];                // their lanes, in lane order
+    double ring[NR * NCPA_RING];   // NR staged arrays
+};
+using ModesShared = ModesSharedN<2>;
+
+template <int MS, class Pen, class Sh>
+__device__ __forceinline__ void modes_item(const ModeProb &P, const Pen &pen, const ModeWork wk, Sh &sh_,
+                                           const ModesOut &out) {
+    double *ms_s = sh_.ms_s, *hb_lo = sh_.hb_lo, *hb_hi = sh_.hb_hi, *s_ring = sh_.ring;
+    int *ms_c = sh_.ms_c, *h_list = sh_.h_list;
+    const int lane = (int)threadIdx.x;
+    const int j = wk.j0 + lane;
+    const bool active = j < P.n_hi;
+    double lo = P.lo, hi = P.hi;
+    int cl = P.n_lo, ch = P.n_hi;
+    // phase 0: multisection.  The 64 lanes evaluate the count at 64 distinct shifts spread
+    // over the bracket of the block's eigenvalues; every lane then takes the tightest
+    // bracket of its own eigenvalue from the 64 counts.  Two passes isolate most eigenvalues,
+    // work that bisection spends ~log2(n) sweeps on per lane.
+    const int last = min(P.n_hi - 1, wk.j0 + 63) - wk.j0;   // last active lane
+    double blo = lo, bhi = hi;
+    for (int pass = 0; pass < MS; ++pass) {
+        const double sh = blo + (double)(lane + 1) * ((bhi - blo) * (1.0 / 65.0));
+        const int c = pen.count(s_ring, sh);
+        ms_s[lane] = sh;
+        ms_c[lane] = c;
+        __syncthreads();
+        for (int l = 0; l < 64; ++l) {           // counts are non-decreasing in the shift
+            const int cc = ms_c[l];
+            const double ss = ms_s[l];
+            if (cc <= j) {
+                if (ss > lo) { lo = ss; cl = cc; }
+            } else if (ss < hi) {
+                hi = ss;
+                ch = cc;
+            }
+        }
+        __syncthreads();
+        blo = __shfl(lo, 0);
+        bhi = __shfl(hi, last);
+    }
+    double sig = 0.5 * (lo + hi);
+    bool done = !active;
+    int my_iters = MS;
+    bool collapsed = false;
+    // phases 1-2.  Lanes whose eigenvalue has converged (and lanes past the problem's last
+    // index) do not idle: each iteration they are dealt round-robin to the unconverged
+    // eigenvalues of the block and evaluate extra shifts spread over those brackets; every
+    // unconverged lane then tightens its bracket from all 64 (shift, count) pairs, so the
+    // stragglers that set the block's duration converge several bits per sweep.
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    double prev_step = INFINITY;   // the last Newton step taken (INFINITY after a bisection)
+    for (int it = 0; it < NCPA_MAXIT; ++it) {
+        const unsigned long long um = __ballot(!done);
+        if (um == 0ull) break;
+        const int nu = __popcll(um);
+        if (!done) {
+            hb_lo[lane] = lo;
+            hb_hi[lane] = hi;
+            h_list[__popcll(um & below)] = lane;
+        }
+        __syncthreads();
+        double sh = sig;
+        if (done) {
+            const int h = __popcll(~um & below);        // rank among the helpers
+            const int idx = h % nu, kk = h / nu;
+            const int nh = (64 - nu - idx + nu - 1) / nu;  // helpers dealt to eigenvalue idx
+            const int u = h_list[idx];
+            const double a = hb_lo[u], b = hb_hi[u];
+            sh = a + (double)(kk + 1) * ((b - a) / (double)(nh + 1));
+        }
+        const bool newton = !done && (ch - cl == 1);
+        int cnt;
+        double p0 = 0.0, q0 = 0.0;
+        if (__any(newton)) {
+            const NewtonOut o = pen.newton(s_ring, sh);
+            cnt = o.cnt;
+            p0 = o.p0;
+            q0 = o.q0;
+        } else {
+            cnt = pen.count(s_ring, sh);
+        }
+        ms_s[lane] = sh;
+        ms_c[lane] = cnt;
+        __syncthreads();
+        if (!done) {
+            for (int l = 0; l < 64; ++l) {           // counts are non-decreasing in the shift
+                const int cc = ms_c[l];
+                const double ss = ms_s[l];
+                if (cc <= j) {
+                    if (ss > lo) { lo = ss; cl = cc; }
+                } else if (ss < hi) {
+                    hi = ss;
+                    ch = cc;
+                }
+            }
+            sig = 0.5 * (lo + hi);
+            double step = INFINITY;
+            if (newton && q0 != 0.0 && isfinite(p0) && isfinite(q0)) {
+                const double dlt = -p0 / q0;
+                const double cand = sh + dlt;
+                const double adl = fabs(dlt);
+                // Converged when the step is at the tolerance, or (pencils with
+                // Pen::early_accept) when two successive Newton steps show the quadratic
+                // regime (ratio below 1e-3) and predict the next step, |dlt|^3 / |dlt_prev|^2,
+                // at the tolerance: the sweep that would only confirm it is skipped.  The
+                // prediction takes the previous step as asymptotic; det T(k) of the wide-angle
+                // pencil has other zero branches close enough to break that, so it waits for
+                // the step itself.
+                const double rat = adl / prev_step;   // 0 after a bisection: no estimate yet
+                if (adl <= P.tol ||
+                    (Pen::early_accept && prev_step < INFINITY && rat <= 1e-3 && adl * rat * rat <= P.tol)) {
+                    sig = cand;
+                    done = true;
+                    my_iters = MS + it + 1;
+                } else if (cand > lo && cand < hi) {
+                    sig = cand;
+                    step = adl;
+                }
+            }
+            prev_step = step;
+            if (!done && (hi - lo) <= P.tol) {
+                sig = 0.5 * (lo + hi);
+                done = true;
+                collapsed = (ch - cl) != 1;
+                my_iters = -(MS + it + 1);
+            }
+        }
+        __syncthreads();
+    }
+    if (!done) my_iters = -NCPA_MAXIT - 1;
+    pen.finish(P, s_ring, sig, collapsed, ch - cl, j, active, my_iters, out);
+}
+
+// ------------------------------------------------------------------------------------
+// eigenvectors at a strided set of rows (plus the source row)
+// ------------------------------------------------------------------------------------
+// Slots 0..nrec-1 are rows row0 + k*stride, slot nrec the source row.  For each mode (lane)
+// k_rows records theta_{i-1} (forward continuant from the ground) and phi_{i+1} (backward
+// continuant from the top) at every recorded row i, with their power-of-two exponents, and
+// q_0 = d det/ds at the eigenvalue.  k_rows_combine then forms
+//     u_i^2 = -theta_{i-1} phi_{i+1} / q_0                 (adjugate diagonal)
+// picks the row of largest |u| as reference r and writes
+//     u_i = [-(-b)^d theta_{lo-1} phi_{hi+1} / q_0] / u_r,  (lo, hi) = sorted(i, r)
+// so every factor comes from the direction in which its recurrence is stable.  Signs are
+// fixed so that u(source row) >= 0, the convention psi(zs) >= 0 of the mode products.
+struct RowsArgs {
+    int row0, stride, nrec;
+    size_t ld;      // scratch / output leading dimension (total modes)
+};
+
+// (the pencil supplies the shift of a mode, the row values at that shift and the factor
+// by which q_0 is scaled: the wide-angle vectors are normalised with (1 + k^2) q_0.)
+// The sweeps keep their row loops free of branches: every row's continuant goes to a
+// 16-row LDS buffer (one column per lane) and the recorded rows of a chunk are copied out
+// once the chunk is done, before its rescale.
+template <class Pen>
+__device__ __forceinline__ void rows_item(const ModeProb &P, const Pen &pen, const ModeWork wk,
+                                          const double *__restrict__ k2, RowsArgs ra, double *__restrict__ TH,
+                                          int *__restrict__ THE, double *__restrict__ PH, int *__restrict__ PHE,
+                                          double *__restrict__ Q0, int *__restrict__ QEX, double *s_ring,
+                                          double (*rec)[64]) {
+    const int lane = (int)threadIdx.x;
+    const int j = wk.j0 + lane;
+    const bool active = j < P.n_hi;
+    const size_t o = (size_t)P.out_off + (size_t)(active ? (j - P.n_lo) : 0);
+    const double s = active ? pen.shift(k2[o]) : 0.5 * (P.lo + P.hi);   // the pencil's shift
+    const int n = P.n;
+    const size_t src = (size_t)ra.nrec * ra.ld + o;   // slot of the source row
+    const int last_rec = ra.row0 + (ra.nrec - 1) * ra.stride;
+    // forward: theta_k for k = 0 .. n-2, recorded at row k+1
+    {
+        double t1 = 1.0, t2 = 0.0;
+        int ex = 0;
+        int slot = 0, next = ra.row0;   // next recorded row and its slot
+        if (next == 0) {
+            if (active) { TH[o] = 1.0; THE[o] = 0; }
+            slot = 1;
+            next = ra.nrec > 1 ? ra.row0 + ra.stride : INT_MAX;
+        }
+        if (P.srow == 0 && active) { TH[src] = 1.0; THE[src] = 0; }
+        pen.up(
+            s_ring, n - 1, s,
+            [&](double am, int k) {
+                const double t0 = fma(am, t1, -t2);
+                t2 = t1;
+                t1 = t0;
+                rec[k & (NCPA_CHUNK - 1)][lane] = t1;
+            },
+            [&](int lo, int m) {
+                // rows lo+1 .. lo+m now hold theta_{row-1}
+                for (; next <= lo + m; ++slot, next = slot < ra.nrec ? next + ra.stride : INT_MAX)
+                    if (active) {
+                        TH[(size_t)slot * ra.ld + o] = rec[(next - 1) & (NCPA_CHUNK - 1)][lane];
+                        THE[(size_t)slot * ra.ld + o] = ex;
+                    }
+                if (P.srow >= lo + 1 && P.srow <= lo + m && active) {
+                    TH[src] = rec[(P.srow - 1) & (NCPA_CHUNK - 1)][lane];
+                    THE[src] = ex;
+                }
+                const int kk = pow2_exponent(t1);
+                t1 = ldexp(t1, -kk);
+                t2 = ldexp(t2, -kk);
+                ex += kk;
+            });
+    }
+    // backward: phi_i for i = n-1 .. 0, recorded at row i-1; q for the normalisation
+    {
+        double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
+        int ex = 0;
+        int slot = ra.nrec - 1, next = last_rec;
+        if (next == n - 1) {
+            if (active) { PH[(size_t)slot * ra.ld + o] = 1.0; PHE[(size_t)slot * ra.ld + o] = 0; }
+            --slot;
+            next = slot >= 0 ? next - ra.stride : INT_MIN;
+        }
+        if (P.srow == n - 1 && active) { PH[src] = 1.0; PHE[src] = 0; }
+        pen.down(
+            s_ring, n, s,
+            [&](double am, int i) {
+                const double p0 = fma(am, p1, -p2);
+                const double q0 = fma(am, q1, -(q2 + p1));
+                p2 = p1;
+                p1 = p0;
+                q2 = q1;
+                q1 = q0;
+                rec[i & (NCPA_CHUNK - 1)][lane] = p1;
+            },
+            [&](int lo, int m) {
+                // rows lo-1 .. lo+m-2 now hold phi_{row+1}
+                for (; next >= lo - 1; --slot, next = slot >= 0 ? next - ra.stride : INT_MIN)
+                    if (active) {
+                        PH[(size_t)slot * ra.ld + o] = rec[(next + 1) & (NCPA_CHUNK - 1)][lane];
+                        PHE[(size_t)slot * ra.ld + o] = ex;
+                    }
+                if (P.srow >= lo - 1 && P.srow <= lo + m - 2 && active) {
+                    PH[src] = rec[(P.srow + 1) & (NCPA_CHUNK - 1)][lane];
+                    PHE[src] = ex;
+                }
+                const int k = pow2_exponent(p1);
+                p1 = ldexp(p1, -k);
+                p2 = ldexp(p2, -k);
+                q1 = ldexp(q1, -k);
+                q2 = ldexp(q2, -k);
+                ex += k;
+            });
+        if (active) {
+            Q0[o] = q1 * pen.qscale(s);
+            QEX[o] = ex;
+        }
+    }
+}
+
+}  // namespace ncpa

@@ -1,0 +1,282 @@
+// FP64 Sturm sweeps over the wide-angle (high-Mach) modal pencil.
+//
+// WModeSolver solves the quadratic eigenvalue problem (k^2 M + k C + D) v = 0 with
+// D = tridiag(1/h^2, -2/h^2 + kd_i, 1/h^2), M = diag(md_i), C = diag(cd_i)
+// (src/libmodes/EigenEngine.cpp:220-250, WModeSolver.cpp:549-557).  For real k the matrix
+//     T(k) = D + k C + k^2 M
+// is symmetric tridiagonal, and the sweeps work on dz^2 T(k) = tridiag(1, a_i(k), 1) with
+//     a_i(k) = A_i + k (B_i + k C_i),  A = -2 + dz^2 kd,  B = dz^2 cd,  C = dz^2 md.
+// With md = (wc/c0)^2 - 1 < 0 and cd = -2 omega wc/c0^2, dT/dk = C + 2kM is negative definite
+// for k > omega |wc|/(c0^2 - wc^2) on every row, which holds on the trapped interval of any
+// subsonic atmosphere: every eigenvalue branch mu_i(k) of T(k) then decreases strictly, so
+//     n(k) = number of negative eigenvalues of T(k)
+// is non-decreasing in k and the QEP eigenvalues in (k_lo, k_hi] are exactly the
+// n(k_hi) - n(k_lo) zero crossings, branch i crossing at the i-th one.  The search of the
+// effective-sound-speed kernel therefore runs unchanged with k as the shift: count(k) is
+// the sign-change count of the continuants of dz^2 T(k) at shift 0, and Newton works on
+// det dz^2 T(k) with the k-derivative continuant
+//     q_i = a_i'(k) phi_{i+1} + a_i(k) q_{i+1} - q_{i+2},   a_i' = B_i + 2 k C_i.
+// At a converged k the unit null vector u of dz^2 T(k) has the adjugate products of the
+// effective-sound-speed case (sturm.h) with the identity-shift derivative continuant
+// q^mu; the reference's vector is the first half of a unit 2N vector (v, k v)
+// (EigenEngine.cpp:376-381), so psi = u / sqrt(dz (1 + k^2)) and the normalising
+// derivative is q^mu (1 + k^2).
+//
+// Three arrays (four with the absorption weights) are staged per sweep through LDS rings
+// of two 256-row tiles each, as in sturm.h; the rows are read back 16 at a time.
+#pragma once
+#include "sturm.h"
+
+namespace ncpa {
+
+// Rows n-1 .. 0 of NA arrays staged together: step(v, i) with v[a] = arr[a][i], then
+// chunk_end(lo, m) after each chunk of rows [lo, lo + m) (first the n % 16 top rows, then
+// 16-row chunks downwards).  ring holds NA rings of NCPA_RING doubles.  Full-wave entry.
+template <int NA, class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_down_n(const double *const (&arr)[NA], double *ring, int n, Step step,
+                                             ChunkEnd chunk_end) {
+    if (n <= 0) return;
+    const int lane = (int)(threadIdx.x & 63);
+    TileStage st[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) st[a] = TileStage{arr[a], ring + a * NCPA_RING, n, lane};
+    const int ntile = (n + NCPA_TILE - 1) / NCPA_TILE;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) { st[a].load(ntile - 1); st[a].store(ntile - 1); st[a].load(ntile - 2); }
+    wave_lds_sync();
+    const int rem = n % NCPA_CHUNK;
+    for (int c = ntile - 1; c >= 0; --c) {
+        const int off = (c & 1) * NCPA_TILE;
+        const int base = c * NCPA_TILE;
+        int top = min(n, base + NCPA_TILE);
+        if (c == ntile - 1 && rem) {
+            for (int i = n - 1; i >= n - rem; --i) {
+                double v[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) v[a] = ring[a * NCPA_RING + off + i - base];
+                step(v, i);
+            }
+            chunk_end(n - rem, rem);
+            top = n - rem;
+        }
+        for (int cb = top - NCPA_CHUNK; cb >= base; cb -= NCPA_CHUNK) {
+            double va[NA][NCPA_CHUNK];
+#pragma unroll
+            for (int a = 0; a < NA; ++a) lds_chunk(va[a], ring + a * NCPA_RING + off + (cb - base));
+#pragma unroll
+            for (int u = NCPA_CHUNK - 1; u >= 0; --u) {
+                double v[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) v[a] = va[a][u];
+                step(v, cb + u);
+            }
+            chunk_end(cb, NCPA_CHUNK);
+        }
+        if (c > 0) {
+            wave_lds_sync();
+#pragma unroll
+            for (int a = 0; a < NA; ++a) { st[a].store(c - 1); st[a].load(c - 2); }
+            wave_lds_sync();
+        }
+    }
+}
+
+// Rows 0 .. m-1 (ascending) of NA arrays, chunk_end(lo, m') after every chunk of 16 rows and
+// after the last m % 16 rows.  Full-wave entry.
+template <int NA, class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_up_n(const double *const (&arr)[NA], double *ring, int m, Step step,
+                                           ChunkEnd chunk_end) {
+    if (m <= 0) return;
+    const int lane = (int)(threadIdx.x & 63);
+    TileStage st[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) st[a] = TileStage{arr[a], ring + a * NCPA_RING, m, lane};
+    const int ntile = (m + NCPA_TILE - 1) / NCPA_TILE;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) { st[a].load(0); st[a].store(0); st[a].load(1 < ntile ? 1 : -1); }
+    wave_lds_sync();
+    for (int c = 0; c < ntile; ++c) {
+        const int off = (c & 1) * NCPA_TILE;
+        const int base = c * NCPA_TILE;
+        const int endt = min(m, base + NCPA_TILE);
+        for (int cb = base; cb < endt; cb += NCPA_CHUNK) {
+            const int len = min(NCPA_CHUNK, endt - cb);
+            if (len == NCPA_CHUNK) {
+                double va[NA][NCPA_CHUNK];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) lds_chunk(va[a], ring + a * NCPA_RING + off + (cb - base));
+#pragma unroll
+                for (int u = 0; u < NCPA_CHUNK; ++u) {
+                    double v[NA];
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) v[a] = va[a][u];
+                    step(v, cb + u);
+                }
+            } else {
+                for (int i = cb; i < cb + len; ++i) {
+                    double v[NA];
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) v[a] = ring[a * NCPA_RING + off + i - base];
+                    step(v, i);
+                }
+            }
+            chunk_end(cb, len);
+        }
+        if (c + 1 < ntile) {
+            wave_lds_sync();
+#pragma unroll
+            for (int a = 0; a < NA; ++a) { st[a].store(c + 1); st[a].load(c + 2 < ntile ? c + 2 : -1); }
+            wave_lds_sync();
+        }
+    }
+}
+
+__device__ __forceinline__ double wide_row(const double (&v)[3], double k) { return fma(fma(v[2], k, v[1]), k, v[0]); }
+
+// The wide-angle pencil of one problem: rows A, B, C of dz^2 T(k) and absorption weights W.
+// ring: 4 staged arrays (4 NCPA_RING doubles).
+struct WideSweeps {
+    const double *A, *B, *C, *W;
+    int n;
+
+    // number of negative eigenvalues of dz^2 T(k)
+    __device__ __forceinline__ int count(double *ring, double k) const {
+        double p1 = 1.0, p2 = 0.0;
+        int cnt = 0;
+        unsigned bits = 0u;
+        const double *const arr[3] = {A, B, C};
+        sweep_down_n<3>(
+            arr, ring, n,
+            [&](const double (&v)[3], int) {
+                const double p0 = fma(wide_row(v, k), p1, -p2);
+                bits = push_sign(bits, p0);
+                p2 = p1;
+                p1 = p0;
+            },
+            [&](int, int m) {
+                cnt += sign_changes(bits, m);
+                bits &= 1u;
+                const int e = pow2_exponent(p1);
+                p1 = ldexp(p1, -e);
+                p2 = ldexp(p2, -e);
+            });
+        return cnt;
+    }
+
+    // count plus det dz^2 T(k) and its k-derivative (Newton step -p0/q0 in k)
+    __device__ __forceinline__ NewtonOut newton(double *ring, double k) const {
+        double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
+        int cnt = 0;
+        unsigned bits = 0u;
+        const double k2 = k + k;
+        const double *const arr[3] = {A, B, C};
+        sweep_down_n<3>(
+            arr, ring, n,
+            [&](const double (&v)[3], int) {
+                const double am = wide_row(v, k);
+                const double da = fma(k2, v[2], v[1]);
+                const double p0 = fma(am, p1, -p2);
+                const double q0 = fma(am, q1, fma(da, p1, -q2));
+                bits = push_sign(bits, p0);
+                p2 = p1;
+                p1 = p0;
+                q2 = q1;
+                q1 = q0;
+            },
+            [&](int, int m) {
+                cnt += sign_changes(bits, m);
+                bits &= 1u;
+                const int e = pow2_exponent(p1);
+                p1 = ldexp(p1, -e);
+                p2 = ldexp(p2, -e);
+                q1 = ldexp(q1, -e);
+                q2 = ldexp(q2, -e);
+            });
+        NewtonOut o;
+        o.cnt = cnt;
+        o.p0 = p1;
+        o.q0 = q1;
+        return o;
+    }
+
+    // backward sweep at a converged k: det, its identity-shift derivative q^mu (q0), the
+    // absorption derivative (e0) and the records phi_{s+1}, phi_{r+1}
+    __device__ __forceinline__ FinalOut final_sweep(double *ring, double k, int srow, int rrow) const {
+        double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0, e1 = 0.0, e2 = 0.0;
+        int ex = 0;
+        FinalOut o;
+        o.phi_s1 = SVal{1.0, 0};
+        o.phi_r1 = SVal{1.0, 0};
+        const double *const arr[4] = {A, B, C, W};
+        sweep_down_n<4>(
+            arr, ring, n,
+            [&](const double (&v)[4], int i) {
+                const double am = fma(fma(v[2], k, v[1]), k, v[0]);
+                const double p0 = fma(am, p1, -p2);
+                const double q0 = fma(am, q1, -(q2 + p1));
+                const double e0 = fma(am, e1, fma(v[3], p1, -e2));
+                p2 = p1;
+                p1 = p0;
+                q2 = q1;
+                q1 = q0;
+                e2 = e1;
+                e1 = e0;
+                if (i == srow + 1) o.phi_s1 = SVal{p1, ex};
+                if (i == rrow + 1) o.phi_r1 = SVal{p1, ex};
+            },
+            [&](int, int) {
+                const int e = pow2_exponent(p1);
+                p1 = ldexp(p1, -e);
+                p2 = ldexp(p2, -e);
+                q1 = ldexp(q1, -e);
+                q2 = ldexp(q2, -e);
+                e1 = ldexp(e1, -e);
+                e2 = ldexp(e2, -e);
+                ex += e;
+            });
+        o.p0 = p1;
+        o.q0 = q1;
+        o.e0 = e1;
+        o.ex = ex;
+        return o;
+    }
+
+    // forward continuants theta_{s-1}, theta_{r-1} of dz^2 T(k) (theta_{-1} = 1)
+    __device__ __forceinline__ void theta(double *ring, double k, int srow, int rrow, SVal &th_s, SVal &th_r) const {
+        th_s = SVal{1.0, 0};
+        th_r = SVal{1.0, 0};
+        const int top = srow > rrow ? srow : rrow;
+        double t1 = 1.0, t2 = 0.0;
+        int ex = 0;
+        const double *const arr[3] = {A, B, C};
+        sweep_up_n<3>(
+            arr, ring, top,
+            [&](const double (&v)[3], int i) {
+                const double t0 = fma(wide_row(v, k), t1, -t2);
+                t2 = t1;
+                t1 = t0;
+                if (i == srow - 1) th_s = SVal{t1, ex};
+                if (i == rrow - 1) th_r = SVal{t1, ex};
+            },
+            [&](int, int) {
+                const int e = pow2_exponent(t1);
+                t1 = ldexp(t1, -e);
+                t2 = ldexp(t2, -e);
+                ex += e;
+            });
+    }
+
+    // rows-pass sweeps with the row value a_i(k) handed to step(am, i)
+    template <class Step, class ChunkEnd>
+    __device__ __forceinline__ void up(double *ring, int m, double k, Step step, ChunkEnd ce) const {
+        const double *const arr[3] = {A, B, C};
+        sweep_up_n<3>(arr, ring, m, [&](const double (&v)[3], int i) { step(wide_row(v, k), i); }, ce);
+    }
+    template <class Step, class ChunkEnd>
+    __device__ __forceinline__ void down(double *ring, int m, double k, Step step, ChunkEnd ce) const {
+        const double *const arr[3] = {A, B, C};
+        sweep_down_n<3>(arr, ring, m, [&](const double (&v)[3], int i) { step(wide_row(v, k), i); }, ce);
+    }
+};
+
+}  // namespace ncpa

@@ -118,6 +118,15 @@ class Context:
         self.check(self.lib.ncpa_ess_solve(self.h, C.byref(grid), arr, len(problems), C.byref(h)))
         return Modes(self, h, len(problems), grid)
 
+    def solve_wide(self, grid, problems):
+        """Wide-angle (WMod) solve of a batch; the modes' grid spacing is the matrix
+        dz = (maxheight - z_min)/Nz of WModeSolver::solve."""
+        arr = (Problem * len(problems))(*problems)
+        h = C.c_void_p()
+        self.check(self.lib.ncpa_wmod_solve(self.h, C.byref(grid), arr, len(problems), C.byref(h)))
+        gm = Grid(grid.Nz, grid.z_min, (grid.maxheight - grid.z_min) / grid.Nz, grid.maxheight)
+        return Modes(self, h, len(problems), gm)
+
 
 def ess_eigen(ctx, diag, dz, k_min, k_max, max_modes=4000):
     """Drop-in of EigenEngine::doESSCalculation: (k2 [nconv], psi [Nz, nconv])."""
@@ -132,6 +141,21 @@ def ess_eigen(ctx, diag, dz, k_min, k_max, max_modes=4000):
                                      max_modes, C.byref(nconv), _dp(k2), _dp(v), max_modes))
     n = nconv.value
     return k2[:n], v[:, :n]
+
+
+def wmod_eigen(ctx, kd, md, cd, dz, k_min, k_max, max_modes=4000):
+    """Drop-in of EigenEngine::doWideAngleCalculation + doSelect: (kH [nconv], v [Nz, nconv])."""
+    kd = np.ascontiguousarray(kd, dtype=np.float64)
+    md = np.ascontiguousarray(md, dtype=np.float64)
+    cd = np.ascontiguousarray(cd, dtype=np.float64)
+    Nz = kd.size
+    nconv = C.c_int()
+    kh = np.zeros(max_modes)
+    v = np.zeros((Nz, max_modes))
+    ctx.check(ctx.lib.ncpa_wmod_eigen(ctx.h, _dp(kd), _dp(md), _dp(cd), Nz, dz, 1e-8, k_min, k_max,
+                                      max_modes, C.byref(nconv), _dp(kh), _dp(v), max_modes))
+    n = nconv.value
+    return kh[:n], v[:, :n]
 
 
 class Event:

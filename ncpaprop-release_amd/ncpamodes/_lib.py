@@ -43,6 +43,7 @@ EXPORTS = [
     "ncpa_ess_solve", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
     "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d",
     "ncpa_field2d_stage", "ncpa_field2d_contract", "ncpa_field_stage_free", "ncpa_field_compare", "ncpa_ess_eigen",
+    "ncpa_wmod_solve", "ncpa_wmod_eigen",
     "ncpa_fft_batch", "ncpa_kpert", "ncpa_bb_transfer", "ncpa_bb_pulse", "ncpa_bb_source",
     "ncpa_bb_plan_create", "ncpa_bb_plan_set_problems", "ncpa_bb_plan_run", "ncpa_bb_plan_result",
     "ncpa_bb_plan_tuning", "ncpa_bb_plan_destroy",
@@ -136,6 +137,9 @@ def load():
                                          C.POINTER(C.c_double), C.POINTER(C.c_size_t)]),
         "ncpa_ess_eigen": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int,
                                      ip, dp, dp, C.c_int]),
+        "ncpa_wmod_solve": (C.c_int, [vp, P(Grid), P(Problem), C.c_int, P(vp)]),
+        "ncpa_wmod_eigen": (C.c_int, [vp, dp, dp, dp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, ip, dp, dp, C.c_int]),
         "ncpa_fft_batch": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int]),
         "ncpa_kpert": (C.c_int, [vp, vp, vp, C.c_int, vp, vp]),
         "ncpa_bb_transfer": (C.c_int, [vp, C.c_int, C.c_int, ip, ip, vp, vp, vp, C.c_int, dp, C.c_int,

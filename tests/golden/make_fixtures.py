@@ -8,6 +8,8 @@ Copies (data files only, unchanged or line subsets):
   * samples/NCPA_canonical_profile_zuvwtdp.dat                         -> canonical_profile.dat
   * docs/.../modess_gp_files/code_outputs/tloss_1d.nm, tloss_1d.lossless.nm
         (Modess, canonical profile, 0.1 Hz, az 90, --write_2d_tloss; pre-2.0.1 scaling)
+  * docs/.../wmod_gp_files/code_outputs/wtloss_1d.nm, wtloss_1d.lossless.nm
+        (WMod, canonical profile, 0.1 Hz, az 90, --write_2d_tloss; pre-2.0.1 scaling)
   * samples/disp_256freqs_0p5hz_NCPA_canonic.dat, lines for 6 frequencies -> disp256_subset.dat
   * docs/.../modbb_gp_files/code_outputs/myDispersionFile.dat, 4 lines -> disp250_subset.dat
   * docs/.../modbb_gp_files/code_outputs/source_spectrum.dat            -> source_spectrum.dat
@@ -26,6 +28,7 @@ REF = "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODESS = os.path.join(REF, "docs/ncpaprop_run_manual_TeX/figs/modess_gp_files/code_outputs")
 MODBB = os.path.join(REF, "docs/ncpaprop_run_manual_TeX/figs/modbb_gp_files/code_outputs")
+WMOD = os.path.join(REF, "docs/ncpaprop_run_manual_TeX/figs/wmod_gp_files/code_outputs")
 
 
 def subset(src, dst, line_numbers):
@@ -41,6 +44,8 @@ def main():
                     os.path.join(HERE, "canonical_profile.dat"))
     for f in ("tloss_1d.nm", "tloss_1d.lossless.nm"):
         shutil.copyfile(os.path.join(MODESS, f), os.path.join(HERE, f))
+    for f in ("wtloss_1d.nm", "wtloss_1d.lossless.nm"):
+        shutil.copyfile(os.path.join(WMOD, f), os.path.join(HERE, f))
     # 0-based line indices: f = (i+1)/512 Hz
     subset(os.path.join(REF, "samples/disp_256freqs_0p5hz_NCPA_canonic.dat"),
            os.path.join(HERE, "disp256_subset.dat"), [0, 4, 25, 64, 127, 255])

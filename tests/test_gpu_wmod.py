@@ -1,0 +1,168 @@
+"""GPU parity: wide-angle (WMod) modes through the C ABI vs the FP64 oracle, the reference
+goldens and closed forms."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CANON = os.path.join(G, "canonical_profile.dat")
+
+
+def _uniform(N, dz, f, c, M):
+    from test_oracle_wmod import uniform_pencil, uniform_wavenumbers
+    return uniform_pencil(N, dz, f, c, M), uniform_wavenumbers(N, dz, f, c, M)
+
+
+@pytest.mark.parametrize("M", [0.0, 0.3, -0.3])
+def test_eigen_uniform_wind_closed_form(gpu_ctx, M):
+    """ncpa_wmod_eigen on constant c and wind: k to 1e-12 of the closed form, every vector
+    normalised as the reference's (sum psi^2 dz (1 + k^2) = 1)."""
+    import ncpamodes
+    N, dz, f, c = 1500, 50.0, 0.5, 340.0
+    print("case uniform M=%g" % M, flush=True)
+    (kd, md, cd), ref = _uniform(N, dz, f, c, M)
+    om = 2 * math.pi * f
+    kmin, kmax = 0.5 * om / c, 1.5 * om / c
+    ref = ref[(ref >= kmin) & (ref <= kmax)]
+    k, v = ncpamodes.wmod_eigen(gpu_ctx, kd, md, cd, dz, kmin, kmax)
+    assert k.size == ref.size > 10
+    assert np.max(np.abs(k / ref - 1)) < 1e-12
+    assert np.allclose((v * v).sum(axis=0) * dz * (1 + k * k), 1.0, rtol=1e-10)
+
+
+def test_eigen_vs_oracle(gpu_ctx):
+    """Canonical profile, 2000 rows, 0.2 Hz, az 270: k to 1e-11, |psi| to 1e-8 of max."""
+    import ncpamodes
+    from oracle import atmos, wmod
+    print("case eigen canonical N=2000 f=0.2", flush=True)
+    nodes = atmos.AtmosphereNodes(atmos.read_profile(CANON))
+    grid = wmod.WGrid(nodes, 2000, 150.0, 0.0)
+    kd, md, cd, diag, ceffz, kmin, kmax, used = wmod.modal_trace(grid, 0.2, 270.0, 0.0, 0.0, True)
+    ko, vo = wmod.eigen(kd, md, cd, grid.dz_m, kmin, kmax)
+    k, v = ncpamodes.wmod_eigen(gpu_ctx, kd, md, cd, grid.dz_m, kmin, kmax)
+    assert k.size == ko.size > 20
+    assert np.max(np.abs(k / ko - 1)) < 1e-11
+    assert np.max(np.abs(np.abs(v) - np.abs(vo))) < 1e-8 * np.abs(vo).max()
+
+
+def test_eigen_zero_wind_matches_ess_eigen(gpu_ctx):
+    """Without wind the wide-angle pencil is D - k^2 I: ncpa_wmod_eigen k^2 == ncpa_ess_eigen
+    eigenvalues and psi sqrt(1 + k^2) == the ESS vectors (the last ESS row re-uses diag[N-2],
+    so the test operator has equal last two diagonal entries)."""
+    import ncpamodes
+    print("case zero wind", flush=True)
+    N, dz, f = 3000, 40.0, 0.3
+    om = 2 * math.pi * f
+    z = np.arange(N) * dz
+    c = 300.0 + 40.0 * np.exp(-((z - 50e3) / 10e3) ** 2) + 0.0008 * z
+    kd = (om / c) ** 2
+    kd[-1] = kd[-2]
+    diag = kd.copy()
+    kd[0] += 1.0 / dz ** 2
+    diag[0] += 1.0 / dz ** 2
+    kmin, kmax = om / c.max(), om / c.min()
+    k, v = ncpamodes.wmod_eigen(gpu_ctx, kd, -np.ones(N), np.zeros(N), dz, kmin, kmax)
+    k2, u = ncpamodes.ess_eigen(gpu_ctx, diag, dz, kmin, kmax)
+    assert k.size == k2.size > 10
+    assert np.max(np.abs(k * k / k2 - 1)) < 1e-12
+    assert np.max(np.abs(np.abs(v) * np.sqrt(1 + k * k)[None, :] - np.abs(u))) < 1e-9 * np.abs(u).max()
+
+
+@pytest.fixture(scope="module")
+def canon_01(gpu_ctx):
+    """WMod canonical run of the reference goldens: 0.1 Hz, az 90, WKB off, 20000 rows."""
+    import ncpamodes
+    from oracle import atmos, wmod
+    pid, info = gpu_ctx.read_profile(CANON)
+    g = gpu_ctx.grid(pid)
+    m = gpu_ctx.solve_wide(g, [ncpamodes.problem(pid, 0.1, 90.0)])
+    nodes = atmos.AtmosphereNodes(atmos.read_profile(CANON))
+    sol = wmod.solve(nodes=nodes, freq=0.1, azimuth=90.0, turnoff_wkb=True)
+    return m, sol
+
+
+def test_solve_vs_oracle(canon_01):
+    m, sol = canon_01
+    print("case solve canonical 0.1 Hz", flush=True)
+    k, psr, ps, pr = m.problem_modes(0)
+    assert k.size == sol.k.size
+    assert np.max(np.abs(np.sqrt((k ** 2).real) / sol.k - 1)) < 1e-10
+    assert np.allclose(k.imag, sol.k_pert.imag, rtol=1e-6, atol=1e-16)
+    ref = sol.psi[sol.i_s] * sol.psi[sol.i_r]
+    assert np.max(np.abs(psr - ref)) <= 1e-7 * np.max(np.abs(ref))
+
+
+def test_tl1d_vs_goldens(canon_01):
+    """The same bounds as the oracle's golden test: lossless coherent 0.01 dB, lossless
+    incoherent 1e-6 relative (pins the 1/(1 + k^2) vector normalisation)."""
+    from oracle import ess
+    m, sol = canon_01
+    print("case tl1d goldens", flush=True)
+    g = np.loadtxt(os.path.join(G, "wtloss_1d.nm"))
+    gl = np.loadtxt(os.path.join(G, "wtloss_1d.lossless.nm"))
+    p, inc = m.tl1d(1000.0, 1000)
+    pl, incl = m.tl1d(1000.0, 1000, lossless=True)
+    pg = 1000 * (g[:, 1] + 1j * g[:, 2])
+    pgl = 1000 * (gl[:, 1] + 1j * gl[:, 2])
+    msk = ess.tl_db(pgl) > -140
+    assert np.max(np.abs(ess.tl_db(pl[0]) - ess.tl_db(pgl))[msk]) < 0.01
+    assert np.max(np.abs(incl[0] / (1000 * gl[:, 3]) - 1)) < 1e-6
+    assert np.max(np.abs(inc[0] / (1000 * g[:, 3]) - 1)) < 1e-3
+    notnull = ess.tl_db(pg) > ess.tl_db(1000 * g[:, 3]) - 10
+    assert np.max(np.abs(ess.tl_db(p[0]) - ess.tl_db(pg))[notnull]) < 0.1
+    r, po, inco = ess.tl1d(sol.k_pert, sol.psi[sol.i_s], sol.psi[sol.i_r], 1000.0, 1000)
+    assert np.max(np.abs(ess.tl_db(p[0]) - ess.tl_db(po))[ess.tl_db(po) > -140]) < 1e-4
+
+
+def test_rows_and_field_vs_oracle(canon_01):
+    """Eigenvector rows (the reference's 500 2-D rows) to 1e-7 of max |psi|; the FP64 2-D
+    field at the receiver row equals the 1-D TL."""
+    from oracle import ess
+    m, sol = canon_01
+    print("case rows + field", flush=True)
+    rows = ess.tl2d_rows(20000)
+    stride = int(rows[1] - rows[0])
+    R = m.rows(0, stride, rows.size)
+    ref = sol.psi[rows]
+    assert np.max(np.abs(np.abs(R) - np.abs(ref))) < 1e-7 * np.abs(ref).max()
+    f = m.field2d(0, stride, 2, 1000.0, 500, precision=0)
+    p, inc = m.tl1d(1000.0, 500)
+    assert np.allclose(f[0, 0], p[0], rtol=1e-10, atol=1e-12 * np.abs(p[0]).max())
+
+
+@pytest.mark.parametrize("f,az,zs", [(0.5, 45.0, 0.0), (0.25, 200.0, 20.0)])
+def test_synthetic_vs_oracle(gpu_ctx, synth_profile, f, az, zs):
+    """Jet profile (winds to ~70 m/s), 6000 rows, elevated source: counts, k, Im k, psi products."""
+    import ncpamodes
+    from oracle import wmod
+    print("case synthetic f=%g az=%g zs=%g" % (f, az, zs), flush=True)
+    pid, info = gpu_ctx.read_profile(synth_profile)
+    g = gpu_ctx.grid(pid, Nz=6000)
+    m = gpu_ctx.solve_wide(g, [ncpamodes.problem(pid, f, az, zs * 1000, zs * 1000)])
+    k, psr, ps, pr = m.problem_modes(0)
+    sol = wmod.solve(synth_profile, freq=f, azimuth=az, Nz=6000, zs_km=zs, zr_km=zs, turnoff_wkb=True)
+    assert k.size == sol.k.size > 10
+    assert np.max(np.abs(np.sqrt((k ** 2).real) / sol.k - 1)) < 1e-10
+    assert np.allclose(k.imag, sol.k_pert.imag, rtol=1e-6, atol=1e-16)
+    ref = sol.psi[sol.i_s] * sol.psi[sol.i_r]
+    assert np.max(np.abs(psr - ref)) <= 1e-7 * np.max(np.abs(ref))
+
+
+def test_batch_equals_single(gpu_ctx, synth_profile):
+    """A batch of (f, az) problems returns per problem what single solves return."""
+    import ncpamodes
+    print("case batch", flush=True)
+    pid, info = gpu_ctx.read_profile(synth_profile)
+    g = gpu_ctx.grid(pid, Nz=6000)
+    probs = [ncpamodes.problem(pid, f, az) for f, az in [(0.2, 0.0), (0.4, 90.0), (0.3, 180.0), (0.1, 270.0)]]
+    mb = gpu_ctx.solve_wide(g, probs)
+    for i, pb in enumerate(probs):
+        ms = gpu_ctx.solve_wide(g, [pb])
+        kb = mb.problem_modes(i)[0]
+        ks = ms.problem_modes(0)[0]
+        assert kb.size == ks.size
+        assert np.array_equal(kb, ks)
