@@ -59,11 +59,12 @@ def build(verbose=False, jobs=8):
 
 
 def build_cli(objdir, verbose=False):
-    """The Modess / ModBB command lines (host C++ over the C ABI), linked against the library."""
+    """The Modess / WMod / ModBB command lines (host C++ over the C ABI), linked against the library."""
     bindir = os.path.join(HERE, "bin")
     os.makedirs(bindir, exist_ok=True)
     common = ["cli/params.cpp", "cli/ess_run.cpp"]
     progs = {"Modess": common + ["cli/modess_main.cpp"],
+             "WMod": common + ["cli/wmod_main.cpp"],
              "ModBB": common + ["cli/bb_run.cpp", "cli/modbb_main.cpp"]}
     objs = {}
     procs = []

@@ -22,13 +22,75 @@ namespace ncpacli {
 
 using ncpa::PI;
 
+// Options of Modess (src/modess/modess_parameters.cpp:6-222) or, wide, of WMod
+// (src/wmod/wmod_parameters.cpp:6-214: the same set without --modal_starter_file and
+// --write_speeds; both default --paramfile to modess.param).
+void configure_modes_options(Options &p, bool wide) {
+    p.flag("help");
+    p.flag("h");
+    p.string("paramfile", "modess.param");
+    p.flag("printparams");
+    p.string("atmosfile");
+    p.required("atmosfile");
+    p.real("freq");
+    p.required("freq");
+    p.real_gt("freq", 0.0);
+    p.string("atmosheaderfile", "");
+    p.real("maxheight_km", 150.0);
+    p.real_ge("maxheight_km", 0.1);
+    p.real("zground_km", 0.0);
+    p.integer("Nz_grid", 20000);
+    p.int_ge("Nz_grid", 10);
+    p.real("sourceheight_km", 0.0);
+    p.real("receiverheight_km", 0.0);
+    p.real("maxrange_km", 1000.0);
+    p.real_ge("maxrange_km", 0.01);
+    p.integer("Nrng_steps", 1000);
+    p.int_gt("Nrng_steps", 0);
+    p.string("ground_impedence_model", "rigid");
+    p.one_of("ground_impedence_model", {"rigid"});
+    p.string("use_attn_file", "");
+    if (!wide) p.string("modal_starter_file", "");
+    p.string("dispersion_file", "");
+    p.flag("append_dispersion_file");
+    p.string("filetag", "");
+    p.flag("singleprop");
+    p.flag("multiprop");
+    p.radio("operation_mode", {"singleprop", "multiprop"});
+    p.real("azimuth");
+    p.real_ge("azimuth", 0.0);
+    p.real_lt("azimuth", 360.0);
+    p.required_if("azimuth", "singleprop");
+    for (const char *k : {"azimuth_start", "azimuth_end", "azimuth_step"}) {
+        p.real(k);
+        p.real_ge(k, 0.0);
+        p.real_le(k, 360.0);
+        p.required_if(k, "multiprop");
+    }
+    for (const char *k : {"write_2d_tloss", "write_lossless", "write_phase_speeds", "write_modes",
+                          "write_atm_profile", "Lamb_wave_BC", "turnoff_WKB", "wvnum_filter"})
+        p.flag(k);
+    if (!wide) p.flag("write_speeds");
+    p.real("c_min");
+    p.real_gt("c_min", 0.0);
+    p.required_if("c_min", "wvnum_filter");
+    p.real("c_max");
+    p.real_gt("c_max", 0.0);
+    p.required_if("c_max", "wvnum_filter");
+    // dummy parameters the solver reads (modess_parameters.cpp:200-205, wmod_parameters.cpp:181-184)
+    p.flag("broadband");
+    p.real("f_min", 0.0);
+    p.real("f_max", 0.0);
+    p.real("f_step", 0.0);
+}
+
 EssRun ess_from_options(const Options &p) {
     EssRun r;
     r.atmosfile = p.get_string("atmosfile");
     r.atmosheaderfile = p.get_string("atmosheaderfile");
     r.gnd_imp_model = p.get_string("ground_impedence_model");
     r.usrattfile = p.get_string("use_attn_file");
-    r.modstartfile = p.get_string("modal_starter_file");
+    if (p.has("modal_starter_file")) r.modstartfile = p.get_string("modal_starter_file");
     r.z_min = p.get_real("zground_km") * 1000.0;
     r.maxrange = p.get_real("maxrange_km") * 1000.0;
     r.maxheight = p.get_real("maxheight_km") * 1000.0;
@@ -40,7 +102,7 @@ EssRun ess_from_options(const Options &p) {
     r.write_2D_TLoss = p.get_bool("write_2d_tloss");
     r.write_atmosphere = p.get_bool("write_atm_profile");
     r.write_phase_speeds = p.get_bool("write_phase_speeds");
-    r.write_speeds = p.get_bool("write_speeds");
+    if (p.has("write_speeds")) r.write_speeds = p.get_bool("write_speeds");
     r.write_modes = p.get_bool("write_modes");
     r.dispersion_file = p.get_string("dispersion_file");
     r.append_dispersion_file = p.get_bool("append_dispersion_file");
@@ -54,7 +116,7 @@ EssRun ess_from_options(const Options &p) {
         r.c_min = p.get_real("c_min");
         r.c_max = p.get_real("c_max");
     }
-    // WKB is not used when any of these are saved (ESSModeSolver.cpp:78-81)
+    // WKB is not used when any of these are saved (ESSModeSolver.cpp:78-81, WModeSolver.cpp:75-77)
     if (r.write_phase_speeds || r.write_speeds || r.write_2D_TLoss || r.write_modes || !r.dispersion_file.empty())
         r.turnoff_WKB = true;
     if (r.Nby2Dprop) {   // ESSModeSolver.cpp:84-88
@@ -73,6 +135,7 @@ EssRun ess_from_options(const Options &p) {
     } else {
         r.freqs.push_back(p.get_real("freq"));
     }
+    if (p.has("method")) r.wide = p.get_string("method") == "wmod";   // ModBB_main.cpp:86-90
     if (r.gnd_imp_model != "rigid")
         throw std::invalid_argument("This ground impedance model is not implemented yet: " + r.gnd_imp_model);
     return r;
@@ -194,7 +257,9 @@ int ess_run(const EssRun &r) {
         if (g.z_min > r.z_min)
             std::cout << "Adjusting minimum altitude to profile ground height of " << g.z_min << " m" << std::endl;
         const int Nz = g.Nz;
-        const double dz = g.dz;
+        // WModeSolver::solve spaces the matrix rows by (maxheight - z_min)/Nz (WModeSolver.cpp:297)
+        const double dz = r.wide ? (g.maxheight - g.z_min) / Nz : g.dz;
+        const char *ext = r.wide ? ".wnm" : ".nm";   // WModeSolver.h:4-34 file names
         std::vector<double> rho(Nz);
         s.check(ncpa_profile_sample(s.ctx, pid, &g, rho.data(), nullptr, nullptr, nullptr));
         double adm = 0.0;
@@ -223,7 +288,10 @@ int ess_run(const EssRun &r) {
                 probs.push_back(q);
                 if (fi == 0) azs.push_back(azi);
             }
-        s.check(ncpa_ess_solve(s.ctx, &g, probs.data(), P, &s.modes));
+        if (r.wide)
+            s.check(ncpa_wmod_solve(s.ctx, &g, probs.data(), P, &s.modes));
+        else
+            s.check(ncpa_ess_solve(s.ctx, &g, probs.data(), P, &s.modes));
         std::vector<int> counts(P);
         int total = 0;
         s.check(ncpa_modes_counts(s.modes, counts.data(), &total));
@@ -250,8 +318,8 @@ int ess_run(const EssRun &r) {
             }
         }
         auto tl_name = [&](bool multi, bool lossless) {
-            return r.user_tag + (multi ? (lossless ? "Nby2D_tloss_1d.lossless.nm" : "Nby2D_tloss_1d.nm")
-                                       : (lossless ? "tloss_1d.lossless.nm" : "tloss_1d.nm"));
+            return r.user_tag + (multi ? (lossless ? "Nby2D_tloss_1d.lossless" : "Nby2D_tloss_1d")
+                                       : (lossless ? "tloss_1d.lossless" : "tloss_1d")) + ext;
         };
         const int last = P - 1;
 
@@ -265,7 +333,10 @@ int ess_run(const EssRun &r) {
                 std::cout << std::endl << "Now processing azimuth = " << azi << " (" << it + 1 << " of " << naz << ")"
                           << std::endl;
                 std::printf("______________________________________________________________________\n\n");
-                std::printf(" -> Normal mode solution at %5.3f Hz and %5.2f deg (%d modes)...\n", freq, azi, nm);
+                if (r.wide)
+                    std::printf(" -> Solving wide-angle problem at %g Hz and %g deg (%d modes)...\n", freq, azi, nm);
+                else
+                    std::printf(" -> Normal mode solution at %5.3f Hz and %5.2f deg (%d modes)...\n", freq, azi, nm);
                 std::printf(" -> Discrete spectrum: %5.2f m/s to %5.2f m/s\n", 2 * PI * freq / kmax[p],
                             2 * PI * freq / kmin[p]);
                 if (r.Nby2Dprop) {
@@ -331,7 +402,7 @@ int ess_run(const EssRun &r) {
                             s.check(ncpa_field2d(s.ctx, s.modes, d_rows, nrows, dr, n_r, ll, NCPA_PREC_FP64, d_out));
                             std::vector<double> fld(per);
                             s.download(fld.data(), d_out + (size_t)p * per, per);
-                            const std::string name = r.user_tag + (ll ? "tloss_2d.lossless.nm" : "tloss_2d.nm");
+                            const std::string name = r.user_tag + (ll ? "tloss_2d.lossless" : "tloss_2d") + ext;
                             FILE *f = open_or_throw(name, "w");
                             for (int i = 0; i < n_r; ++i) {
                                 for (int j = 0; j < nrows; ++j) {
@@ -348,7 +419,7 @@ int ess_run(const EssRun &r) {
                     if (r.write_phase_speeds) {
                         // ModeSolver::writePhaseSpeeds (ModeSolver.cpp:603-614)
                         std::cout << "Writing to file: phase speeds" << std::endl;
-                        const std::string name = r.user_tag + "phasespeeds.nm";
+                        const std::string name = r.user_tag + "phasespeeds" + ext;
                         FILE *f = open_or_throw(name, "w");
                         for (int j = 0; j < nm; ++j)
                             std::fprintf(f, "%d %f %15.8e\n", j, (2 * PI * freq) / kre[o + j], kim[o + j]);
@@ -365,7 +436,7 @@ int ess_run(const EssRun &r) {
                             std::cout << "Writing to file: the modes and the phase and group speeds...\n";
                             for (int j = 0; j < nm; ++j) {
                                 char name[64];
-                                std::snprintf(name, sizeof(name), "mode_%d.nm", j);
+                                std::snprintf(name, sizeof(name), "mode_%d%s", j, ext);
                                 FILE *f = open_or_throw(r.user_tag + name, "w");
                                 double chk = 0.0;
                                 for (int n = 0; n < Nz; ++n) {
@@ -380,24 +451,27 @@ int ess_run(const EssRun &r) {
                         } else {
                             std::cout << "Writing to file: the modal phase speeds and the group speeds...\n";
                         }
-                        // ESSModeSolver::writePhaseAndGroupSpeeds (ESSModeSolver.cpp:726-752)
-                        std::vector<double> ceff(Nz);
-                        s.check(ncpa_modes_ceff(s.modes, p, ceff.data()));
-                        const std::string name = r.user_tag + "speeds.nm";
-                        FILE *f = open_or_throw(name, "w");
-                        const double omega = 2 * PI * freq;
-                        for (int j = 0; j < nm; ++j) {
-                            const double v_phase = omega / kre[o + j];
-                            double vg = 0.0;
-                            for (int n = 0; n < Nz; ++n) {
-                                const double x = v[(size_t)n * nt + o + j];
-                                vg += x * x / (ceff[n] * ceff[n]);
+                        // ESSModeSolver::writePhaseAndGroupSpeeds (ESSModeSolver.cpp:726-752);
+                        // WModeSolver writes the mode files only (WModeSolver.cpp:431-434)
+                        if (!r.wide) {
+                            std::vector<double> ceff(Nz);
+                            s.check(ncpa_modes_ceff(s.modes, p, ceff.data()));
+                            const std::string name = r.user_tag + "speeds.nm";
+                            FILE *f = open_or_throw(name, "w");
+                            const double omega = 2 * PI * freq;
+                            for (int j = 0; j < nm; ++j) {
+                                const double v_phase = omega / kre[o + j];
+                                double vg = 0.0;
+                                for (int n = 0; n < Nz; ++n) {
+                                    const double x = v[(size_t)n * nt + o + j];
+                                    vg += x * x / (ceff[n] * ceff[n]);
+                                }
+                                vg = 1.0 / (vg * v_phase * dz);
+                                std::fprintf(f, "%4d %9.3f %9.3f %15.8e\n", j + 1, v_phase, vg, kim[o + j]);
                             }
-                            vg = 1.0 / (vg * v_phase * dz);
-                            std::fprintf(f, "%4d %9.3f %9.3f %15.8e\n", j + 1, v_phase, vg, kim[o + j]);
+                            std::fclose(f);
+                            std::printf("           Phase and group speeds saved in file %s.\n", name.c_str());
                         }
-                        std::fclose(f);
-                        std::printf("           Phase and group speeds saved in file %s.\n", name.c_str());
                     }
                 }
                 // the dispersion file and the atmosphere dump belong to the single-azimuth branch
@@ -415,7 +489,7 @@ int ess_run(const EssRun &r) {
                     std::fclose(f);
                 }
                 if (!r.Nby2Dprop && r.write_atmosphere && fi == 0) {
-                    const std::string name = r.user_tag + "atm_profile.nm";
+                    const std::string name = r.user_tag + "atm_profile" + ext;
                     std::cout << "Writing atmosphere to " << name << std::endl;
                     write_atmosphere(name, info, zn, c0n, wsn, wdn, Tn, Pn, rhon, azi);
                 }

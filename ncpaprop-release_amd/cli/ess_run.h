@@ -1,6 +1,7 @@
-// The effective-sound-speed normal-mode run behind the Modess and ModBB --dispersion
-// command lines: ESSModeSolver::setParams + ESSModeSolver::solve (src/libmodes/ESSModeSolver.cpp)
-// on the GPU through the ncpamodes C ABI, with the reference's output files and formats.
+// The normal-mode run behind the Modess, WMod and ModBB --dispersion command lines:
+// ESSModeSolver / WModeSolver setParams + solve (src/libmodes/ESSModeSolver.cpp,
+// WModeSolver.cpp) on the GPU through the ncpamodes C ABI, with the reference's output files
+// and formats.
 #pragma once
 #include <string>
 #include <vector>
@@ -21,9 +22,14 @@ struct EssRun {
     std::vector<double> freqs;     // f_vec (one entry unless broadband)
     double azi_min = 0.0, azi_step = 0.0;
     int Naz = 1;
+    bool wide = false;             // WModeSolver (WMod, ModBB --method wmod) instead of ESSModeSolver
 };
 
-// ESSModeSolver::setParams option reading (ESSModeSolver.cpp:30-124)
+// Options of Modess, or (wide) of WMod
+void configure_modes_options(Options &p, bool wide);
+
+// ESSModeSolver::setParams option reading (ESSModeSolver.cpp:30-124; WModeSolver::setParams,
+// WModeSolver.cpp:30-115, reads the same options)
 EssRun ess_from_options(const Options &p);
 
 // Runs every (frequency, azimuth) problem as one batch on the GPU and writes the outputs.

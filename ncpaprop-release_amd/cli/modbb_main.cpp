@@ -1,7 +1,7 @@
-// ModBB: broadband normal modes (effective sound speed), --dispersion then --propagation, on the
-// GPU.  Options, defaults and validation follow src/modbb/modbb_parameters.cpp:6-260 and the
-// program flow src/modbb/ModBB_main.cpp:33-121.  --method wmod is accepted by the option
-// checks but not built here (the wide-angle solver is a different eigenproblem).
+// ModBB: broadband normal modes, --dispersion (--method modess: effective sound speed;
+// --method wmod: wide-angle high-Mach) then --propagation, on the GPU.  Options, defaults and
+// validation follow src/modbb/modbb_parameters.cpp:6-260 and the program flow
+// src/modbb/ModBB_main.cpp:33-121.
 #include <ctime>
 #include <fstream>
 #include <iostream>
@@ -134,10 +134,6 @@ int main(int argc, char **argv) {
     int rc = 0;
     try {
         if (p.get_bool("dispersion")) {
-            if (p.get_string("method") != "modess") {
-                std::cerr << "--method wmod (wide-angle high-Mach modes) is not part of this build" << std::endl;
-                return 1;
-            }
             const std::string disp = p.get_string("dispersion_file");
             if (!disp.empty()) std::ofstream(disp, std::ios_base::trunc).close();   // ModBB_main.cpp:77-82
             rc = ess_run(ess_from_options(p));

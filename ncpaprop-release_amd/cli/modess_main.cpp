@@ -12,64 +12,6 @@ using namespace ncpacli;
 
 namespace {
 
-void configure(Options &p) {
-    p.flag("help");
-    p.flag("h");
-    p.string("paramfile", "modess.param");
-    p.flag("printparams");
-    p.string("atmosfile");
-    p.required("atmosfile");
-    p.real("freq");
-    p.required("freq");
-    p.real_gt("freq", 0.0);
-    p.string("atmosheaderfile", "");
-    p.real("maxheight_km", 150.0);
-    p.real_ge("maxheight_km", 0.1);
-    p.real("zground_km", 0.0);
-    p.integer("Nz_grid", 20000);
-    p.int_ge("Nz_grid", 10);
-    p.real("sourceheight_km", 0.0);
-    p.real("receiverheight_km", 0.0);
-    p.real("maxrange_km", 1000.0);
-    p.real_ge("maxrange_km", 0.01);
-    p.integer("Nrng_steps", 1000);
-    p.int_gt("Nrng_steps", 0);
-    p.string("ground_impedence_model", "rigid");
-    p.one_of("ground_impedence_model", {"rigid"});
-    p.string("use_attn_file", "");
-    p.string("modal_starter_file", "");
-    p.string("dispersion_file", "");
-    p.flag("append_dispersion_file");
-    p.string("filetag", "");
-    p.flag("singleprop");
-    p.flag("multiprop");
-    p.radio("operation_mode", {"singleprop", "multiprop"});
-    p.real("azimuth");
-    p.real_ge("azimuth", 0.0);
-    p.real_lt("azimuth", 360.0);
-    p.required_if("azimuth", "singleprop");
-    for (const char *k : {"azimuth_start", "azimuth_end", "azimuth_step"}) {
-        p.real(k);
-        p.real_ge(k, 0.0);
-        p.real_le(k, 360.0);
-        p.required_if(k, "multiprop");
-    }
-    for (const char *k : {"write_2d_tloss", "write_lossless", "write_phase_speeds", "write_speeds", "write_modes",
-                          "write_atm_profile", "Lamb_wave_BC", "turnoff_WKB", "wvnum_filter"})
-        p.flag(k);
-    p.real("c_min");
-    p.real_gt("c_min", 0.0);
-    p.required_if("c_min", "wvnum_filter");
-    p.real("c_max");
-    p.real_gt("c_max", 0.0);
-    p.required_if("c_max", "wvnum_filter");
-    // dummy parameters the solver reads (modess_parameters.cpp:200-205)
-    p.flag("broadband");
-    p.real("f_min", 0.0);
-    p.real("f_max", 0.0);
-    p.real("f_step", 0.0);
-}
-
 void usage(std::ostream &os) {
     os << "Modess -- effective sound speed normal modes (MI355X)\n\n"
           "Required: --atmosfile <file> --freq <Hz> and one of --singleprop --azimuth <deg> |\n"
@@ -89,7 +31,7 @@ void usage(std::ostream &os) {
 
 int main(int argc, char **argv) {
     Options p;
-    configure(p);
+    configure_modes_options(p, false);
     try {
         p.parse_command_line(argc, argv);
         if (p.was_found("help") || p.was_found("h")) {

@@ -267,6 +267,8 @@ void Options::print_failed(std::ostream &os) const {
     for (auto &m : failed_) os << m << std::endl;
 }
 
+bool Options::has(const std::string &key) const { return find(key) != nullptr; }
+
 bool Options::was_found(const std::string &key) const {
     const Option *o = find(key);
     if (!o) throw std::invalid_argument("No parameter '" + key + "' has been specified!");

@@ -52,6 +52,7 @@ public:
     bool validate();
     void print_failed(std::ostream &os) const;
 
+    bool has(const std::string &key) const;   // the option is configured
     bool was_found(const std::string &key) const;
     bool get_bool(const std::string &key) const;
     int get_int(const std::string &key) const;

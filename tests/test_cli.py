@@ -1,5 +1,5 @@
-"""Modess / ModBB command lines: option handling (CPU) and end-to-end runs against the FP64
-oracle (GPU).  The programs are the C++ drop-ins built into ncpaprop-release_amd/bin."""
+"""Modess / WMod / ModBB command lines: option handling (CPU) and end-to-end runs against the
+FP64 oracle (GPU).  The programs are the C++ drop-ins built into ncpaprop-release_amd/bin."""
 import os
 import re
 import subprocess
@@ -18,7 +18,7 @@ def run(prog, args, cwd, timeout=300):
 
 
 def test_programs_built():
-    for prog in ("Modess", "ModBB"):
+    for prog in ("Modess", "WMod", "ModBB"):
         assert os.access(os.path.join(BIN, prog), os.X_OK), prog
 
 
@@ -50,6 +50,21 @@ def test_param_file_and_override(tmp_path):
     assert "Parameter validation failed" not in out.stdout
     out = run("Modess", ["--freq", "-2"], tmp_path)
     assert "freq (-2.000000) must be greater than 0.000000." in out.stdout
+
+
+def test_wmod_options(tmp_path):
+    """WMod takes the Modess options except --modal_starter_file and --write_speeds
+    (wmod_parameters.cpp:6-214); validation messages as Modess."""
+    out = run("WMod", ["--help"], ROOT)
+    assert out.returncode == 1 and "tloss_1d.wnm" in out.stdout and "write_speeds" not in out.stdout
+    out = run("WMod", ["--freq", "0.1"], tmp_path)
+    assert "atmosfile is not present." in out.stdout
+    out = run("WMod", ["--atmosfile", "x", "--freq", "0.1", "--singleprop", "--azimuth", "90", "--write_speeds"],
+              tmp_path)
+    assert "Unknown option '--write_speeds'." in out.stdout
+    out = run("WMod", ["--atmosfile", "x", "--freq", "0.1", "--singleprop", "--azimuth", "90",
+                       "--modal_starter_file", "s.pe"], tmp_path)
+    assert "Unknown option '--modal_starter_file'." in out.stdout
 
 
 def test_modbb_validation(tmp_path):
@@ -201,3 +216,73 @@ def test_modbb_dispersion_then_propagation(tmp_path, synth_profile):
     src = np.loadtxt(tmp_path / "source_spectrum_input.dat")
     S, _ = obb.source_spectrum("pulse1", disp["f"], nfft, disp["f"][2] - disp["f"][1])
     assert np.allclose(src[:, 1] + 1j * src[:, 2], S, rtol=1e-5, atol=1e-6 * np.abs(S).max())
+
+
+@pytest.mark.gpu
+def test_wmod_singleprop_outputs(tmp_path, synth_profile):
+    """WMod end to end: .wnm files (WModeSolver.h:4-34), 1-D / 2-D TL, phase speeds, mode files
+    and the dispersion line against the wide-angle oracle; matrix rows every H/Nz."""
+    from oracle import ess, wmod
+    f, az, Nz = 0.3, 60.0, 6000
+    print("case WMod singleprop f=%g az=%g Nz=%d" % (f, az, Nz), flush=True)
+    args = ["--atmosfile", synth_profile, "--freq", str(f), "--singleprop", "--azimuth", str(az),
+            "--Nz_grid", str(Nz), "--maxrange_km", "500", "--Nrng_steps", "500", "--write_2d_tloss",
+            "--write_lossless", "--write_phase_speeds", "--write_modes", "--write_atm_profile",
+            "--dispersion_file", "disp.dat"]
+    out = run("WMod", args, tmp_path)
+    assert out.returncode == 0, out.stdout + out.stderr
+    sol = wmod.solve(synth_profile, freq=f, azimuth=az, Nz=Nz, turnoff_wkb=True)
+    nm = sol.k.size
+    a = np.loadtxt(tmp_path / "tloss_1d.wnm")
+    r, po, inco = ess.tl1d(sol.k_pert, sol.psi[sol.i_s], sol.psi[sol.i_r], 1000.0, 500)
+    assert a.shape == (500, 4) and np.allclose(a[:, 0], r / 1000.0, atol=1e-6)
+    tl, tlo = _tl_db(a[:, 1] + 1j * a[:, 2]), _tl_db(po)
+    assert np.max(np.abs(tl - tlo)[tlo > -140]) <= 0.1
+    assert np.max(np.abs(a[:, 3] / inco - 1)) <= 1e-6
+    assert (tmp_path / "tloss_1d.lossless.wnm").exists() and not (tmp_path / "tloss_1d.nm").exists()
+    rows = ess.tl2d_rows(Nz)
+    t2 = np.loadtxt(tmp_path / "tloss_2d.wnm")
+    assert t2.shape == (500 * rows.size, 4)
+    _, F = ess.tl2d(sol.k_pert, sol.psi[rows], sol.psi[sol.i_s], 1000.0, 500)
+    ours = (t2[:, 2] + 1j * t2[:, 3]).reshape(500, rows.size)
+    tl2, tl2o = _tl_db(ours), _tl_db(F)
+    assert np.max(np.abs(tl2 - tl2o)[tl2o > -140]) <= 0.1
+    assert np.allclose(t2[:rows.size, 1], rows * sol.grid.dz_m / 1000.0, atol=1e-6)
+    ph = np.loadtxt(tmp_path / "phasespeeds.wnm")
+    assert ph.shape == (nm, 3) and np.allclose(ph[:, 1], 2 * np.pi * f / sol.k_pert.real, rtol=1e-8)
+    for j in (0, nm // 2, nm - 1):
+        md = np.loadtxt(tmp_path / ("mode_%d.wnm" % j))
+        assert md.shape == (Nz, 2)
+        assert np.allclose(md[1, 0], sol.grid.dz_m / 1000.0, rtol=1e-6)
+        assert np.max(np.abs(np.abs(md[:, 1]) - np.abs(sol.psi[:, j]))) <= 1e-6 * np.max(np.abs(sol.psi[:, j])) + 1e-7
+    assert not (tmp_path / "speeds.nm").exists()
+    d = (tmp_path / "disp.dat").read_text().split()
+    assert float(d[0]) == pytest.approx(f) and int(d[1]) == nm
+    q = np.array(d[4:], dtype=float).reshape(nm, 4)
+    assert np.allclose(q[:, 0], sol.k_pert.real, rtol=1e-9)
+    ref = sol.psi[sol.i_s] * sol.psi[sol.i_r]
+    assert np.max(np.abs(q[:, 2] * q[:, 3] - ref)) <= 1e-6 * np.max(np.abs(ref))
+    assert (tmp_path / "atm_profile.wnm").exists()
+
+
+@pytest.mark.gpu
+def test_modbb_wmod_dispersion(tmp_path, synth_profile):
+    """ModBB --dispersion --method wmod writes the wide-angle dispersion lines (ModBB_main.cpp:86-90)."""
+    from oracle import wmod
+    Nz = 4000
+    print("case ModBB wmod dispersion Nz=%d" % Nz, flush=True)
+    out = run("ModBB", ["--dispersion", "--dispersion_file", "disp.dat", "--atmosfile", synth_profile,
+                        "--azimuth", "90", "--f_min", "0.02", "--f_step", "0.02", "--f_max", "0.1",
+                        "--Nz_grid", str(Nz), "--method", "wmod"], tmp_path)
+    assert out.returncode == 0 and "validation failed" not in out.stdout, out.stdout + out.stderr
+    lines = (tmp_path / "disp.dat").read_text().splitlines()
+    assert len(lines) == 5
+    for i in (0, 4):
+        v = lines[i].split()
+        f = float(v[0])
+        sol = wmod.solve(synth_profile, freq=f, azimuth=90.0, Nz=Nz, turnoff_wkb=True)
+        q = np.array(v[4:], dtype=float).reshape(int(v[1]), 4)
+        assert q.shape[0] == sol.k.size
+        assert np.allclose(q[:, 0], sol.k_pert.real, rtol=1e-9)
+        ref = sol.psi[sol.i_s] * sol.psi[sol.i_r]
+        assert np.max(np.abs(q[:, 2] * q[:, 3] - ref)) <= 1e-6 * np.max(np.abs(ref))
