@@ -23,7 +23,10 @@ against FP64 (time, speed-up, max/RMS dB error).
 16 receiver ranges (200-1000 km) and pulse synthesis (NFFT = max(4096, 4 Nfreq) = 4100, the
 reference's rule) with a Gaussian-windowed 0.5 Hz source given as a spectrum.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload field|broadband|serving|precision]
+--workload wmod: the field workload's 64-problem batch through the wide-angle (WMod) solver
+(eigenpairs + attenuation + 1-D TL, FP64), the effective-sound-speed batch timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload field|broadband|serving|precision|wmod]
                     [--schedule pipeline|streams] [--streams S] [--no-cpu]
 
 Prints one JSON line.  cpu_baseline: the FP64 oracle (scipy LAPACK bisection + inverse
@@ -53,6 +56,7 @@ NR = 10000
 MAXRANGE_M = 1000e3
 FIELD_ROWS, FIELD_STRIDE = 1500, 13        # 1500 depths, 0 - 146 km AGL
 FLOPS_PER_STURM_STEP = 3.0                 # one row of one sweep
+WIDE_CPU_STRIDE = 8                        # wide-angle CPU baseline: every 8th eigenvalue branch
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
 PREC_NAMES = {0: "f64", 1: "f32", 2: "bf16x3", 3: "bf16x6", 4: "fp16x3", 5: "fp16x4", 6: "bf16x6f",
               7: "fp16x4f"}
@@ -111,6 +115,26 @@ def cpu_worker(spec):
         t_f = time.perf_counter() - t
         out["solve_s"] = t_eig + t_tl + t_f * (NR / nr_s)
         out["parts"] = [t_eig, t_tl, t_f]
+    elif spec["workload"] == "wmod":
+        # a wide-angle solve on a sample of its eigenvalue branches (every WIDE_CPU_STRIDE-th),
+        # the eigen time scaled by the branch ratio; trace, perturbation and TL timed in full
+        from oracle import atmos, wmod
+        t = time.perf_counter()
+        grid = wmod.WGrid(atmos.AtmosphereNodes(atmos.read_profile(spec["path"])), NZ, 150.0, 0.0)
+        grid.set_attenuation(FREQ)
+        kd, md, cd, _, _, kmin, kmax, _ = wmod.modal_trace(grid, FREQ, spec["az"], 0.0, 0.0, True)
+        t_tr = time.perf_counter() - t
+        t = time.perf_counter()
+        k, v = wmod.eigen(kd, md, cd, grid.dz_m, kmin, kmax, stride=WIDE_CPU_STRIDE)
+        t_eig = (time.perf_counter() - t) * WIDE_CPU_STRIDE
+        t = time.perf_counter()
+        kp = ess.perturb(k, v, wmod.perturb_weights(grid, FREQ))
+        kp = np.repeat(kp, WIDE_CPU_STRIDE)
+        ps = np.repeat(v[0], WIDE_CPU_STRIDE)
+        ess.tl1d(kp, ps, ps, MAXRANGE_M / NR, NR)
+        t_tl = time.perf_counter() - t
+        out["solve_s"] = t_tr + t_eig + t_tl
+        out["parts"] = [t_tr, t_eig, t_tl]
     else:
         from oracle import broadband as obb
         t = time.perf_counter()
@@ -143,10 +167,10 @@ def cpu_baseline(workload, n_workers):
         env[k] = "1"
     tmp = tempfile.mkdtemp(prefix="ncpa_cpu_")
     specs = []
-    if workload == "field":
+    if workload in ("field", "wmod"):
         paths = write_profiles(tmp)
         for i in range(n_workers):
-            specs.append({"workload": "field", "path": paths[i % len(paths)],
+            specs.append({"workload": workload, "path": paths[i % len(paths)],
                           "az": AZIMUTHS[(i * 3) % len(AZIMUTHS)], "field_ranges": 100})
     else:
         path = write_profiles(tmp, [0])[0]
@@ -600,6 +624,98 @@ def broadband_main(args, cpu):
     return line
 
 
+FLOPS_PER_WIDE_STEP = 6.0   # one row of one wide-angle count sweep: a_i(k) (2 FMA) + continuant (1 FMA)
+
+
+def wmod_main(args, cpu):
+    """Wide-angle (WMod) solves: one step = the 64-problem batch of the field workload (1 Hz,
+    8 profiles x 8 azimuths, Nz = 20000) through ncpa_wmod_solve (every real eigenvalue of
+    the quadratic problem in [k_min, k_max], perturbative attenuation) + 1-D TL at 10000
+    ranges, all FP64.  The same batch through the effective-sound-speed solver is timed
+    beside it."""
+    import numpy as np
+    import ncpamodes
+    tmp = tempfile.mkdtemp(prefix="ncpa_bench_")
+    paths = write_profiles(tmp)
+    ctx = ncpamodes.Context(0)
+    pids = [ctx.read_profile(p)[0] for p in paths]
+    grid = ctx.grid(pids[0], Nz=NZ)
+    problems = [ncpamodes.problem(pid, FREQ, az) for pid in pids for az in AZIMUTHS]
+    P = len(problems)
+    dr = MAXRANGE_M / NR
+    bufs = [ctx.empty(P * NR) for _ in range(3)]
+
+    def step(wide):
+        m = ctx.solve_wide(grid, problems) if wide else ctx.solve(grid, problems)
+        m.tl1d_device(dr, NR, False, *[b.ptr for b in bufs])
+        return m
+
+    # one step with kernel timing: the stage split and the Sturm-step count
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    m = step(True)
+    ctx.synchronize()
+    total_modes = m.total
+    its = np.abs(m.iterations()).astype(np.float64)
+    # sweeps per eigenvalue: the search's count/Newton sweeps + the final and forward sweeps
+    sweeps = float((its + 2.0).sum())
+    fams = {n: ctx.kernel_time(n) for n in ("setup", "interval", "modes", "tl1d")}
+    ctx.set_timing(False)
+    m.free()
+
+    def timed(wide):
+        for _ in range(args.warmup):
+            step(wide).free()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(wide).free()
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    elapsed = timed(True)
+    ess_elapsed = timed(False)
+    for b in bufs:
+        b.free()
+    ctx.close()
+    value = P * args.steps / elapsed
+    modes_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
+    achieved = FLOPS_PER_WIDE_STEP * sweeps * NZ / modes_s / 1e12
+    line = {
+        "metric": "wide-angle (WMod) modal TL solves/sec, FP64",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "WMod batch: 64 solves at 1 Hz (8 seeded jet profiles x 8 azimuths), "
+                               "Nz=20000, every real eigenvalue of the quadratic problem in [k_min, k_max] "
+                               "with perturbative attenuation, 1-D TL at 10000 ranges (0.1-1000 km)",
+                   "global_batch": P, "modes_per_step": int(total_modes), "parallelism": "single GPU"},
+        "stage_ms": {k: v[0] for k, v in fams.items()},
+        "ess_same_batch": {"solves_per_s": P * args.steps / ess_elapsed,
+                           "wide_over_ess_time": elapsed / ess_elapsed},
+        "roofline": {"kernel": "k_wmodes<2>", "bound": "mfma", "achieved": achieved,
+                     "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"],
+                     "traffic": None,
+                     "algorithmic": "6 FP64 flops per row per sweep (the row value a_i(k) = A + k(B + kC) "
+                                    "and the continuant; Newton sweeps do 12, counted as 6); sweeps = "
+                                    "search sweeps + 2 per eigenvalue; FP64 VALU, MI355X FP64 peak"},
+        "cpu_baseline": None,
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = {
+            "value": cpu["rate"], "unit": "solves/s", "cores": cpu["workers"], "kind": "port",
+            "sample": "%d concurrent 1-thread processes, each one 1 Hz wide-angle solve (FP64 oracle: "
+                      "branch crossings by Brent on LAPACK bisection, inverse-iteration vectors) on every "
+                      "%dth eigenvalue branch, eigen time scaled by %d; trace, perturbation and 1-D TL at "
+                      "10000 ranges in full; mean %.1f s per solve"
+                      % (cpu["workers"], WIDE_CPU_STRIDE, WIDE_CPU_STRIDE, cpu["mean_solve_s"])}
+        line["speedup_vs_cpu_all_cores"] = value / cpu["rate"]
+    return line
+
+
 def precision_main(args):
     """Config 3: the 2-D field at 2 Hz on 1500 depths x 10000 ranges, every contraction precision
     against the FP64 contraction of the same modes: time per launch (HIP events), speed-up over
@@ -730,7 +846,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["field", "broadband", "serving", "precision"], default="field")
+    ap.add_argument("--workload", choices=["field", "broadband", "serving", "precision", "wmod"], default="field")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", choices=["pipeline", "streams"], default="pipeline",
                     help="field workload headline: two-stage pipeline over two streams, or "
@@ -758,6 +874,8 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(args.workload, cpu_cores())
     if args.workload == "field":
         line = field_main(args, cpu)
+    elif args.workload == "wmod":
+        line = wmod_main(args, cpu)
     else:
         line = broadband_main(args, cpu)
     print(json.dumps(line), flush=True)

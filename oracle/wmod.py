@@ -132,15 +132,17 @@ def _mu(P, k, i):
                                                select_range=(i, i), lapack_driver="stebz")[0])
 
 
-def eigen(kd, md, cd, dz, k_min, k_max, vectors=True):
+def eigen(kd, md, cd, dz, k_min, k_max, vectors=True, stride=1):
     """All real QEP eigenvalues in (k_min, k_max], ascending, with the reference's vector
     normalisation v = x_{0:N} / sqrt(dz), |x|_2 = 1 (EigenEngine.cpp:376-381).
+    stride > 1 computes every stride-th branch only (a timing sample).
 
     Returns (k [nconv], v [Nz, nconv] or None)."""
     P = pencil(kd, md, cd, dz)
     n_lo, n_hi = _n_negative(P, k_min), _n_negative(P, k_max)
+    branches = range(n_lo, n_hi, stride)
     ks = []
-    for i in range(n_lo, n_hi):
+    for i in branches:
         ks.append(scipy.optimize.brentq(lambda k: _mu(P, k, i), k_min, k_max, xtol=1e-300,
                                         rtol=4 * np.finfo(float).eps, maxiter=200))
     ks = np.array(ks)
@@ -148,7 +150,7 @@ def eigen(kd, md, cd, dz, k_min, k_max, vectors=True):
         return ks, None
     V = np.empty((kd.size, ks.size))
     e = np.ones(kd.size - 1)
-    for j, (i, k) in enumerate(zip(range(n_lo, n_hi), ks)):
+    for j, (i, k) in enumerate(zip(branches, ks)):
         _, u = scipy.linalg.eigh_tridiagonal(_diag_at(P, k), e, select="i", select_range=(i, i),
                                              lapack_driver="stebz")
         V[:, j] = u[:, 0] / math.sqrt(dz * (1.0 + k * k))
