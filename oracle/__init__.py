@@ -27,4 +27,8 @@ Pinning (what checks each part; see tests/test_oracle_*.py):
                                        and the reference tloss_1d goldens (x1000 scaling).
   * broadband synthesis             -- numpy.fft == FFTW_FORWARD, closed-form pulses, the
                                        reference source_spectrum golden.
+  * wide-angle modes (wmod.py)      -- the reference WMod goldens wtloss_1d{,.lossless}.nm,
+                                       the reference's 2N x 2N linearisation solved densely,
+                                       the constant-c / constant-Mach closed form, and the
+                                       zero-wind reduction to the ESS operator.
 """
