@@ -277,6 +277,33 @@ int ncpa_wmod_eigen(ncpa_ctx_t *ctx, const double *kd, const double *md, const d
                     double dz, double tol, double k_min, double k_max, int max_modes, int *nconv,
                     double *kH, double *v, int ldv);
 
+/* ---- dispersion files (host only: no device, no context) -------------------------- */
+
+/* The records ModeSolver::writeDispersion writes (src/libmodes/ModeSolver.cpp:569-589), one
+ * line per record: "%.7e   %d    %.7e   %.7e" f, n, rho(z_src), rho(z_rcv), then per mode
+ * "   %.12e   %.12e" Re k, Im k and "   %.12e   %.12e" psi(z_src), psi(z_rcv).  Record r's
+ * modes are entries [sum count[0..r), + count[r]) of the mode arrays.  Formatted on
+ * n_threads threads (<= 0: the host's cores) with the same bytes as printf, written with
+ * one fwrite; append != 0 opens the file "a", else "w". */
+int ncpa_dispersion_write(const char *path, int append, int n_records, const double *f, const int *count,
+                          const double *rho_s, const double *rho_r, const double *k_re,
+                          const double *k_im, const double *psi_s, const double *psi_r,
+                          int n_threads);
+
+/* Read a dispersion file as ModalBroadbandPropagator::read_dispersion_file does
+ * (src/libbroadband/ModalBroadbandPropagator.cpp:98-138): whitespace-separated numbers,
+ * records while a frequency reads.  The file is memory-mapped and its numbers parsed on
+ * n_threads threads.  NCPA_ERR_ARG: unreadable file, or a record cut short / a mode count
+ * that is not a whole number in [0, NCPA_MAX_MODES] (the reference's format error). */
+typedef struct ncpa_dispersion ncpa_dispersion_t;
+int ncpa_dispersion_read(const char *path, int n_threads, ncpa_dispersion_t **out);
+int ncpa_dispersion_info(const ncpa_dispersion_t *d, int *n_records, size_t *n_modes);
+/* Copies of the records (any pointer may be NULL): f, count, rho_s, rho_r [n_records];
+ * k_re, k_im, psi_s, psi_r [n_modes], file order. */
+int ncpa_dispersion_arrays(const ncpa_dispersion_t *d, double *f, int *count, double *rho_s,
+                           double *rho_r, double *k_re, double *k_im, double *psi_s, double *psi_r);
+void ncpa_dispersion_free(ncpa_dispersion_t *d);
+
 /* ---- broadband pulse synthesis (ModBB --propagation) ------------------------------ */
 
 /* Batched complex FFT of `batch` contiguous transforms of n points (interleaved re/im

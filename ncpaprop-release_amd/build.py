@@ -14,13 +14,13 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libncpamodes.so")
 SOURCES = ["csrc/ctx.cpp", "csrc/ess.hip", "csrc/wide.hip", "csrc/field2d.hip", "csrc/broadband.hip", "csrc/plan.cpp",
-           "host/atmosphere.cpp"]
+           "host/atmosphere.cpp", "host/dispersion.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-command-line-argument",
+    "-Wno-unused-command-line-argument", "-pthread",
 ]
 
 
@@ -49,7 +49,7 @@ def build(verbose=False, jobs=8):
         if verbose and text.strip():
             sys.stderr.write(text)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC"]
+    cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC", "-pthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

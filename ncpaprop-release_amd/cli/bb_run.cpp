@@ -41,37 +41,44 @@ struct Dispersion {
     double rho_s = 1.0, rho_r = 1.0;
 };
 
-// ModalBroadbandPropagator::read_dispersion_file (ModalBroadbandPropagator.cpp:98-138)
+// ModalBroadbandPropagator::read_dispersion_file (ModalBroadbandPropagator.cpp:98-138), the
+// parse done by ncpa_dispersion_read (memory-mapped, multi-threaded)
 Dispersion read_dispersion(const std::string &path) {
-    FILE *fp = std::fopen(path.c_str(), "r");
-    if (!fp) throw std::invalid_argument("file << " + path + " could not be opened.\n");
+    {
+        FILE *fp = std::fopen(path.c_str(), "r");
+        if (!fp) throw std::invalid_argument("file << " + path + " could not be opened.\n");
+        std::fclose(fp);
+    }
     std::printf("--> Opening dispersion file %s ...if the program chokes here the file may be corrupt.\n", path.c_str());
+    ncpa_dispersion_t *h = nullptr;
+    if (ncpa_dispersion_read(path.c_str(), 0, &h) != NCPA_OK) throw Fail(path + ": read_dispersion_data format error");
+    int nrec = 0;
+    size_t nm = 0;
+    ncpa_dispersion_info(h, &nrec, &nm);
+    std::vector<double> f(nrec), rs(nrec), rr(nrec), ps(nm), pr(nm);
+    std::vector<int> cnt(nrec);
     Dispersion d;
-    d.f.push_back(0.0);
+    d.kre.resize(nm);
+    d.kim.resize(nm);
+    ncpa_dispersion_arrays(h, f.data(), cnt.data(), rs.data(), rr.data(), d.kre.data(), d.kim.data(), ps.data(),
+                           pr.data());
+    ncpa_dispersion_free(h);
+    d.f.push_back(0.0);   // the DC bin (ModalBroadbandPropagator.cpp:114-115)
     d.count.push_back(0);
     d.begin.push_back(0);
-    double fr;
-    while (std::fscanf(fp, "%lf", &fr) == 1) {
-        int n = 0;
-        if (std::fscanf(fp, "%d %lf %lf", &n, &d.rho_s, &d.rho_r) != 3 || n < 0 || n > NCPA_MAX_MODES) {
-            std::fclose(fp);
-            throw Fail(path + ": read_dispersion_data format error");
-        }
-        d.f.push_back(fr);
-        d.begin.push_back((int)d.kre.size());
-        d.count.push_back(n);
-        for (int m = 0; m < n; ++m) {
-            double x1, x2, y1, y2;
-            if (std::fscanf(fp, "%lf %lf %lf %lf", &x1, &x2, &y1, &y2) != 4) {
-                std::fclose(fp);
-                throw Fail(path + ": read_dispersion_data format error");
-            }
-            d.kre.push_back(x1);
-            d.kim.push_back(x2);
-            d.psr.push_back(y1 * y2);
-        }
+    size_t o = 0;
+    for (int r = 0; r < nrec; ++r) {
+        d.f.push_back(f[r]);
+        d.begin.push_back((int)o);
+        d.count.push_back(cnt[r]);
+        o += (size_t)cnt[r];
     }
-    std::fclose(fp);
+    d.psr.resize(nm);
+    for (size_t j = 0; j < nm; ++j) d.psr[j] = ps[j] * pr[j];
+    if (nrec > 0) {   // rho of the last record read
+        d.rho_s = rs[nrec - 1];
+        d.rho_r = rr[nrec - 1];
+    }
     std::printf("--> Found modes at %d positive frequencies in file %s\n", (int)d.f.size(), path.c_str());
     if (d.f.size() < 3) throw Fail("the dispersion file needs at least two frequencies");
     return d;

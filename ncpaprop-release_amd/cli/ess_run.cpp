@@ -476,24 +476,27 @@ int ess_run(const EssRun &r) {
                 }
                 // the dispersion file and the atmosphere dump belong to the single-azimuth branch
                 // (ESSModeSolver.cpp:454-484)
-                if (!r.Nby2Dprop && !r.dispersion_file.empty() && (r.append_dispersion_file || p == last)) {
-                    // ModeSolver::writeDispersion (ModeSolver.cpp:569-589)
+                if (!r.Nby2Dprop && !r.dispersion_file.empty() && (r.append_dispersion_file || p == last))
                     std::printf("Writing to file %s: dispersion at freq = %8.3f Hz...\n", r.dispersion_file.c_str(), freq);
-                    FILE *f = open_or_throw(r.dispersion_file, r.append_dispersion_file ? "a" : "w");
-                    std::fprintf(f, "%.7e   %d    %.7e   %.7e", freq, nm, rho[n_zsrc], rho[n_zrcv]);
-                    for (int j = 0; j < nm; ++j) {
-                        std::fprintf(f, "   %.12e   %.12e", kre[o + j], kim[o + j]);
-                        std::fprintf(f, "   %.12e   %.12e", ps[o + j], pr[o + j]);
-                    }
-                    std::fprintf(f, "\n");
-                    std::fclose(f);
-                }
                 if (!r.Nby2Dprop && r.write_atmosphere && fi == 0) {
                     const std::string name = r.user_tag + "atm_profile" + ext;
                     std::cout << "Writing atmosphere to " << name << std::endl;
                     write_atmosphere(name, info, zn, c0n, wsn, wdn, Tn, Pn, rhon, azi);
                 }
             }
+        }
+        // ModeSolver::writeDispersion (ModeSolver.cpp:569-589) of every problem, in the reference's
+        // order and open modes: appended records, or ("w" per problem) the last one
+        if (!r.Nby2Dprop && !r.dispersion_file.empty()) {
+            const int r0 = r.append_dispersion_file ? 0 : last;
+            const int nrec = P - r0;
+            std::vector<double> fr(nrec), rs(nrec, rho[n_zsrc]), rr(nrec, rho[n_zrcv]);
+            for (int q = 0; q < nrec; ++q) fr[q] = r.freqs[(r0 + q) / naz];
+            const size_t o0 = off[r0];
+            if (ncpa_dispersion_write(r.dispersion_file.c_str(), r.append_dispersion_file ? 1 : 0, nrec, fr.data(),
+                                      counts.data() + r0, rs.data(), rr.data(), kre.data() + o0, kim.data() + o0,
+                                      ps.data() + o0, pr.data() + o0, 0) != NCPA_OK)
+                throw Fail("cannot write " + r.dispersion_file);
         }
     } catch (const std::exception &e) {
         std::cerr << e.what() << std::endl;

@@ -44,6 +44,8 @@ EXPORTS = [
     "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d",
     "ncpa_field2d_stage", "ncpa_field2d_contract", "ncpa_field_stage_free", "ncpa_field_compare", "ncpa_ess_eigen",
     "ncpa_wmod_solve", "ncpa_wmod_eigen",
+    "ncpa_dispersion_write", "ncpa_dispersion_read", "ncpa_dispersion_info", "ncpa_dispersion_arrays",
+    "ncpa_dispersion_free",
     "ncpa_fft_batch", "ncpa_kpert", "ncpa_bb_transfer", "ncpa_bb_pulse", "ncpa_bb_source",
     "ncpa_bb_plan_create", "ncpa_bb_plan_set_problems", "ncpa_bb_plan_run", "ncpa_bb_plan_result",
     "ncpa_bb_plan_tuning", "ncpa_bb_plan_destroy",
@@ -138,6 +140,12 @@ def load():
         "ncpa_ess_eigen": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int,
                                      ip, dp, dp, C.c_int]),
         "ncpa_wmod_solve": (C.c_int, [vp, P(Grid), P(Problem), C.c_int, P(vp)]),
+        "ncpa_dispersion_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, ip, dp, dp, dp, dp, dp, dp,
+                                            C.c_int]),
+        "ncpa_dispersion_read": (C.c_int, [C.c_char_p, C.c_int, P(vp)]),
+        "ncpa_dispersion_info": (C.c_int, [vp, ip, P(C.c_size_t)]),
+        "ncpa_dispersion_arrays": (C.c_int, [vp, dp, ip, dp, dp, dp, dp, dp, dp]),
+        "ncpa_dispersion_free": (None, [vp]),
         "ncpa_wmod_eigen": (C.c_int, [vp, dp, dp, dp, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_int, ip, dp, dp, C.c_int]),
         "ncpa_fft_batch": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int]),
