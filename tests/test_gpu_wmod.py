@@ -166,3 +166,60 @@ def test_batch_equals_single(gpu_ctx, synth_profile):
         ks = ms.problem_modes(0)[0]
         assert kb.size == ks.size
         assert np.array_equal(kb, ks)
+
+
+@pytest.mark.parametrize("opts", [
+    {"wkb": True},                                   # ground-to-ground WKB cut-off (dz = H/Nz in the integral)
+    {"wvnum_filter": (300.0, 360.0)},                # k_min = w/c_max, k_max = w/c_min
+    {"lamb": True},                                  # Lamb-wave ground admittance
+])
+def test_options_vs_oracle(gpu_ctx, synth_profile, opts):
+    """Interval options of WModeSolver::solve (WModeSolver.cpp:340-360, 572-620): the same modes
+    as the oracle with the WKB cut-off, the wavenumber filter and the Lamb-wave boundary term."""
+    import ncpamodes
+    from oracle import atmos, ess, wmod
+    f, az, Nz = 0.4, 120.0, 5000
+    print("case options %s" % opts, flush=True)
+    pid, info = gpu_ctx.read_profile(synth_profile)
+    g = gpu_ctx.grid(pid, Nz=Nz)
+    nodes = atmos.AtmosphereNodes(atmos.read_profile(synth_profile))
+    grid = wmod.WGrid(nodes, Nz, 150.0, 0.0)
+    adm = ess.admittance(grid, True) if opts.get("lamb") else 0.0
+    if opts.get("lamb"):
+        assert abs(gpu_ctx.admittance(pid, g.z_min) - adm) <= 1e-12 * abs(adm)
+    m = gpu_ctx.solve_wide(g, [ncpamodes.problem(pid, f, az, wkb=opts.get("wkb", False), admittance=adm,
+                                                 wvnum_filter=opts.get("wvnum_filter"))])
+    k, psr, ps, pr = m.problem_modes(0)
+    sol = wmod.solve(nodes=nodes, grid=grid, freq=f, azimuth=az, turnoff_wkb=not opts.get("wkb", False),
+                     lamb_wave_bc=bool(opts.get("lamb")), wvnum_filter=opts.get("wvnum_filter"))
+    kmin, kmax = m.interval()
+    assert kmin[0] == pytest.approx(sol.k_min, rel=1e-13) and kmax[0] == pytest.approx(sol.k_max, rel=1e-12)
+    if opts.get("wkb"):
+        assert sol.used_wkb and sol.k_max < 2 * np.pi * f / sol.ceffz.min()
+    assert k.size == sol.k.size > 5
+    assert np.max(np.abs(np.sqrt((k ** 2).real) / sol.k - 1)) < 1e-10
+    ref = sol.psi[sol.i_s] * sol.psi[sol.i_r]
+    assert np.max(np.abs(psr - ref)) <= 1e-7 * np.max(np.abs(ref))
+
+
+def test_multiprop_cli_vs_oracle(tmp_path, synth_profile):
+    """WMod --multiprop: Nby2D_tloss_1d.wnm blocks per azimuth against the oracle's TL."""
+    import subprocess
+    from oracle import ess, wmod
+    f, Nz = 0.2, 4000
+    print("case WMod multiprop", flush=True)
+    exe = os.path.join(os.path.dirname(G), "..", "ncpaprop-release_amd", "bin", "WMod")
+    out = subprocess.run([exe, "--atmosfile", synth_profile, "--freq", str(f), "--multiprop", "--azimuth_start", "0",
+                          "--azimuth_end", "180", "--azimuth_step", "90", "--Nz_grid", str(Nz), "--maxrange_km", "300",
+                          "--Nrng_steps", "300", "--turnoff_WKB"], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    blocks = [b for b in (tmp_path / "Nby2D_tloss_1d.wnm").read_text().split("\n\n") if b.strip()]
+    assert len(blocks) == 3
+    for az, blk in zip((0.0, 90.0, 180.0), blocks):
+        a = np.array([l.split() for l in blk.strip().splitlines()], dtype=float)
+        assert a.shape == (300, 5) and np.all(a[:, 1] == az)
+        sol = wmod.solve(synth_profile, freq=f, azimuth=az, Nz=Nz, turnoff_wkb=True)
+        _, po, _ = ess.tl1d(sol.k_pert, sol.psi[sol.i_s], sol.psi[sol.i_r], 1000.0, 300)
+        tl, tlo = ess.tl_db(a[:, 2] + 1j * a[:, 3]), ess.tl_db(po)
+        assert np.max(np.abs(tl - tlo)[tlo > -140]) <= 0.01
