@@ -33,7 +33,7 @@ struct IntervalOut {
     double k_min, k_max, lo, hi;
     int n_lo, n_hi;
     int used_wkb;
-    int pad;
+    int not_monotone;   // wide-angle: dT/dk is not negative definite on [k_min, k_max]
 };
 
 // Per problem of the eigenvalue kernels.  a: the rows of dz^2 T (unit off-diagonal); lo,

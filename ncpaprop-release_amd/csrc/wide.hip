@@ -159,6 +159,15 @@ __global__ void __launch_bounds__(256) k_winterval(const ProbDev *__restrict__ p
     interval_bounds(pr, Nz, dz_m, CE + size_t(p) * Nz, shd, shi, false, k_min, k_max, used_wkb);
     if (threadIdx.x == 0) { s_k[0] = k_min; s_k[1] = k_max; }
     __syncthreads();
+    // the counts are exact only while every branch decreases: dz^2 dT/dk = diag(B + 2kC) < 0
+    // on every row, checked at both ends of the interval (it is linear in k)
+    double worst = -DBL_MAX;
+    {
+        const double *Bp = BC + size_t(p) * Nz, *Cp = BC + plane + size_t(p) * Nz;
+        for (int i = threadIdx.x; i < Nz; i += blockDim.x)
+            worst = fmax(worst, fmax(fma(2.0 * k_min, Cp[i], Bp[i]), fma(2.0 * k_max, Cp[i], Bp[i])));
+    }
+    worst = -block_min(-worst, shd);
     if (threadIdx.x < 64) {   // the whole first wave sweeps (lanes 0 and 1 keep their counts)
         const size_t o = size_t(p) * Nz;
         const WideSweeps ws{A + o, BC + o, BC + plane + o, nullptr, Nz};
@@ -172,6 +181,7 @@ __global__ void __launch_bounds__(256) k_winterval(const ProbDev *__restrict__ p
         out[p].lo = k_min;   // shifts of the wide-angle search are wavenumbers
         out[p].hi = k_max;
         out[p].used_wkb = used_wkb;
+        out[p].not_monotone = (worst >= 0.0 && k_max > k_min) ? 1 : 0;
     }
 }
 
@@ -298,6 +308,12 @@ int ncpa_wmod_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem
     e = hipMemcpyAsync(m->iv.data(), d_iv, sizeof(IntervalOut) * P, hipMemcpyDeviceToHost, ctx->stream);
     if (e != hipSuccess) return fail(hip_error(ctx, e, "copy intervals"));
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(hip_error(ctx, e, "interval sync"));
+    for (int p = 0; p < P; ++p)
+        if (m->iv[p].not_monotone)
+            return fail(set_error(ctx, NCPA_ERR_ARG,
+                                  "ncpa_wmod_solve: problem " + std::to_string(p) +
+                                      ": the wind is too strong for the wide-angle counts (omega w + k (c0^2 - w^2) "
+                                      "not positive on every row for k in [k_min, k_max])"));
     rc = launch_modes(ctx, m, srow, rrow);
     ctx->pool.put(d_nodes);
     ctx->pool.put(d_iv);
@@ -366,6 +382,10 @@ int ncpa_wmod_eigen(ncpa_ctx_t *ctx, const double *kd, const double *md, const d
         return done(set_error(ctx, NCPA_ERR_MAX_MODES,
                               "ncpa_wmod_eigen: " + std::to_string(nm) + " eigenpairs exceed max_modes"));
     if (ldv < nm) return done(set_error(ctx, NCPA_ERR_ARG, "ncpa_wmod_eigen: ldv smaller than the number of modes"));
+    for (int i = 0; i < Nz; ++i)   // dT/dk = diag(cd + 2 k md) must be negative on the interval
+        if (!(cd[i] + 2.0 * k_min * md[i] < 0.0 && cd[i] + 2.0 * k_max * md[i] < 0.0))
+            return done(set_error(ctx, NCPA_ERR_ARG, "ncpa_wmod_eigen: cd + 2 k md is not negative on row " +
+                                                         std::to_string(i) + " for k in [k_min, k_max]"));
     m->iv.resize(1);
     m->iv[0] = IntervalOut{k_min, k_max, k_min, k_max, cnt[0], cnt[1], 0, 0};
     int rc = launch_modes(ctx, m, std::vector<int>{0}, std::vector<int>{0});

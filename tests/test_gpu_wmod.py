@@ -223,3 +223,17 @@ def test_multiprop_cli_vs_oracle(tmp_path, synth_profile):
         _, po, _ = ess.tl1d(sol.k_pert, sol.psi[sol.i_s], sol.psi[sol.i_r], 1000.0, 300)
         tl, tlo = ess.tl_db(a[:, 2] + 1j * a[:, 3]), ess.tl_db(po)
         assert np.max(np.abs(tl - tlo)[tlo > -140]) <= 0.01
+
+
+def test_supersonic_wind_rejected(gpu_ctx):
+    """The counts need dT/dk < 0 on [k_min, k_max] (every branch decreasing): a pencil whose
+    cd + 2 k md changes sign is refused with an argument error, nothing is computed."""
+    import ncpamodes
+    print("case supersonic", flush=True)
+    N, dz, f, c = 500, 50.0, 0.5, 340.0
+    om = 2 * math.pi * f
+    kd = np.full(N, (om / c) ** 2)
+    kd[0] += 1.0 / dz ** 2
+    M = np.where(np.arange(N) > 300, -1.2, 0.2)     # a supersonic head-wind layer
+    with pytest.raises(ncpamodes.NcpaError):
+        ncpamodes.wmod_eigen(gpu_ctx, kd, M * M - 1.0, -2 * om * M / c, dz, 0.5 * om / c, 1.5 * om / c)
