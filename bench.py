@@ -698,7 +698,7 @@ def wmod_main(args, cpu):
                            "wide_over_ess_time": elapsed / ess_elapsed},
         "roofline": {"kernel": "k_wmodes<2>", "bound": "mfma", "achieved": achieved,
                      "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"],
-                     "traffic": None,
+                     "traffic": measured_traffic("k_wmodes<2>"),
                      "algorithmic": "6 FP64 flops per row per sweep (the row value a_i(k) = A + k(B + kC) "
                                     "and the continuant; Newton sweeps do 12, counted as 6); sweeps = "
                                     "search sweeps + 2 per eigenvalue; FP64 VALU, MI355X FP64 peak"},
