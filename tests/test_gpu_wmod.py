@@ -237,3 +237,35 @@ def test_supersonic_wind_rejected(gpu_ctx):
     M = np.where(np.arange(N) > 300, -1.2, 0.2)     # a supersonic head-wind layer
     with pytest.raises(ncpamodes.NcpaError):
         ncpamodes.wmod_eigen(gpu_ctx, kd, M * M - 1.0, -2 * om * M / c, dz, 0.5 * om / c, 1.5 * om / c)
+
+
+def test_full_size_residuals(gpu_ctx, synth_profile):
+    """The benchmark's size (1 Hz, Nz = 20000, ~500 modes): every mode satisfies
+    (D + k C + k^2 M) psi = 0 to 1e-9 relative (operator assembled on the host from the
+    oracle's trace, EigenEngine.cpp:257-302) and the reference's normalisation
+    sum psi^2 dz (1 + k^2) = 1 to 1e-9; eigenvalues strictly ascending inside [k_min, k_max]."""
+    import ncpamodes
+    from oracle import atmos, wmod
+    f, az, Nz = 1.0, 45.0, 20000
+    print("case full size f=%g az=%g" % (f, az), flush=True)
+    pid, info = gpu_ctx.read_profile(synth_profile)
+    g = gpu_ctx.grid(pid, Nz=Nz)
+    m = gpu_ctx.solve_wide(g, [ncpamodes.problem(pid, f, az)])
+    k2 = (m.problem_modes(0)[0] ** 2).real
+    k = np.sqrt(k2)
+    psi = m.rows(0, 1, Nz)                      # [Nz, modes]
+    grid = wmod.WGrid(atmos.AtmosphereNodes(atmos.read_profile(synth_profile)), Nz, 150.0, 0.0)
+    kd, md, cd, _, _, kmin, kmax, _ = wmod.modal_trace(grid, f, az, 0.0, 0.0, True)
+    dz = grid.dz_m
+    assert k.size > 300 and np.all(np.diff(k) > 0) and k[0] >= kmin and k[-1] <= kmax
+    assert np.allclose((psi * psi).sum(axis=0) * dz * (1 + k2), 1.0, rtol=1e-9, atol=0)
+    # residual of the quadratic problem, rows scaled by dz^2: (A + k B + k^2 C) psi + off-diagonals
+    A, B, C = -2.0 + dz * dz * kd, dz * dz * cd, dz * dz * md
+    diag = A[:, None] + k[None, :] * (B[:, None] + k[None, :] * C[:, None])
+    r = diag * psi
+    r[:-1] += psi[1:]
+    r[1:] += psi[:-1]
+    scale = 4.0 * np.abs(psi).max(axis=0)
+    res = float(np.max(np.abs(r).max(axis=0) / scale))
+    print("modes %d, max relative residual %.2e" % (k.size, res), flush=True)
+    assert res < 1e-9
