@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) k_winterval(const ProbDev *__restrict__ p
 }
 
 // counts at two wavenumbers (lane 0: lo, lane 1: hi) for one pencil
-__global__ void k_wcounts2(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ C,
+__global__ void __launch_bounds__(64) k_wcounts2(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ C,
                            int n, double lo, double hi, int *__restrict__ out) {
     __shared__ double s_ring[3 * NCPA_RING];
     const WideSweeps ws{A, B, C, nullptr, n};
