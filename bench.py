@@ -847,7 +847,7 @@ def serving_main(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["field", "broadband", "serving", "precision", "wmod"], default="field")
     ap.add_argument("--no-cpu", action="store_true")
