@@ -269,3 +269,22 @@ def test_full_size_residuals(gpu_ctx, synth_profile):
     res = float(np.max(np.abs(r).max(axis=0) / scale))
     print("modes %d, max relative residual %.2e" % (k.size, res), flush=True)
     assert res < 1e-9
+
+
+@pytest.mark.parametrize("f,Nz,canon", [(0.02, 20000, True), (1.5, 6000, False)])
+def test_frequency_extremes_vs_oracle(gpu_ctx, synth_profile, f, Nz, canon):
+    """Low frequency (0.02 Hz, few modes, k ~ 4e-4) on the canonical profile and 1.5 Hz on the jet
+    profile: counts, k to 1e-10, psi products to 1e-7 of max."""
+    import ncpamodes
+    from oracle import wmod
+    path = CANON if canon else synth_profile
+    print("case extremes f=%g Nz=%d" % (f, Nz), flush=True)
+    pid, info = gpu_ctx.read_profile(path)
+    g = gpu_ctx.grid(pid, Nz=Nz)
+    m = gpu_ctx.solve_wide(g, [ncpamodes.problem(pid, f, 90.0)])
+    k, psr, ps, pr = m.problem_modes(0)
+    sol = wmod.solve(path, freq=f, azimuth=90.0, Nz=Nz, turnoff_wkb=True)
+    assert k.size == sol.k.size > 2
+    assert np.max(np.abs(np.sqrt((k ** 2).real) / sol.k - 1)) < 1e-10
+    ref = sol.psi[sol.i_s] * sol.psi[sol.i_r]
+    assert np.max(np.abs(psr - ref)) <= 1e-7 * np.max(np.abs(ref))
