@@ -25,8 +25,10 @@ reference's rule) with a Gaussian-windowed 0.5 Hz source given as a spectrum.
 
 --workload wmod: the field workload's 64-problem batch through the wide-angle (WMod) solver
 (eigenpairs + attenuation + 1-D TL, FP64), the effective-sound-speed batch timed beside it.
+--workload broadband-wmod: the broadband job with wide-angle dispersion (ModBB --method wmod).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload field|broadband|serving|precision|wmod]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload field|broadband|serving|precision|wmod|broadband-wmod]
                     [--schedule pipeline|streams] [--streams S] [--no-cpu]
 
 Prints one JSON line.  cpu_baseline: the FP64 oracle (scipy LAPACK bisection + inverse
@@ -719,6 +721,79 @@ def wmod_main(args, cpu):
     return line
 
 
+def broadband_wmod_main(args):
+    """The broadband job's dispersion set (1024 frequencies x 8 azimuths, Nz = 20000) through the
+    wide-angle solver (ModBB --dispersion --method wmod), eagerly (the graph plan is built for
+    the effective-sound-speed kernels), followed by the same transfer functions and pulse
+    synthesis.  One step = one job."""
+    import numpy as np
+    import ncpamodes
+    from ncpamodes import broadband as bb
+    tmp = tempfile.mkdtemp(prefix="ncpa_bench_")
+    path = write_profiles(tmp, [0])[0]
+    ctx = ncpamodes.Context(0)
+    pid = ctx.read_profile(path)[0]
+    grid = ctx.grid(pid, Nz=NZ)
+    f = np.arange(BB_NF + 1) * BB_FSTEP
+    problems = [ncpamodes.problem(pid, f[i], az) for az in AZIMUTHS for i in range(1, BB_NF + 1)]
+    ranges = np.array(BB_RANGES_KM) * 1e3
+    nfft = bb.default_nfft(4096, f.size)
+    S, _ = bb.source(ctx, "spectrum", f, nfft, -1.0, np.stack([f, bb_source_spectrum(f).real,
+                                                                bb_source_spectrum(f).imag], axis=1))
+    nset = len(AZIMUTHS)
+
+    def job(timing=False):
+        m = ctx.solve_wide(grid, problems)
+        W, kb = bb.synthesize_from_modes(ctx, m, nset, f, ranges, np.ones(nset), S, BB_MAXCEL, nfft)
+        return m, W, kb
+
+    def release(res):
+        m, W, kb = res
+        for b in (W,) + kb:
+            b.free()
+        m.free()
+
+    release(job())
+    ctx.synchronize()
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    res = job()
+    ctx.synchronize()
+    total_modes = res[0].total
+    its = np.abs(res[0].iterations()).astype(np.float64)
+    sweeps = float((its + 2.0).sum())
+    fams = {n: ctx.kernel_time(n) for n in ("setup", "interval", "modes", "bb_transfer", "fft")}
+    ctx.set_timing(False)
+    release(res)
+    for _ in range(args.warmup):
+        release(job())
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        release(job())
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.close()
+    P = len(problems)
+    modes_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
+    achieved = FLOPS_PER_WIDE_STEP * sweeps * NZ / modes_s / 1e12
+    return {
+        "metric": "wide-angle (WMod) dispersion solves/sec, broadband job, FP64",
+        "value": P * args.steps / elapsed, "unit": "dispersion solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "ModBB --method wmod job: wide-angle dispersion at 1024 frequencies (2/1024 - 2 Hz) "
+                               "x 8 azimuths, Nz=20000, + transfer functions at 16 ranges + pulse synthesis NFFT "
+                               "%d; eager (one batched solve per job)" % nfft,
+                   "global_batch": P, "modes_per_step": int(total_modes), "parallelism": "single GPU"},
+        "stage_ms": {k: v[0] for k, v in fams.items()},
+        "roofline": {"kernel": "k_wmodes<2>", "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f64"],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"], "traffic": None,
+                     "algorithmic": "6 FP64 flops per row per sweep, as the wmod workload"},
+        "cpu_baseline": None,
+    }
+
+
 def precision_main(args):
     """Config 3: the 2-D field at 2 Hz on 1500 depths x 10000 ranges, every contraction precision
     against the FP64 contraction of the same modes: time per launch (HIP events), speed-up over
@@ -849,7 +924,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["field", "broadband", "serving", "precision", "wmod"], default="field")
+    ap.add_argument("--workload", choices=["field", "broadband", "serving", "precision", "wmod", "broadband-wmod"],
+                    default="field")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", choices=["pipeline", "streams"], default="pipeline",
                     help="field workload headline: two-stage pipeline over two streams, or "
@@ -873,6 +949,9 @@ def main():
         return 0
     if args.workload == "serving":
         print(json.dumps(serving_main(args)), flush=True)
+        return 0
+    if args.workload == "broadband-wmod":
+        print(json.dumps(broadband_wmod_main(args)), flush=True)
         return 0
     cpu = None if args.no_cpu else cpu_baseline(args.workload, cpu_cores())
     if args.workload == "field":

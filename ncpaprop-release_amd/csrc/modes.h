@@ -120,15 +120,15 @@ __device__ inline void interval_bounds(const ProbDev &pr, int Nz, double dz, con
 // is non-decreasing in the shift: `pen` supplies count(ring, s), newton(ring, s) (count,
 // det and d det/ds) and finish(...) (phase 3 and the outputs).  EssPencil is dz^2 T - s;
 // the wide-angle pencil (wide.h) is dz^2 T(k) with the shift k inside the diagonal.
-template <int NR>
+template <int RING>
 struct ModesSharedN {
     double ms_s[64];
     int ms_c[64];
     double hb_lo[64], hb_hi[64];   // brackets of the unconverged eigenvalues
     int h_list[64];                // their lanes, in lane order
-    double ring[NR * NCPA_RING];   // NR staged arrays
+    double ring[RING];             // the staged arrays (RING doubles)
 };
-using ModesShared = ModesSharedN<2>;
+using ModesShared = ModesSharedN<2 * NCPA_RING>;
 
 template <int MS, class Pen, class Sh>
 __device__ __forceinline__ void modes_item(const ModeProb &P, const Pen &pen, const ModeWork wk, Sh &sh_,

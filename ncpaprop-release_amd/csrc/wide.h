@@ -23,17 +23,58 @@
 // derivative is q^mu (1 + k^2).
 //
 // Three arrays (four with the absorption weights) are staged per sweep through LDS rings
-// of two 256-row tiles each, as in sturm.h; the rows are read back 16 at a time.
+// of two 128-row tiles each (the pattern of sturm.h at half its tile): 8 KB of rings and
+// 8-row reads keep a block at ~10 KB of LDS and a wave under 128 VGPRs, so large batches run
+// 4 waves per SIMD.
 #pragma once
 #include "sturm.h"
 
 namespace ncpa {
 
+// Two staging geometries: tiles of T = 256 rows read back 16 rows at a time (fewest
+// syncs per row: small batches, where each wave's own instruction stream sets the time), and
+// T = 128 rows read 8 at a time (8 KB of rings, under 128 VGPRs: large batches run 4 waves per
+// SIMD).  A ring holds two tiles of one array: 2 T doubles.
+#define NCPA_WTILE_BIG 256
+#define NCPA_WTILE_SMALL 128
+
+template <int T>
+struct WTileStage {
+    const double *g;
+    double *ring;
+    int n;
+    int lane;
+    double pre[T / 64];
+    __device__ __forceinline__ void load(int c) {
+#pragma unroll
+        for (int u = 0; u < T / 64; ++u) {
+            const int r = c * T + lane + 64 * u;
+            pre[u] = (c >= 0 && r < n) ? g[r] : 0.0;
+        }
+    }
+    __device__ __forceinline__ void store(int c) {
+        double *s = ring + (c & 1) * T;
+#pragma unroll
+        for (int u = 0; u < T / 64; ++u) s[lane + 64 * u] = pre[u];
+    }
+};
+
+// G rows of NA staged arrays (rings of 2 T doubles) starting at ring offset off
+template <int T, int NA, int G>
+__device__ __forceinline__ void lds_rows(double (&v)[NA][G], const double *ring, int off) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const double *s = ring + a * 2 * T + off;
+#pragma unroll
+        for (int u = 0; u < G; ++u) v[a][u] = s[u];
+    }
+}
+
 // Rows n-1 .. 0 of NA arrays staged together: step(v, i) with v[a] = arr[a][i], then
 // chunk_end(lo, m) after each chunk of rows [lo, lo + m) (first the n % 16 top rows, then
 // 16-row chunks downwards).  ring holds NA rings of NCPA_RING doubles.  Full-wave entry.
 template <int NA, class Step, class ChunkEnd>
-__device__ __forceinline__ void sweep_down_n(const double *const (&arr)[NA], double *ring, int n, Step step,
+__device__ __forceinline__ void sweep_down_big(const double *const (&arr)[NA], double *ring, int n, Step step,
                                              ChunkEnd chunk_end) {
     if (n <= 0) return;
     const int lane = (int)(threadIdx.x & 63);
@@ -84,7 +125,7 @@ __device__ __forceinline__ void sweep_down_n(const double *const (&arr)[NA], dou
 // Rows 0 .. m-1 (ascending) of NA arrays, chunk_end(lo, m') after every chunk of 16 rows and
 // after the last m % 16 rows.  Full-wave entry.
 template <int NA, class Step, class ChunkEnd>
-__device__ __forceinline__ void sweep_up_n(const double *const (&arr)[NA], double *ring, int m, Step step,
+__device__ __forceinline__ void sweep_up_big(const double *const (&arr)[NA], double *ring, int m, Step step,
                                            ChunkEnd chunk_end) {
     if (m <= 0) return;
     const int lane = (int)(threadIdx.x & 63);
@@ -131,10 +172,133 @@ __device__ __forceinline__ void sweep_up_n(const double *const (&arr)[NA], doubl
     }
 }
 
+// Rows n-1 .. 0 of NA arrays staged together: step(v, i) with v[a] = arr[a][i], then
+// chunk_end(lo, m) after each chunk of rows [lo, lo + m) (first the n % 16 top rows, then
+// 16-row chunks downwards).  ring holds NA rings of 2 T doubles.  Full-wave entry.
+template <int T, int NA, class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_down_compact(const double *const (&arr)[NA], double *ring, int n, Step step,
+                                             ChunkEnd chunk_end) {
+    constexpr int G = T >= 256 ? 16 : 8;   // rows per LDS read group
+    if (n <= 0) return;
+    const int lane = (int)(threadIdx.x & 63);
+    WTileStage<T> st[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) st[a] = WTileStage<T>{arr[a], ring + a * 2 * T, n, lane};
+    const int ntile = (n + T - 1) / T;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) { st[a].load(ntile - 1); st[a].store(ntile - 1); st[a].load(ntile - 2); }
+    wave_lds_sync();
+    const int rem = n % NCPA_CHUNK;
+    for (int c = ntile - 1; c >= 0; --c) {
+        const int off = (c & 1) * T;
+        const int base = c * T;
+        int top = min(n, base + T);
+        if (c == ntile - 1 && rem) {
+            for (int i = n - 1; i >= n - rem; --i) {
+                double v[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) v[a] = ring[a * 2 * T + off + i - base];
+                step(v, i);
+            }
+            chunk_end(n - rem, rem);
+            top = n - rem;
+        }
+        for (int cb = top - NCPA_CHUNK; cb >= base; cb -= NCPA_CHUNK) {
+#pragma unroll
+            for (int h = NCPA_CHUNK / G - 1; h >= 0; --h) {   // the chunk's read groups, top first
+                double va[NA][G];
+                lds_rows<T, NA, G>(va, ring, off + (cb + G * h - base));
+#pragma unroll
+                for (int u = G - 1; u >= 0; --u) {
+                    double v[NA];
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) v[a] = va[a][u];
+                    step(v, cb + G * h + u);
+                }
+            }
+            chunk_end(cb, NCPA_CHUNK);
+        }
+        if (c > 0) {
+            wave_lds_sync();
+#pragma unroll
+            for (int a = 0; a < NA; ++a) { st[a].store(c - 1); st[a].load(c - 2); }
+            wave_lds_sync();
+        }
+    }
+}
+
+// Rows 0 .. m-1 (ascending) of NA arrays, chunk_end(lo, m') after every chunk of 16 rows and
+// after the last m % 16 rows.  Full-wave entry.
+template <int T, int NA, class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_up_compact(const double *const (&arr)[NA], double *ring, int m, Step step,
+                                           ChunkEnd chunk_end) {
+    constexpr int G = T >= 256 ? 16 : 8;
+    if (m <= 0) return;
+    const int lane = (int)(threadIdx.x & 63);
+    WTileStage<T> st[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) st[a] = WTileStage<T>{arr[a], ring + a * 2 * T, m, lane};
+    const int ntile = (m + T - 1) / T;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) { st[a].load(0); st[a].store(0); st[a].load(1 < ntile ? 1 : -1); }
+    wave_lds_sync();
+    for (int c = 0; c < ntile; ++c) {
+        const int off = (c & 1) * T;
+        const int base = c * T;
+        const int endt = min(m, base + T);
+        for (int cb = base; cb < endt; cb += NCPA_CHUNK) {
+            const int len = min(NCPA_CHUNK, endt - cb);
+            if (len == NCPA_CHUNK) {
+#pragma unroll
+                for (int h = 0; h < NCPA_CHUNK / G; ++h) {
+                    double va[NA][G];
+                    lds_rows<T, NA, G>(va, ring, off + (cb + G * h - base));
+#pragma unroll
+                    for (int u = 0; u < G; ++u) {
+                        double v[NA];
+#pragma unroll
+                        for (int a = 0; a < NA; ++a) v[a] = va[a][u];
+                        step(v, cb + G * h + u);
+                    }
+                }
+            } else {
+                for (int i = cb; i < cb + len; ++i) {
+                    double v[NA];
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) v[a] = ring[a * 2 * T + off + i - base];
+                    step(v, i);
+                }
+            }
+            chunk_end(cb, len);
+        }
+        if (c + 1 < ntile) {
+            wave_lds_sync();
+#pragma unroll
+            for (int a = 0; a < NA; ++a) { st[a].store(c + 1); st[a].load(c + 2 < ntile ? c + 2 : -1); }
+            wave_lds_sync();
+        }
+    }
+}
+
+// The sweeps of a WideSweeps<T>: T = 256 the 16-row reads of sturm.h's tiles, else compact.
+template <int T, int NA, class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_down_n(const double *const (&arr)[NA], double *ring, int n, Step step,
+                                             ChunkEnd chunk_end) {
+    if constexpr (T == NCPA_TILE) sweep_down_big<NA>(arr, ring, n, step, chunk_end);
+    else sweep_down_compact<T, NA>(arr, ring, n, step, chunk_end);
+}
+template <int T, int NA, class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_up_n(const double *const (&arr)[NA], double *ring, int m, Step step,
+                                           ChunkEnd chunk_end) {
+    if constexpr (T == NCPA_TILE) sweep_up_big<NA>(arr, ring, m, step, chunk_end);
+    else sweep_up_compact<T, NA>(arr, ring, m, step, chunk_end);
+}
+
 __device__ __forceinline__ double wide_row(const double (&v)[3], double k) { return fma(fma(v[2], k, v[1]), k, v[0]); }
 
 // The wide-angle pencil of one problem: rows A, B, C of dz^2 T(k) and absorption weights W.
-// ring: 4 staged arrays (4 NCPA_RING doubles).
+// ring: 4 staged arrays (4 x 2 T doubles).
+template <int T>
 struct WideSweeps {
     const double *A, *B, *C, *W;
     int n;
@@ -145,7 +309,7 @@ struct WideSweeps {
         int cnt = 0;
         unsigned bits = 0u;
         const double *const arr[3] = {A, B, C};
-        sweep_down_n<3>(
+        sweep_down_n<T, 3>(
             arr, ring, n,
             [&](const double (&v)[3], int) {
                 const double p0 = fma(wide_row(v, k), p1, -p2);
@@ -170,7 +334,7 @@ struct WideSweeps {
         unsigned bits = 0u;
         const double k2 = k + k;
         const double *const arr[3] = {A, B, C};
-        sweep_down_n<3>(
+        sweep_down_n<T, 3>(
             arr, ring, n,
             [&](const double (&v)[3], int) {
                 const double am = wide_row(v, k);
@@ -208,7 +372,7 @@ struct WideSweeps {
         o.phi_s1 = SVal{1.0, 0};
         o.phi_r1 = SVal{1.0, 0};
         const double *const arr[4] = {A, B, C, W};
-        sweep_down_n<4>(
+        sweep_down_n<T, 4>(
             arr, ring, n,
             [&](const double (&v)[4], int i) {
                 const double am = fma(fma(v[2], k, v[1]), k, v[0]);
@@ -249,7 +413,7 @@ struct WideSweeps {
         double t1 = 1.0, t2 = 0.0;
         int ex = 0;
         const double *const arr[3] = {A, B, C};
-        sweep_up_n<3>(
+        sweep_up_n<T, 3>(
             arr, ring, top,
             [&](const double (&v)[3], int i) {
                 const double t0 = fma(wide_row(v, k), t1, -t2);
@@ -270,12 +434,12 @@ struct WideSweeps {
     template <class Step, class ChunkEnd>
     __device__ __forceinline__ void up(double *ring, int m, double k, Step step, ChunkEnd ce) const {
         const double *const arr[3] = {A, B, C};
-        sweep_up_n<3>(arr, ring, m, [&](const double (&v)[3], int i) { step(wide_row(v, k), i); }, ce);
+        sweep_up_n<T, 3>(arr, ring, m, [&](const double (&v)[3], int i) { step(wide_row(v, k), i); }, ce);
     }
     template <class Step, class ChunkEnd>
     __device__ __forceinline__ void down(double *ring, int m, double k, Step step, ChunkEnd ce) const {
         const double *const arr[3] = {A, B, C};
-        sweep_down_n<3>(arr, ring, m, [&](const double (&v)[3], int i) { step(wide_row(v, k), i); }, ce);
+        sweep_down_n<T, 3>(arr, ring, m, [&](const double (&v)[3], int i) { step(wide_row(v, k), i); }, ce);
     }
 };
 

@@ -91,9 +91,10 @@ __global__ void k_wsetup_grid(const ProbDev *__restrict__ probs, int Nz, double 
 }
 
 // The pencil of problem p: A and W rows from the ModeProb, B and C from the BC planes.
+template <int T>
 struct WidePencil {
     static constexpr bool early_accept = false;
-    WideSweeps s;
+    WideSweeps<T> s;
 
     __device__ __forceinline__ int count(double *ring, double k) const { return s.count(ring, k); }
     __device__ __forceinline__ NewtonOut newton(double *ring, double k) const { return s.newton(ring, k); }
@@ -138,10 +139,13 @@ struct WidePencil {
     }
 };
 
-__device__ __forceinline__ WideSweeps wide_sweeps(const ModeProb &P, int prob, const double *BC, size_t plane) {
+template <int T>
+__device__ __forceinline__ WideSweeps<T> wide_sweeps(const ModeProb &P, int prob, const double *BC, size_t plane) {
     const size_t o = (size_t)prob * P.n;
-    return WideSweeps{P.a, BC + o, BC + plane + o, P.w, P.n};
+    return WideSweeps<T>{P.a, BC + o, BC + plane + o, P.w, P.n};
 }
+
+constexpr int WT_ONE = NCPA_WTILE_SMALL;   // geometry of the one-off sweeps (interval, counts, rows)
 
 // k_min, k_max (no WKB fallback, WModeSolver.cpp:572-620) and n(k_min), n(k_max)
 __global__ void __launch_bounds__(256) k_winterval(const ProbDev *__restrict__ probs, int Nz, double dz_m,
@@ -151,7 +155,7 @@ __global__ void __launch_bounds__(256) k_winterval(const ProbDev *__restrict__ p
     __shared__ double shd[8];
     __shared__ int shi[8];
     __shared__ double s_k[2];
-    __shared__ double s_ring[3 * NCPA_RING];
+    __shared__ double s_ring[3 * 2 * WT_ONE];
     const int p = blockIdx.x;
     const ProbDev pr = probs[p];
     double k_min, k_max;
@@ -170,7 +174,7 @@ __global__ void __launch_bounds__(256) k_winterval(const ProbDev *__restrict__ p
     worst = -block_min(-worst, shd);
     if (threadIdx.x < 64) {   // the whole first wave sweeps (lanes 0 and 1 keep their counts)
         const size_t o = size_t(p) * Nz;
-        const WideSweeps ws{A + o, BC + o, BC + plane + o, nullptr, Nz};
+        const WideSweeps<WT_ONE> ws{A + o, BC + o, BC + plane + o, nullptr, Nz};
         const int c = ws.count(s_ring, s_k[threadIdx.x ? 1 : 0]);
         if (threadIdx.x == 0) out[p].n_lo = c;
         if (threadIdx.x == 1) out[p].n_hi = c;
@@ -188,21 +192,22 @@ __global__ void __launch_bounds__(256) k_winterval(const ProbDev *__restrict__ p
 // counts at two wavenumbers (lane 0: lo, lane 1: hi) for one pencil
 __global__ void __launch_bounds__(64) k_wcounts2(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ C,
                            int n, double lo, double hi, int *__restrict__ out) {
-    __shared__ double s_ring[3 * NCPA_RING];
-    const WideSweeps ws{A, B, C, nullptr, n};
+    __shared__ double s_ring[3 * 2 * WT_ONE];
+    const WideSweeps<WT_ONE> ws{A, B, C, nullptr, n};
     const int c = ws.count(s_ring, threadIdx.x ? hi : lo);   // whole wave
     if (threadIdx.x < 2) out[threadIdx.x] = c;
 }
 
-// one block per work item (as k_modes; the wide-angle path has no persistent form)
-template <int MS>
+// one block per work item (as k_modes; the wide-angle path has no persistent form), staged in
+// tiles of T rows
+template <int MS, int T>
 __global__ void __launch_bounds__(64) k_wmodes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
                                                const double *__restrict__ BC, size_t plane, ModesOut out) {
-    __shared__ ModesSharedN<4> sh;
+    __shared__ ModesSharedN<4 * 2 * T> sh;
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob < 0) return;
     const ModeProb &P = mp[wk.prob];
-    const WidePencil pen{wide_sweeps(P, wk.prob, BC, plane)};
+    const WidePencil<T> pen{wide_sweeps<T>(P, wk.prob, BC, plane)};
     modes_item<MS>(P, pen, wk, sh, out);
 }
 
@@ -211,22 +216,36 @@ __global__ void __launch_bounds__(64) k_wrows(const ModeProb *__restrict__ mp, c
                                               const double *__restrict__ k2, RowsArgs ra, double *__restrict__ TH,
                                               int *__restrict__ THE, double *__restrict__ PH, int *__restrict__ PHE,
                                               double *__restrict__ Q0, int *__restrict__ QEX) {
-    __shared__ double s_ring[3 * NCPA_RING];
+    __shared__ double s_ring[3 * 2 * WT_ONE];
     __shared__ double rec[NCPA_CHUNK][64];
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
-    const WidePencil pen{wide_sweeps(P, wk.prob, BC, plane)};
+    const WidePencil<WT_ONE> pen{wide_sweeps<WT_ONE>(P, wk.prob, BC, plane)};
     rows_item(P, pen, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec);
+}
+
+// Batches of more work items than the GPU has SIMDs run many waves per SIMD, where the
+// compact geometry's occupancy wins (broadband: -13 %); below, each wave's own instruction
+// stream sets the time and the 256-row tiles win (1 Hz batch of 64 problems: -10 %).
+#define NCPA_WIDE_COMPACT_ITEMS 4096
+
+template <int T>
+void launch_wmodes_t(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, const double *BC,
+                     size_t plane, const ModesOut &mo) {
+    switch (ms) {
+        case 1: hipLaunchKernelGGL((k_wmodes<1, T>), grid, dim3(64), 0, st, mp, work, BC, plane, mo); break;
+        case 3: hipLaunchKernelGGL((k_wmodes<3, T>), grid, dim3(64), 0, st, mp, work, BC, plane, mo); break;
+        default: hipLaunchKernelGGL((k_wmodes<2, T>), grid, dim3(64), 0, st, mp, work, BC, plane, mo); break;
+    }
 }
 
 void launch_k_wmodes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work,
                      const double *BC, size_t plane, const ModesOut &mo) {
-    switch (ms) {
-        case 1: hipLaunchKernelGGL(k_wmodes<1>, grid, dim3(64), 0, st, mp, work, BC, plane, mo); break;
-        case 3: hipLaunchKernelGGL(k_wmodes<3>, grid, dim3(64), 0, st, mp, work, BC, plane, mo); break;
-        default: hipLaunchKernelGGL(k_wmodes<2>, grid, dim3(64), 0, st, mp, work, BC, plane, mo); break;
-    }
+    if (grid.x >= NCPA_WIDE_COMPACT_ITEMS)
+        launch_wmodes_t<NCPA_WTILE_SMALL>(ms, grid, st, mp, work, BC, plane, mo);
+    else
+        launch_wmodes_t<NCPA_WTILE_BIG>(ms, grid, st, mp, work, BC, plane, mo);
 }
 
 void launch_k_wrows(dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, const double *BC,

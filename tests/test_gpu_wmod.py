@@ -288,3 +288,24 @@ def test_frequency_extremes_vs_oracle(gpu_ctx, synth_profile, f, Nz, canon):
     assert np.max(np.abs(np.sqrt((k ** 2).real) / sol.k - 1)) < 1e-10
     ref = sol.psi[sol.i_s] * sol.psi[sol.i_r]
     assert np.max(np.abs(psr - ref)) <= 1e-7 * np.max(np.abs(ref))
+
+
+def test_compact_geometry_matches(gpu_ctx, synth_profile):
+    """Batches of more than 4096 work items run the compact staging (128-row tiles, 8-row
+    reads); the arithmetic per lane is the same as with 256-row tiles, so a problem's modes
+    from a 600-problem batch equal, bit for bit, those of its solve alone."""
+    import ncpamodes
+    print("case compact geometry", flush=True)
+    pid, info = gpu_ctx.read_profile(synth_profile)
+    g = gpu_ctx.grid(pid)
+    azs = np.linspace(0.0, 359.0, 600)
+    mb = gpu_ctx.solve_wide(g, [ncpamodes.problem(pid, 1.0, a) for a in azs])
+    assert int(np.sum((mb.counts + 63) // 64)) >= 4096
+    kb, psrb, _, _ = mb.download()
+    for i in (0, 311, 599):
+        ms = gpu_ctx.solve_wide(g, [ncpamodes.problem(pid, 1.0, azs[i])])
+        ks, psrs, _, _ = ms.download()
+        s = slice(mb.offsets[i], mb.offsets[i + 1])
+        assert np.array_equal(kb[s], ks) and np.array_equal(psrb[s], psrs)
+        ms.free()
+    mb.free()
