@@ -785,7 +785,8 @@ void release_modes(ncpa_modes *m) {
     if (!m || !m->ctx) return;
     DevicePool &pl = m->ctx->pool;
     void *bufs[] = {m->d_offsets, m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters,
-                    m->d_A, m->d_W, m->d_CE, m->d_mp, m->d_work, m->d_mode_prob, m->d_psis_rows, m->d_BC};
+                    m->d_A, m->d_W, m->d_CE, m->d_mp, m->d_work, m->d_mode_prob, m->d_psis_rows, m->d_BC,
+                    m->d_PQ, m->d_beta};
     for (void *b : bufs) pl.put(b);
 }
 
@@ -883,7 +884,7 @@ int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, con
         const ModesOut mo{m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters};
         if (m->wide)
             launch_k_wmodes(ctx->ms_passes, dim3((unsigned)work.size()), ctx->stream, m->d_mp, m->d_work,
-                            m->d_BC, (size_t)P * Nz, mo);
+                            m->d_BC, (size_t)P * Nz, m->d_PQ, m->d_beta, mo);
         else
             launch_k_modes(ctx->ms_passes, dim3((unsigned)work.size()), ctx->stream, m->d_mp, m->d_work,
                            (int)work.size(), nullptr, nullptr, mo);
@@ -922,7 +923,7 @@ int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, do
         TimedLaunch tl(ctx, "rows");
         if (m->wide)
             launch_k_wrows(dim3((unsigned)m->n_work), ctx->stream, m->d_mp, m->d_work, m->d_BC,
-                           (size_t)m->P * m->grid.Nz, m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
+                           (size_t)m->P * m->grid.Nz, m->d_PQ, m->d_beta, m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
         else
             hipLaunchKernelGGL(k_rows, dim3((unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
                                m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);

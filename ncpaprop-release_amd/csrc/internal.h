@@ -94,10 +94,10 @@ void launch_node_tables(hipStream_t st, const ProbDev *probs, int P, int ldn, do
 // wide.hip: the wide-angle eigenvalue and eigenvector-rows kernels; BC holds the B rows of
 // every problem (plane elements) followed by the C rows
 void launch_k_wmodes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work,
-                     const double *BC, size_t plane, const ModesOut &mo);
+                     const double *BC, size_t plane, const double *PQ, const double *beta, const ModesOut &mo);
 void launch_k_wrows(dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, const double *BC,
-                    size_t plane, const double *k2, const RowsArgs &ra, double *TH, int *THE, double *PH,
-                    int *PHE, double *Q0, int *QEX);
+                    size_t plane, const double *PQ, const double *beta, const double *k2, const RowsArgs &ra,
+                    double *TH, int *THE, double *PH, int *PHE, double *Q0, int *QEX);
 
 // broadband.hip
 void bb_launch_kpert(hipStream_t st, int n_cap, const int *n_dev, const double *k2, const double *absn,
@@ -128,6 +128,7 @@ struct ncpa_modes {
     int *d_iters = nullptr;
     double *d_A = nullptr, *d_W = nullptr, *d_CE = nullptr;   // per-problem grid samples
     double *d_BC = nullptr;                    // wide-angle B and C rows
+    double *d_PQ = nullptr, *d_beta = nullptr; // wide-angle factored rows P, Q and ground terms
     double *d_psis_rows = nullptr;             // psi(zs) from the last eigenvector-rows pass
     ncpa::ModeProb *d_mp = nullptr;
     ncpa::ModeWork *d_work = nullptr;

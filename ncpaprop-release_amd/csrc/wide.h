@@ -298,16 +298,52 @@ __device__ __forceinline__ double wide_row(const double (&v)[3], double k) { ret
 
 // The wide-angle pencil of one problem: rows A, B, C of dz^2 T(k) and absorption weights W.
 // ring: 4 staged arrays (4 x 2 T doubles).
-template <int T>
+//
+// With PQ the counts and Newton sweeps use the factored rows of the physical operator
+// (WModeSolver.cpp:549-553): with P = dz omega/c0, Q = dz wc/c0,
+//     A = P^2 - 2 (+ beta on row 0),  B = -2 P Q,  C = Q^2 - dz^2,
+//     a_i(k) = (P_i - k Q_i)^2 - 2 - dz^2 k^2 (+ beta = 1/(dz adm + 1) on row 0),
+// two staged arrays instead of three (the rows 1 .. n-1 are swept, row 0 is stepped last).
+template <int T, bool PQ = false>
 struct WideSweeps {
     const double *A, *B, *C, *W;
     int n;
+    const double *P = nullptr, *Q = nullptr;
+    double h2 = 0.0, beta = 0.0;
 
     // number of negative eigenvalues of dz^2 T(k)
     __device__ __forceinline__ int count(double *ring, double k) const {
         double p1 = 1.0, p2 = 0.0;
         int cnt = 0;
         unsigned bits = 0u;
+        if constexpr (PQ) {
+            const double K = fma(-h2 * k, k, -2.0);
+            const double *const arr[2] = {P + 1, Q + 1};
+            auto chunk_end = [&](int, int m) {
+                cnt += sign_changes(bits, m);
+                bits &= 1u;
+                const int e = pow2_exponent(p1);
+                p1 = ldexp(p1, -e);
+                p2 = ldexp(p2, -e);
+            };
+            sweep_down_n<T, 2>(
+                arr, ring, n - 1,
+                [&](const double (&v)[2], int) {
+                    const double t = fma(-k, v[1], v[0]);
+                    const double p0 = fma(fma(t, t, K), p1, -p2);
+                    bits = push_sign(bits, p0);
+                    p2 = p1;
+                    p1 = p0;
+                },
+                chunk_end);
+            const double t = fma(-k, Q[0], P[0]);
+            const double p0 = fma(fma(t, t, K) + beta, p1, -p2);
+            bits = push_sign(bits, p0);
+            p2 = p1;
+            p1 = p0;
+            chunk_end(0, 1);
+            return cnt;
+        }
         const double *const arr[3] = {A, B, C};
         sweep_down_n<T, 3>(
             arr, ring, n,
@@ -332,6 +368,44 @@ struct WideSweeps {
         double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
         int cnt = 0;
         unsigned bits = 0u;
+        if constexpr (PQ) {
+            // q carried as -q/2: da/dk = -2 (Q t + dz^2 k) with t = P - k Q
+            const double K = fma(-h2 * k, k, -2.0), hk = h2 * k;
+            const double *const arr[2] = {P + 1, Q + 1};
+            auto step = [&](double a, double m) {
+                const double p0 = fma(a, p1, -p2);
+                const double q0 = fma(a, q1, fma(m, p1, -q2));
+                bits = push_sign(bits, p0);
+                p2 = p1;
+                p1 = p0;
+                q2 = q1;
+                q1 = q0;
+            };
+            auto chunk_end = [&](int, int m) {
+                cnt += sign_changes(bits, m);
+                bits &= 1u;
+                const int e = pow2_exponent(p1);
+                p1 = ldexp(p1, -e);
+                p2 = ldexp(p2, -e);
+                q1 = ldexp(q1, -e);
+                q2 = ldexp(q2, -e);
+            };
+            sweep_down_n<T, 2>(
+                arr, ring, n - 1,
+                [&](const double (&v)[2], int) {
+                    const double t = fma(-k, v[1], v[0]);
+                    step(fma(t, t, K), fma(v[1], t, hk));
+                },
+                chunk_end);
+            const double t = fma(-k, Q[0], P[0]);
+            step(fma(t, t, K) + beta, fma(Q[0], t, hk));
+            chunk_end(0, 1);
+            NewtonOut o;
+            o.cnt = cnt;
+            o.p0 = p1;
+            o.q0 = -2.0 * q1;
+            return o;
+        }
         const double k2 = k + k;
         const double *const arr[3] = {A, B, C};
         sweep_down_n<T, 3>(
@@ -371,6 +445,46 @@ struct WideSweeps {
         FinalOut o;
         o.phi_s1 = SVal{1.0, 0};
         o.phi_r1 = SVal{1.0, 0};
+        if constexpr (PQ) {
+            const double K = fma(-h2 * k, k, -2.0);
+            auto step = [&](double am, double w) {
+                const double p0 = fma(am, p1, -p2);
+                const double q0 = fma(am, q1, -(q2 + p1));
+                const double e0 = fma(am, e1, fma(w, p1, -e2));
+                p2 = p1;
+                p1 = p0;
+                q2 = q1;
+                q1 = q0;
+                e2 = e1;
+                e1 = e0;
+            };
+            const double *const arr[3] = {P + 1, Q + 1, W + 1};
+            sweep_down_n<T, 3>(
+                arr, ring, n - 1,
+                [&](const double (&v)[3], int i1) {   // row i1 + 1
+                    const double t = fma(-k, v[1], v[0]);
+                    step(fma(t, t, K), v[2]);
+                    if (i1 == srow) o.phi_s1 = SVal{p1, ex};
+                    if (i1 == rrow) o.phi_r1 = SVal{p1, ex};
+                },
+                [&](int, int) {
+                    const int e = pow2_exponent(p1);
+                    p1 = ldexp(p1, -e);
+                    p2 = ldexp(p2, -e);
+                    q1 = ldexp(q1, -e);
+                    q2 = ldexp(q2, -e);
+                    e1 = ldexp(e1, -e);
+                    e2 = ldexp(e2, -e);
+                    ex += e;
+                });
+            const double t = fma(-k, Q[0], P[0]);
+            step(fma(t, t, K) + beta, W[0]);   // row 0 (no record: records are rows s + 1 >= 1)
+            o.p0 = p1;
+            o.q0 = q1;
+            o.e0 = e1;
+            o.ex = ex;
+            return o;
+        }
         const double *const arr[4] = {A, B, C, W};
         sweep_down_n<T, 4>(
             arr, ring, n,
@@ -412,6 +526,33 @@ struct WideSweeps {
         const int top = srow > rrow ? srow : rrow;
         double t1 = 1.0, t2 = 0.0;
         int ex = 0;
+        if constexpr (PQ) {
+            if (top <= 0) return;
+            const double K = fma(-h2 * k, k, -2.0);
+            const double t = fma(-k, Q[0], P[0]);
+            t2 = t1;
+            t1 = fma(t, t, K) + beta;   // theta_0 (row 0)
+            if (srow == 1) th_s = SVal{t1, ex};
+            if (rrow == 1) th_r = SVal{t1, ex};
+            const double *const arr[2] = {P + 1, Q + 1};
+            sweep_up_n<T, 2>(
+                arr, ring, top - 1,
+                [&](const double (&v)[2], int i1) {   // row i1 + 1
+                    const double tt = fma(-k, v[1], v[0]);
+                    const double t0 = fma(fma(tt, tt, K), t1, -t2);
+                    t2 = t1;
+                    t1 = t0;
+                    if (i1 + 2 == srow) th_s = SVal{t1, ex};
+                    if (i1 + 2 == rrow) th_r = SVal{t1, ex};
+                },
+                [&](int, int) {
+                    const int e = pow2_exponent(t1);
+                    t1 = ldexp(t1, -e);
+                    t2 = ldexp(t2, -e);
+                    ex += e;
+                });
+            return;
+        }
         const double *const arr[3] = {A, B, C};
         sweep_up_n<T, 3>(
             arr, ring, top,
@@ -431,13 +572,46 @@ struct WideSweeps {
     }
 
     // rows-pass sweeps with the row value a_i(k) handed to step(am, i)
+    // (PQ: row 0 stepped first going up and last going down, the chunks of rows 1 .. m-1 between)
     template <class Step, class ChunkEnd>
     __device__ __forceinline__ void up(double *ring, int m, double k, Step step, ChunkEnd ce) const {
+        if constexpr (PQ) {
+            if (m <= 0) return;
+            const double K = fma(-h2 * k, k, -2.0);
+            const double t = fma(-k, Q[0], P[0]);
+            step(fma(t, t, K) + beta, 0);
+            ce(0, 1);
+            const double *const arr[2] = {P + 1, Q + 1};
+            sweep_up_n<T, 2>(
+                arr, ring, m - 1,
+                [&](const double (&v)[2], int i1) {
+                    const double tt = fma(-k, v[1], v[0]);
+                    step(fma(tt, tt, K), i1 + 1);
+                },
+                [&](int lo1, int len) { ce(lo1 + 1, len); });
+            return;
+        }
         const double *const arr[3] = {A, B, C};
         sweep_up_n<T, 3>(arr, ring, m, [&](const double (&v)[3], int i) { step(wide_row(v, k), i); }, ce);
     }
     template <class Step, class ChunkEnd>
     __device__ __forceinline__ void down(double *ring, int m, double k, Step step, ChunkEnd ce) const {
+        if constexpr (PQ) {
+            if (m <= 0) return;
+            const double K = fma(-h2 * k, k, -2.0);
+            const double *const arr[2] = {P + 1, Q + 1};
+            sweep_down_n<T, 2>(
+                arr, ring, m - 1,
+                [&](const double (&v)[2], int i1) {
+                    const double tt = fma(-k, v[1], v[0]);
+                    step(fma(tt, tt, K), i1 + 1);
+                },
+                [&](int lo1, int len) { ce(lo1 + 1, len); });
+            const double t = fma(-k, Q[0], P[0]);
+            step(fma(t, t, K) + beta, 0);
+            ce(0, 1);
+            return;
+        }
         const double *const arr[3] = {A, B, C};
         sweep_down_n<T, 3>(arr, ring, m, [&](const double (&v)[3], int i) { step(wide_row(v, k), i); }, ce);
     }

@@ -56,11 +56,12 @@ __device__ __forceinline__ double spline_f(const double *zn, int n, YF yf, doubl
 // 202-209), matrix spacing dz_m (WModeSolver.cpp:297):
 //   kd = (omega/c0)^2, md = (wc/c0)^2 - 1, cd = -2 omega wc/c0^2 (:549-553), the ground term
 //   1/((dz adm + 1) dz^2) added to kd[0] (:568-570), A = -2 + dz^2 kd, B = dz^2 cd,
-//   C = dz^2 md; W = 2 (omega/c_T) alpha (ModeSolver.cpp:276-278); CE = c_eff (:554).
+//   C = dz^2 md; W = 2 (omega/c_T) alpha (ModeSolver.cpp:276-278); CE = c_eff (:554); the
+//   factored rows P = dz omega/c0, Q = dz wc/c0 of the counts (wide.h).
 __global__ void k_wsetup_grid(const ProbDev *__restrict__ probs, int Nz, double z_min, double dz_h, double dz_m,
                               const double *__restrict__ alpha_nodes, const double *__restrict__ ceff_nodes,
                               int ldn, double *__restrict__ A, double *__restrict__ BC, size_t plane,
-                              double *__restrict__ W, double *__restrict__ CE) {
+                              double *__restrict__ W, double *__restrict__ CE, double *__restrict__ PQ) {
     const int p = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nz) return;
@@ -86,15 +87,17 @@ __global__ void k_wsetup_grid(const ProbDev *__restrict__ probs, int Nz, double 
     A[o] = -2.0 + h2 * kd;
     BC[o] = h2 * cd;
     BC[plane + o] = h2 * md;
+    PQ[o] = dz_m * (om / c0);   // the factored rows: P = dz omega/c0, Q = dz wc/c0
+    PQ[plane + o] = dz_m * (wc / c0);
     W[o] = (om / cT) * alpha * 2.0;
     CE[o] = ceff;
 }
 
 // The pencil of problem p: A and W rows from the ModeProb, B and C from the BC planes.
-template <int T>
+template <int T, bool PQ = false>
 struct WidePencil {
     static constexpr bool early_accept = false;
-    WideSweeps<T> s;
+    WideSweeps<T, PQ> s;
 
     __device__ __forceinline__ int count(double *ring, double k) const { return s.count(ring, k); }
     __device__ __forceinline__ NewtonOut newton(double *ring, double k) const { return s.newton(ring, k); }
@@ -139,10 +142,19 @@ struct WidePencil {
     }
 };
 
-template <int T>
-__device__ __forceinline__ WideSweeps<T> wide_sweeps(const ModeProb &P, int prob, const double *BC, size_t plane) {
+template <int T, bool PQ = false>
+__device__ __forceinline__ WideSweeps<T, PQ> wide_sweeps(const ModeProb &P, int prob, const double *BC,
+                                                         size_t plane, const double *PQp = nullptr,
+                                                         const double *beta = nullptr) {
     const size_t o = (size_t)prob * P.n;
-    return WideSweeps<T>{P.a, BC + o, BC + plane + o, P.w, P.n};
+    WideSweeps<T, PQ> w{P.a, BC + o, BC + plane + o, P.w, P.n};
+    if constexpr (PQ) {
+        w.P = PQp + o;
+        w.Q = PQp + plane + o;
+        w.h2 = P.h2;
+        w.beta = beta[prob];
+    }
+    return w;
 }
 
 constexpr int WT_ONE = NCPA_WTILE_SMALL;   // geometry of the one-off sweeps (interval, counts, rows)
@@ -151,7 +163,7 @@ constexpr int WT_ONE = NCPA_WTILE_SMALL;   // geometry of the one-off sweeps (in
 __global__ void __launch_bounds__(256) k_winterval(const ProbDev *__restrict__ probs, int Nz, double dz_m,
                                                    const double *__restrict__ A, const double *__restrict__ BC,
                                                    size_t plane, const double *__restrict__ CE,
-                                                   IntervalOut *__restrict__ out) {
+                                                   const double *__restrict__ PQ, IntervalOut *__restrict__ out) {
     __shared__ double shd[8];
     __shared__ int shi[8];
     __shared__ double s_k[2];
@@ -173,8 +185,13 @@ __global__ void __launch_bounds__(256) k_winterval(const ProbDev *__restrict__ p
     }
     worst = -block_min(-worst, shd);
     if (threadIdx.x < 64) {   // the whole first wave sweeps (lanes 0 and 1 keep their counts)
+        // the factored rows, as the eigenvalue kernel counts with them
         const size_t o = size_t(p) * Nz;
-        const WideSweeps<WT_ONE> ws{A + o, BC + o, BC + plane + o, nullptr, Nz};
+        WideSweeps<WT_ONE, true> ws{A + o, BC + o, BC + plane + o, nullptr, Nz};
+        ws.P = PQ + o;
+        ws.Q = PQ + plane + o;
+        ws.h2 = dz_m * dz_m;
+        ws.beta = 1.0 / (dz_m * pr.admittance + 1);
         const int c = ws.count(s_ring, s_k[threadIdx.x ? 1 : 0]);
         if (threadIdx.x == 0) out[p].n_lo = c;
         if (threadIdx.x == 1) out[p].n_hi = c;
@@ -200,19 +217,23 @@ __global__ void __launch_bounds__(64) k_wcounts2(const double *__restrict__ A, c
 
 // one block per work item (as k_modes; the wide-angle path has no persistent form), staged in
 // tiles of T rows
-template <int MS, int T>
+template <int MS, int T, bool PQ>
 __global__ void __launch_bounds__(64) k_wmodes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
-                                               const double *__restrict__ BC, size_t plane, ModesOut out) {
-    __shared__ ModesSharedN<4 * 2 * T> sh;
+                                               const double *__restrict__ BC, size_t plane,
+                                               const double *__restrict__ PQp, const double *__restrict__ beta,
+                                               ModesOut out) {
+    __shared__ ModesSharedN<(PQ ? 3 : 4) * 2 * T> sh;   // staged arrays of the final sweep: P, Q, W or A, B, C, W
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob < 0) return;
     const ModeProb &P = mp[wk.prob];
-    const WidePencil<T> pen{wide_sweeps<T>(P, wk.prob, BC, plane)};
+    const WidePencil<T, PQ> pen{wide_sweeps<T, PQ>(P, wk.prob, BC, plane, PQp, beta)};
     modes_item<MS>(P, pen, wk, sh, out);
 }
 
+template <bool PQ>
 __global__ void __launch_bounds__(64) k_wrows(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
                                               const double *__restrict__ BC, size_t plane,
+                                              const double *__restrict__ PQp, const double *__restrict__ beta,
                                               const double *__restrict__ k2, RowsArgs ra, double *__restrict__ TH,
                                               int *__restrict__ THE, double *__restrict__ PH, int *__restrict__ PHE,
                                               double *__restrict__ Q0, int *__restrict__ QEX) {
@@ -221,7 +242,7 @@ __global__ void __launch_bounds__(64) k_wrows(const ModeProb *__restrict__ mp, c
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
-    const WidePencil<WT_ONE> pen{wide_sweeps<WT_ONE>(P, wk.prob, BC, plane)};
+    const WidePencil<WT_ONE, PQ> pen{wide_sweeps<WT_ONE, PQ>(P, wk.prob, BC, plane, PQp, beta)};
     rows_item(P, pen, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec);
 }
 
@@ -230,28 +251,39 @@ __global__ void __launch_bounds__(64) k_wrows(const ModeProb *__restrict__ mp, c
 // stream sets the time and the 256-row tiles win (1 Hz batch of 64 problems: -10 %).
 #define NCPA_WIDE_COMPACT_ITEMS 4096
 
-template <int T>
+template <int T, bool PQ>
 void launch_wmodes_t(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, const double *BC,
-                     size_t plane, const ModesOut &mo) {
+                     size_t plane, const double *PQp, const double *beta, const ModesOut &mo) {
     switch (ms) {
-        case 1: hipLaunchKernelGGL((k_wmodes<1, T>), grid, dim3(64), 0, st, mp, work, BC, plane, mo); break;
-        case 3: hipLaunchKernelGGL((k_wmodes<3, T>), grid, dim3(64), 0, st, mp, work, BC, plane, mo); break;
-        default: hipLaunchKernelGGL((k_wmodes<2, T>), grid, dim3(64), 0, st, mp, work, BC, plane, mo); break;
+        case 1: hipLaunchKernelGGL((k_wmodes<1, T, PQ>), grid, dim3(64), 0, st, mp, work, BC, plane, PQp, beta, mo); break;
+        case 3: hipLaunchKernelGGL((k_wmodes<3, T, PQ>), grid, dim3(64), 0, st, mp, work, BC, plane, PQp, beta, mo); break;
+        default: hipLaunchKernelGGL((k_wmodes<2, T, PQ>), grid, dim3(64), 0, st, mp, work, BC, plane, PQp, beta, mo); break;
     }
 }
 
+// PQ (the factored rows of ncpa_wmod_solve) or nullptr (any kd, md, cd: ncpa_wmod_eigen)
 void launch_k_wmodes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work,
-                     const double *BC, size_t plane, const ModesOut &mo) {
-    if (grid.x >= NCPA_WIDE_COMPACT_ITEMS)
-        launch_wmodes_t<NCPA_WTILE_SMALL>(ms, grid, st, mp, work, BC, plane, mo);
+                     const double *BC, size_t plane, const double *PQ, const double *beta, const ModesOut &mo) {
+    const bool compact = grid.x >= NCPA_WIDE_COMPACT_ITEMS;
+    if (PQ && compact)
+        launch_wmodes_t<NCPA_WTILE_SMALL, true>(ms, grid, st, mp, work, BC, plane, PQ, beta, mo);
+    else if (PQ)
+        launch_wmodes_t<NCPA_WTILE_BIG, true>(ms, grid, st, mp, work, BC, plane, PQ, beta, mo);
+    else if (compact)
+        launch_wmodes_t<NCPA_WTILE_SMALL, false>(ms, grid, st, mp, work, BC, plane, nullptr, nullptr, mo);
     else
-        launch_wmodes_t<NCPA_WTILE_BIG>(ms, grid, st, mp, work, BC, plane, mo);
+        launch_wmodes_t<NCPA_WTILE_BIG, false>(ms, grid, st, mp, work, BC, plane, nullptr, nullptr, mo);
 }
 
 void launch_k_wrows(dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, const double *BC,
-                    size_t plane, const double *k2, const RowsArgs &ra, double *TH, int *THE, double *PH,
-                    int *PHE, double *Q0, int *QEX) {
-    hipLaunchKernelGGL(k_wrows, grid, dim3(64), 0, st, mp, work, BC, plane, k2, ra, TH, THE, PH, PHE, Q0, QEX);
+                    size_t plane, const double *PQ, const double *beta, const double *k2, const RowsArgs &ra,
+                    double *TH, int *THE, double *PH, int *PHE, double *Q0, int *QEX) {
+    if (PQ)
+        hipLaunchKernelGGL(k_wrows<true>, grid, dim3(64), 0, st, mp, work, BC, plane, PQ, beta, k2, ra, TH, THE, PH,
+                           PHE, Q0, QEX);
+    else
+        hipLaunchKernelGGL(k_wrows<false>, grid, dim3(64), 0, st, mp, work, BC, plane, nullptr, nullptr, k2, ra, TH,
+                           THE, PH, PHE, Q0, QEX);
 }
 
 }  // namespace ncpa
@@ -297,6 +329,8 @@ int ncpa_wmod_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem
     m->d_BC = pool_alloc<double>(ctx, 2 * plane);
     m->d_W = pool_alloc<double>(ctx, plane);
     m->d_CE = pool_alloc<double>(ctx, plane);
+    m->d_PQ = pool_alloc<double>(ctx, 2 * plane);
+    m->d_beta = pool_alloc<double>(ctx, P);
     IntervalOut *d_iv = pool_alloc<IntervalOut>(ctx, P);
     auto fail = [&](int code) {
         ctx->pool.put(d_probs);
@@ -306,21 +340,29 @@ int ncpa_wmod_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem
         delete m;
         return code;
     };
-    if (!d_probs || !d_nodes || !m->d_A || !m->d_BC || !m->d_W || !m->d_CE || !d_iv)
+    if (!d_probs || !d_nodes || !m->d_A || !m->d_BC || !m->d_W || !m->d_CE || !m->d_PQ || !m->d_beta || !d_iv)
         return fail(set_error(ctx, NCPA_ERR_ALLOC, "ncpa_wmod_solve: device allocation failed"));
     hipError_t e = hipMemcpyAsync(d_probs, hp.data(), sizeof(ProbDev) * P, hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return fail(hip_error(ctx, e, "copy problems"));
+    {
+        // row 0's ground term of the factored rows: dz^2 / ((dz adm + 1) dz^2) (WModeSolver.cpp:568-570)
+        std::vector<double> beta(P);
+        for (int p = 0; p < P; ++p) beta[p] = 1.0 / (gm.dz * hp[p].admittance + 1);
+        e = hipMemcpyAsync(m->d_beta, beta.data(), sizeof(double) * P, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);   // beta is a host temporary
+        if (e != hipSuccess) return fail(hip_error(ctx, e, "copy ground terms"));
+    }
     {
         TimedLaunch tl(ctx, "setup");
         launch_node_tables(ctx->stream, d_probs, P, max_nodes, d_nodes);
         hipLaunchKernelGGL(k_wsetup_grid, dim3((Nz + 255) / 256, P), dim3(256), 0, ctx->stream, d_probs, Nz,
                            grid->z_min, grid->dz, gm.dz, d_nodes, d_nodes + size_t(P) * max_nodes, max_nodes,
-                           m->d_A, m->d_BC, plane, m->d_W, m->d_CE);
+                           m->d_A, m->d_BC, plane, m->d_W, m->d_CE, m->d_PQ);
     }
     {
         TimedLaunch tl(ctx, "interval");
         hipLaunchKernelGGL(k_winterval, dim3(P), dim3(256), 0, ctx->stream, d_probs, Nz, gm.dz, m->d_A, m->d_BC,
-                           plane, m->d_CE, d_iv);
+                           plane, m->d_CE, m->d_PQ, d_iv);
     }
     if ((e = hipGetLastError()) != hipSuccess) return fail(hip_error(ctx, e, "wide-angle setup kernels"));
     m->iv.resize(P);
