@@ -701,12 +701,12 @@ def wmod_main(args, cpu):
         "stage_ms": {k: v[0] for k, v in fams.items()},
         "ess_same_batch": {"solves_per_s": P * args.steps / ess_elapsed,
                            "wide_over_ess_time": elapsed / ess_elapsed},
-        "roofline": {"kernel": "k_wmodes<2>", "bound": "mfma", "achieved": achieved,
+        "roofline": {"kernel": "k_wmodes<2, 256, true>", "bound": "mfma", "achieved": achieved,
                      "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"],
-                     "traffic": measured_traffic("k_wmodes<2>"),
-                     "algorithmic": "6 FP64 flops per row per sweep (the row value a_i(k) = A + k(B + kC) "
-                                    "and the continuant; Newton sweeps do 12, counted as 6); sweeps = "
-                                    "search sweeps + 2 per eigenvalue; FP64 VALU, MI355X FP64 peak"},
+                     "traffic": measured_traffic("k_wmodes<2, 256, true>"),
+                     "algorithmic": "6 FP64 flops per row per sweep (the row value a_i(k) = (P - kQ)^2 "
+                                    "- 2 - dz^2 k^2 and the continuant; Newton sweeps do 12, counted as 6); "
+                                    "sweeps = search sweeps + 2 per eigenvalue; FP64 VALU, MI355X FP64 peak"},
         "cpu_baseline": None,
     }
     if cpu is not None:
@@ -787,9 +787,9 @@ def broadband_wmod_main(args):
                                "%d; eager (one batched solve per job)" % nfft,
                    "global_batch": P, "modes_per_step": int(total_modes), "parallelism": "single GPU"},
         "stage_ms": {k: v[0] for k, v in fams.items()},
-        "roofline": {"kernel": "k_wmodes<2>", "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f64"],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"], "traffic": None,
-                     "algorithmic": "6 FP64 flops per row per sweep, as the wmod workload"},
+        "roofline": {"kernel": "k_wmodes<2, 128, true>", "bound": "mfma", "achieved": achieved,
+                     "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"],
+                     "traffic": None, "algorithmic": "6 FP64 flops per row per sweep, as the wmod workload"},
         "cpu_baseline": None,
     }
 
