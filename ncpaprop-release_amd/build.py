@@ -3,7 +3,7 @@
     python ncpaprop-release_amd/build.py            # libncpamodes.so next to this file
 
 Every HIP source is compiled with hipcc for gfx950 only; packed-FP32 instruction
-selection is disabled (see DESIGN.md, MI355X notes).
+selection is disabled (-packed-fp32-ops): only scalar FP32/FP64 VALU forms are emitted.
 """
 import os
 import subprocess

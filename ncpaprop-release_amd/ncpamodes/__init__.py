@@ -201,7 +201,8 @@ class FieldStage:
 
 
 class DeviceArray:
-    """A flat device buffer allocated through the C ABI (no torch: see DESIGN.md)."""
+    """A flat device buffer allocated through the C ABI.  Not a torch tensor: torch bundles its
+    own HIP runtime, and two HIP runtimes cannot share a process."""
 
     def __init__(self, ctx, n, dtype=np.float64):
         self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
