@@ -59,3 +59,25 @@ def test_max_modes_rejected_context_usable(gpu_ctx, synth_profile, wide):
     after = solve(g, ok).download()
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("wide", [False, True])
+@pytest.mark.parametrize("Nz", [100, 300])
+def test_small_grids_vs_oracle(gpu_ctx, synth_profile, wide, Nz):
+    """Grids smaller than one 256-row LDS tile (100 rows) and not a multiple of the 16-row
+    chunk (300 rows): every sweep starts in a partial tile.  Counts, k, Im k and psi(zs)psi(zr)
+    against the oracle on the same grid (the matrix is coarse, the parity is exact)."""
+    import ncpamodes
+    from oracle import ess, wmod
+    print("case small grid Nz=%d wide=%s" % (Nz, wide), flush=True)
+    f, az = 0.1, 90.0
+    pid, _ = gpu_ctx.read_profile(synth_profile)
+    g = gpu_ctx.grid(pid, Nz=Nz)
+    m = (gpu_ctx.solve_wide if wide else gpu_ctx.solve)(g, [ncpamodes.problem(pid, f, az)])
+    k, psr, _, _ = m.problem_modes(0)
+    sol = (wmod if wide else ess).solve(synth_profile, freq=f, azimuth=az, Nz=Nz, turnoff_wkb=True)
+    assert k.size == sol.k.size > 0, (k.size, sol.k.size)
+    assert np.max(np.abs(np.sqrt((k ** 2).real) / sol.k - 1)) < 1e-10
+    assert np.allclose(k.imag, sol.k_pert.imag, rtol=1e-6, atol=1e-16)
+    ref = sol.psi[sol.i_s] * sol.psi[sol.i_r]
+    assert np.max(np.abs(psr - ref)) <= 1e-7 * np.max(np.abs(ref))
