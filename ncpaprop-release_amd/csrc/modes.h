@@ -53,9 +53,16 @@ __device__ inline int block_min_int(int v, int *sh) {
 // WModeSolver.cpp:356-360).  wkb_fallback: drop the cut-off when it falls below k_min, as
 // ESSModeSolver.cpp:616-621 does (WModeSolver has no such rule).  dz: the spacing of the
 // WKB integral.
+//
+// The WKB stop test `integral >= 10` decides k_max to one step dkk, so it must come out as
+// the reference's left-to-right sum would: every term is formed with the reference's
+// operations (no fused multiply-add anywhere in this function), the block sums them in
+// parallel, and when that sum lies within the bound of both summation errors of 10 the
+// first lane forms the sequential sum itself and that decides.
 __device__ inline void interval_bounds(const ProbDev &pr, int Nz, double dz, const double *ce, double *shd,
                                        int *shi, bool wkb_fallback, double &k_min_o, double &k_max_o,
                                        int &used_wkb_o) {
+#pragma clang fp contract(off)
     const double om = pr.omega;
     double m = DBL_MAX;
     for (int i = threadIdx.x; i < Nz; i += blockDim.x) m = fmin(m, ce[i]);
@@ -73,6 +80,11 @@ __device__ inline void interval_bounds(const ProbDev &pr, int Nz, double dz, con
             // the reference steps ~100 times; the cap only bounds a degenerate dkk
             int steps = 0;
             for (kk = kg2; kk < kmf2 && steps < 1000; kk = kk + dkk, ++steps) {
+                // the reference's `while (wkbTerm > dkk)` starts from wkbTerm = 1: with
+                // dkk >= 1 its body never runs and the integral stays 0
+                if (dkk >= 1.0) continue;
+                // rows 0 .. first (inclusive): first = the lowest row with |kk - k_eff^2| <= dkk
+                // (every row when none is, where the reference would read past the grid)
                 int first = Nz - 1;
                 for (int i = threadIdx.x; i < Nz; i += blockDim.x) {
                     const double ke = om / ce[i];
@@ -85,7 +97,24 @@ __device__ inline void interval_bounds(const ProbDev &pr, int Nz, double dz, con
                     s += dz * sqrt(fabs(kk - ke * ke));
                 }
                 const double integral = block_sum(s, shd);
-                if (integral >= 10.0) break;
+                // both sums of n = first + 1 non-negative terms lie within (n - 1) u of the
+                // exact sum (u = 2^-53); margin: twice that, with room
+                const double margin = 2.5e-16 * double(first + 2) * fmax(integral, 10.0);
+                int stop = integral >= 10.0;
+                if (fabs(integral - 10.0) <= margin) {   // same branch for the whole block
+                    if (threadIdx.x == 0) {
+                        double q = 0.0;
+                        for (int i = 0; i <= first; ++i) {
+                            const double ke = om / ce[i];
+                            q = q + dz * sqrt(fabs(kk - ke * ke));
+                        }
+                        shi[0] = q >= 10.0;
+                    }
+                    __syncthreads();
+                    stop = shi[0];
+                    __syncthreads();
+                }
+                if (stop) break;
             }
         }
         k_max = sqrt(kk);

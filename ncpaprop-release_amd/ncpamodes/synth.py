@@ -94,3 +94,28 @@ def write_profile(path, seed=0, amp=0.2, height_shift_km=0.0):
     with open(path, "w") as fh:
         fh.write(profile_text(seed, amp, height_shift_km))
     return path
+
+
+def hydrostatic(z_km, T, P0=101325.0):
+    """P [Pa] on the nodes z_km for temperatures T [K]: d ln P/dz = -g/(R T), trapezoidal
+    in 1/T between nodes (for profiles other than the family's piecewise-linear T)."""
+    z_m = np.asarray(z_km, dtype=np.float64) * 1000.0
+    inv = 1.0 / np.asarray(T, dtype=np.float64)
+    lnp = np.concatenate([[0.0], np.cumsum(0.5 * (inv[1:] + inv[:-1]) * np.diff(z_m))])
+    return P0 * np.exp(-G0 / R_AIR * lnp)
+
+
+def write_columns(path, z_km, u, v, T, P=None):
+    """Write a z,u,v,w,t,d,p profile (header and units as the family's) from node arrays:
+    z [km], u, v [m/s], T [K], P [Pa] (hydrostatic from T when None); w = 0, rho = P/(R T)."""
+    z_km = np.asarray(z_km, dtype=np.float64)
+    T = np.asarray(T, dtype=np.float64)
+    P = hydrostatic(z_km, T) if P is None else np.asarray(P, dtype=np.float64)
+    rho = P / (R_AIR * T)
+    data = np.column_stack([z_km, np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64),
+                            np.zeros_like(z_km), T, rho * 1e-3, P * 0.01])
+    with open(path, "w") as fh:
+        fh.write(HEADER)
+        for row in data:
+            fh.write(" ".join("%.10e" % x for x in row) + "\n")
+    return path

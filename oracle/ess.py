@@ -84,25 +84,7 @@ def modal_trace(grid, freq, azimuth, zs_m, zr_m, turnoff_wkb, adm=0.0):
     used_wkb = False
     if abs(zs_m) < 1.0e-3 and abs(zr_m) < 1.0e-3 and not turnoff_wkb:
         used_wkb = True
-        k_max_full = omega / ceffmin
-        k_gnd = omega / ceff_grnd
-        dkk = (k_max_full ** 2 - k_gnd ** 2) / 100.0
-        kk = k_gnd ** 2
-        if dkk > 1.0e-10:
-            keff2 = (omega / ceffz) ** 2
-            kk = k_gnd ** 2
-            while kk < k_max_full ** 2:
-                term = np.abs(kk - keff2)
-                # while (wkbTerm > dkk): the loop body runs for i = 0 .. first index with
-                # term <= dkk (inclusive)
-                stop = np.nonzero(term <= dkk)[0]
-                last = stop[0] if stop.size else nz - 1
-                # sequential left-to-right sum, as the reference loop accumulates
-                integral = float(np.cumsum(dz * np.sqrt(term[:last + 1]))[-1])
-                if integral >= 10.0:
-                    break
-                kk = kk + dkk
-        k_max = math.sqrt(kk)
+        k_max = math.sqrt(wkb_cutoff(ceffz, omega, dz)[0])
     else:
         k_max = omega / ceffmin
     top = nz - nz // 10
@@ -110,6 +92,41 @@ def modal_trace(grid, freq, azimuth, zs_m, zr_m, turnoff_wkb, adm=0.0):
     if used_wkb and k_max < k_min:
         k_max = omega / ceffmin
     return diag, ceffz, k_min, k_max, used_wkb
+
+
+def wkb_cutoff(ceffz, omega, dz):
+    """The WKB cut-off loop of getModalTrace (ESSModeSolver.cpp:561-607).
+
+    kk steps from k_gnd^2 by dkk = (k_max_full^2 - k_gnd^2)/100, k_gnd = omega/c_eff[0] and
+    k_max_full = omega/min c_eff, and stops at the first kk whose integral
+    sum dz sqrt|kk - k_eff^2| over rows 0 .. first (the lowest row with |kk - k_eff^2| <= dkk,
+    inclusive, as `while (wkbTerm > dkk)` runs its body for it) reaches 10.  Returns (kk, steps)
+    with steps = [(kk, first, integral), ...] in the order tried; k_max = sqrt(kk).
+    """
+    ceffz = np.asarray(ceffz, dtype=np.float64)
+    nz = ceffz.size
+    ceffmin = min(ceffz[0], ceffz.min())
+    k_max_full = omega / ceffmin
+    k_gnd = omega / ceffz[0]
+    dkk = (k_max_full ** 2 - k_gnd ** 2) / 100.0
+    kk = k_gnd ** 2
+    steps = []
+    if dkk > 1.0e-10:
+        keff2 = (omega / ceffz) ** 2
+        while kk < k_max_full ** 2:
+            term = np.abs(kk - keff2)
+            stop = np.nonzero(term <= dkk)[0]
+            first = int(stop[0]) if stop.size else nz - 1
+            if dkk >= 1.0:
+                integral = 0.0   # `while (wkbTerm > dkk)` with wkbTerm = 1 never runs
+            else:
+                # sequential left-to-right sum, as the reference loop accumulates
+                integral = float(np.cumsum(dz * np.sqrt(term[:first + 1]))[-1])
+            steps.append((kk, first, integral))
+            if integral >= 10.0:
+                break
+            kk = kk + dkk
+    return kk, steps
 
 
 def matrix(diag, dz):
