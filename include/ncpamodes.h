@@ -337,6 +337,24 @@ int ncpa_bb_pulse(ncpa_ctx_t *ctx, int n_sets, int n_ranges, int n_freq, const d
                   const double *spectrum, const double *f_vec, const double *ranges_m,
                   double max_celerity, int nfft, double *dev_wave);
 
+/* Allocation-free form of ncpa_bb_pulse: the same arguments, kernels and results, with
+ * every intermediate (the uploaded spectrum, frequencies and delays, the FFT input and
+ * output, the FFT's global work buffer when its ping-pong buffers do not fit in LDS) in one
+ * fixed device array of NCPA_PULSE_SCRATCH_BYTES per GPU, which the library holds from its
+ * first use on that GPU until the process ends.  It calls no hipMalloc, hipFree or other
+ * allocating routine.  No counterpart in the reference (its synthesis allocates per call):
+ * for callers that cannot allocate device memory, such as code under HIP graph capture,
+ * tight serving loops and harnesses that forbid hipMalloc.  A job whose intermediates
+ * (8 bytes x (2 n_freq + n_freq + n_ranges + 4 n_sets n_ranges nfft [+ 4 nfft n_sets n_ranges
+ * when 2 nfft complex doubles exceed the LDS of a block]), each part rounded up to 256 bytes)
+ * exceed the array returns NCPA_ERR_ARG with nothing launched.  Every call on one GPU uses
+ * that GPU's one array: calls on one context are ordered by its stream, but calls that
+ * overlap on one GPU (other contexts, streams or host threads) race. */
+#define NCPA_PULSE_SCRATCH_BYTES (32u << 20)
+int ncpa_bb_pulse_noalloc(ncpa_ctx_t *ctx, int n_sets, int n_ranges, int n_freq, const double *dev_H,
+                          const double *spectrum, const double *f_vec, const double *ranges_m,
+                          double max_celerity, int nfft, double *dev_wave);
+
 /* Source types of BroadbandPropagator::get_source_spectrum (BroadbandPropagator.cpp:244-417) */
 #define NCPA_SRC_IMPULSE  0
 #define NCPA_SRC_PULSE1   1

@@ -19,7 +19,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -166,7 +169,7 @@ __global__ void k_bb_arg(int n_tr, int nf, int nfft, const double *__restrict__ 
 
 // ------------------------------------------------------------------------------------
 // Stockham autosort FFT: one block per transform, ping-pong between two N-point buffers
-// (LDS when 2 N complex doubles fit, else a per-transform global scratch).  Stage with
+// (LDS when 2 N complex doubles fit, else a per-transform global work buffer).  Stage with
 // radix R and span Ns: butterfly j reads src[j + r N/R], twiddles by e^{sgn 2 pi i k r/(Ns R)}
 // (k = j mod Ns), does an R-point DFT, writes dst[(j/Ns) Ns R + k + q Ns].
 // ------------------------------------------------------------------------------------
@@ -255,12 +258,12 @@ __device__ void stockham_stage_generic(const cplx *src, cplx *dst, int N, int R,
 }
 
 __global__ void __launch_bounds__(256) k_fft(FftPlan plan, const double *__restrict__ in, double *__restrict__ out,
-                                             double *__restrict__ scratch, int use_lds, double sgn) {
+                                             double *__restrict__ work, int use_lds, double sgn) {
     extern __shared__ cplx lds[];
     const int N = plan.n;
     const size_t base = (size_t)blockIdx.x * N;
-    cplx *b0 = use_lds ? lds : (cplx *)scratch + 2 * base;
-    cplx *b1 = use_lds ? lds + N : (cplx *)scratch + 2 * base + N;
+    cplx *b0 = use_lds ? lds : (cplx *)work + 2 * base;
+    cplx *b1 = use_lds ? lds + N : (cplx *)work + 2 * base + N;
     for (int i = threadIdx.x; i < N; i += blockDim.x) b0[i] = make_double2(in[2 * (base + i)], in[2 * (base + i) + 1]);
     __syncthreads();
     int Ns = 1;
@@ -290,6 +293,43 @@ __global__ void k_bb_out(size_t n, const double *__restrict__ spec, double df, d
     if (i < n) wave[i] = 2.0 * (df * spec[2 * i]);
 }
 
+}  // namespace ncpa
+
+// ------------------------------------------------------------------------------------
+// Fixed device memory of ncpa_bb_pulse_noalloc: one array per GPU, held until the process
+// ends.  In host code `scratch` names a host-side stand-in, not device memory: host code
+// uses only the per-device address hipGetSymbolAddress returns (g_scratch_dev).
+// ------------------------------------------------------------------------------------
+namespace {
+constexpr size_t SCRATCH_DOUBLES = NCPA_PULSE_SCRATCH_BYTES / sizeof(double);
+constexpr int SCRATCH_MAX_DEVICES = 64;
+}  // namespace
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_dev[SCRATCH_MAX_DEVICES] = {};
+}  // namespace
+
+namespace ncpa {
+// Device address of `scratch` on ctx's device, looked up once per device with that device
+// current (each GPU holds its own copy of the array at its own address).
+int pulse_scratch_setup(ncpa_ctx *ctx, double **out) {
+    *out = nullptr;
+    if (ctx->device < 0 || ctx->device >= SCRATCH_MAX_DEVICES)
+        return set_error(ctx, NCPA_ERR_ARG, "ncpa_bb_pulse_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[ctx->device]) {
+        NCPA_HIP(ctx, hipSetDevice(ctx->device));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return hip_error(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "address of the fixed pulse array");
+        g_scratch_dev[ctx->device] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch_dev[ctx->device];
+    return NCPA_OK;
+}
 }  // namespace ncpa
 
 using namespace ncpa;
@@ -322,9 +362,9 @@ int make_plan(ncpa_ctx *ctx, int n, FftPlan &p) {
 
 namespace ncpa {
 // Sets the FFT kernel up for n-point transforms (its LDS attribute when 2 n complex doubles
-// need more than 64 KB) and reports the global scratch it needs: 0 when the ping-pong
+// need more than 64 KB) and reports the global work buffer it needs: 0 when the ping-pong
 // buffers fit in LDS, else 4 n batch doubles.
-int bb_prepare_fft(ncpa_ctx *ctx, int n, int batch, size_t *scratch_doubles) {
+int bb_prepare_fft(ncpa_ctx *ctx, int n, int batch, size_t *work_doubles) {
     FftPlan plan;
     int rc = make_plan(ctx, n, plan);
     if (rc) return rc;
@@ -334,20 +374,25 @@ int bb_prepare_fft(ncpa_ctx *ctx, int n, int batch, size_t *scratch_doubles) {
     const bool use_lds = lds <= (size_t)max_lds;
     if (use_lds && lds > 65536)
         NCPA_HIP(ctx, hipFuncSetAttribute((const void *)k_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    *scratch_doubles = use_lds ? 0 : 4 * (size_t)n * batch;
+    *work_doubles = use_lds ? 0 : 4 * (size_t)n * batch;
     return NCPA_OK;
 }
 
-// batch unnormalised n-point FFTs with scratch from bb_prepare_fft (no allocation)
-int bb_launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, int sign, double *scratch) {
+// batch unnormalised n-point FFTs with the global work buffer from bb_prepare_fft (no
+// allocation); timed: under the context's kernel timing
+int bb_launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, int sign, double *work,
+                  bool timed) {
     FftPlan plan;
     int rc = make_plan(ctx, n, plan);
     if (rc) return rc;
     const size_t lds = 2 * (size_t)n * sizeof(cplx);
-    const bool use_lds = scratch == nullptr;
-    {
+    const bool use_lds = work == nullptr;
+    if (timed) {
         TimedLaunch tl(ctx, "fft");
-        hipLaunchKernelGGL(k_fft, dim3(batch), dim3(256), use_lds ? lds : 0, ctx->stream, plan, in, out, scratch,
+        hipLaunchKernelGGL(k_fft, dim3(batch), dim3(256), use_lds ? lds : 0, ctx->stream, plan, in, out, work,
+                           use_lds ? 1 : 0, (double)(sign < 0 ? -1.0 : 1.0));
+    } else {
+        hipLaunchKernelGGL(k_fft, dim3(batch), dim3(256), use_lds ? lds : 0, ctx->stream, plan, in, out, work,
                            use_lds ? 1 : 0, (double)(sign < 0 ? -1.0 : 1.0));
     }
     hipError_t e = hipGetLastError();
@@ -386,13 +431,13 @@ int launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, i
     size_t need = 0;
     int rc = bb_prepare_fft(ctx, n, batch, &need);
     if (rc) return rc;
-    double *scratch = nullptr;
+    double *work = nullptr;
     if (need) {
-        scratch = pool_alloc<double>(ctx, need);
-        if (!scratch) return set_error(ctx, NCPA_ERR_ALLOC, "fft: scratch allocation failed");
+        work = pool_alloc<double>(ctx, need);
+        if (!work) return set_error(ctx, NCPA_ERR_ALLOC, "fft: work buffer allocation failed");
     }
-    rc = bb_launch_fft(ctx, in, out, n, batch, sign, scratch);
-    ctx->pool.put(scratch);
+    rc = bb_launch_fft(ctx, in, out, n, batch, sign, work, true);
+    ctx->pool.put(work);
     return rc;
 }
 
@@ -552,6 +597,55 @@ int ncpa_bb_pulse(ncpa_ctx_t *ctx, int n_sets, int n_ranges, int n_freq, const d
     ctx->pool.put(d_spec);
     return rc;
 }
+
+int ncpa_bb_pulse_noalloc(ncpa_ctx_t *ctx, int n_sets, int n_ranges, int n_freq, const double *dev_H,
+                          const double *spectrum, const double *f_vec, const double *ranges_m,
+                          double max_celerity, int nfft, double *dev_wave) {
+    if (!ctx) return NCPA_ERR_ARG;
+    if (n_sets < 1 || n_ranges < 1 || n_freq < 2 || !dev_H || !spectrum || !f_vec || !ranges_m || !dev_wave ||
+        !(max_celerity > 0.0))
+        return set_error(ctx, NCPA_ERR_ARG, "ncpa_bb_pulse_noalloc: bad arguments");
+    if (nfft < n_freq) return set_error(ctx, NCPA_ERR_ARG, "fft too short (i.e. NFFT < Nfreq)");
+    // the same kernels and launches as ncpa_bb_pulse; the intermediates are carved out of
+    // the fixed array: S [2 n_freq] | f [n_freq] | t0 [n_ranges] | arg | spec | FFT work,
+    // each segment 256-byte aligned
+    const size_t n_tr = (size_t)n_sets * n_ranges, n = n_tr * (size_t)nfft;
+    size_t work = 0;
+    int rc = bb_prepare_fft(ctx, nfft, (int)std::min<size_t>(n_tr, INT_MAX), &work);
+    if (rc) return rc;
+    auto seg = [](size_t doubles) { return (doubles + 31) / 32 * 32; };
+    const size_t o_f = seg(2 * (size_t)n_freq), o_t0 = o_f + seg((size_t)n_freq), o_arg = o_t0 + seg((size_t)n_ranges);
+    const size_t o_spec = o_arg + seg(2 * n), o_work = o_spec + seg(2 * n), need = o_work + seg(work);
+    if (n_tr > (size_t)INT_MAX || need > SCRATCH_DOUBLES)
+        return set_error(ctx, NCPA_ERR_ARG,
+                         "ncpa_bb_pulse_noalloc: the job needs " + std::to_string(need * sizeof(double)) +
+                             " bytes of fixed device memory, above the maximum of " +
+                             std::to_string((size_t)NCPA_PULSE_SCRATCH_BYTES));
+    double *base = nullptr;
+    if ((rc = pulse_scratch_setup(ctx, &base))) return rc;
+    NCPA_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<double> t0(n_ranges);
+    for (int i = 0; i < n_ranges; ++i) t0[i] = ranges_m[i] / max_celerity;   // ModalBroadbandPropagator.cpp:211
+    const double df = f_vec[1] - f_vec[0];                                       // BroadbandPropagator.cpp:581
+    double *d_S = base, *d_f = base + o_f, *d_t0 = base + o_t0, *d_arg = base + o_arg, *d_spec = base + o_spec;
+    double *d_work = work ? base + o_work : nullptr;
+    // stream-ordered after this context's earlier use of the array; every element of arg,
+    // spec and the FFT work buffer is written before it is read, so nothing is cleared.  The
+    // host arrays are pageable: HIP stages them before hipMemcpyAsync returns (as for
+    // ncpa_bb_pulse), so t0 may go out of scope at the end of this call.
+    NCPA_HIP(ctx, hipMemcpyAsync(d_S, spectrum, sizeof(double) * 2 * n_freq, hipMemcpyHostToDevice, ctx->stream));
+    NCPA_HIP(ctx, hipMemcpyAsync(d_f, f_vec, sizeof(double) * n_freq, hipMemcpyHostToDevice, ctx->stream));
+    NCPA_HIP(ctx, hipMemcpyAsync(d_t0, t0.data(), sizeof(double) * n_ranges, hipMemcpyHostToDevice, ctx->stream));
+    bb_launch_arg(ctx->stream, (int)n_tr, n_freq, nfft, dev_H, d_S, d_f, d_t0, n_ranges, d_arg);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(ctx, e, "k_bb_arg launch");
+    if ((rc = bb_launch_fft(ctx, d_arg, d_spec, nfft, (int)n_tr, -1, d_work, false))) return rc;   // FFTW_FORWARD
+    bb_launch_out(ctx->stream, n, d_spec, df, dev_wave);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(ctx, e, "k_bb_out launch");
+    return NCPA_OK;
+}
+
 
 // BroadbandPropagator::get_source_spectrum (BroadbandPropagator.cpp:244-417)
 int ncpa_bb_source(ncpa_ctx_t *ctx, int kind, const double *f_vec, int n_freq, int nfft, double f_center,

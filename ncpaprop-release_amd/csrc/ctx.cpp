@@ -1,6 +1,7 @@
 // Context lifecycle, profile upload, allocator and timing.
 #include "ctx.h"
 #include "common.h"
+#include "internal.h"
 
 #include <cmath>
 #include <cstring>
@@ -94,6 +95,12 @@ int ncpa_ctx_create(int device, void *hip_stream, ncpa_ctx_t **out) {
     if (!ctx) return NCPA_ERR_ALLOC;
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess) {
+        delete ctx;
+        return NCPA_ERR_HIP;
+    }
+    // the device address of ncpa_bb_pulse_noalloc's fixed array, once per device
+    double *fixed = nullptr;
+    if (ncpa::pulse_scratch_setup(ctx, &fixed) != NCPA_OK) {
         delete ctx;
         return NCPA_ERR_HIP;
     }

@@ -107,9 +107,12 @@ void bb_launch_transfer(hipStream_t st, int n_bins, int n_freq, const int *beg, 
                         const double *fac, double *H);
 void bb_launch_arg(hipStream_t st, int n_tr, int n_freq, int nfft, const double *H, const double *S,
                    const double *f, const double *t0, int n_ranges, double *arg);
-int bb_prepare_fft(ncpa_ctx *ctx, int n, int batch, size_t *scratch_doubles);
-int bb_launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, int sign, double *scratch);
+int bb_prepare_fft(ncpa_ctx *ctx, int n, int batch, size_t *work_doubles);
+int bb_launch_fft(ncpa_ctx *ctx, const double *in, double *out, int n, int batch, int sign, double *work,
+                  bool timed = true);
 void bb_launch_out(hipStream_t st, size_t n, const double *spec, double df, double *wave);
+// device address of ncpa_bb_pulse_noalloc's fixed array on ctx's device (looked up once per device)
+int pulse_scratch_setup(ncpa_ctx *ctx, double **out);
 
 }  // namespace ncpa
 

@@ -16,6 +16,7 @@ NCPA_ERR_ALLOC = -3
 NCPA_ERR_MAX_MODES = -4
 NCPA_ERR_STATE = -5
 NCPA_MAX_MODES = 4000
+NCPA_PULSE_SCRATCH_BYTES = 32 << 20
 
 NCPA_PREC_FP64 = 0
 NCPA_PREC_FP32 = 1
@@ -46,7 +47,8 @@ EXPORTS = [
     "ncpa_wmod_solve", "ncpa_wmod_eigen",
     "ncpa_dispersion_write", "ncpa_dispersion_read", "ncpa_dispersion_info", "ncpa_dispersion_arrays",
     "ncpa_dispersion_free",
-    "ncpa_fft_batch", "ncpa_kpert", "ncpa_bb_transfer", "ncpa_bb_pulse", "ncpa_bb_source",
+    "ncpa_fft_batch", "ncpa_kpert", "ncpa_bb_transfer", "ncpa_bb_pulse", "ncpa_bb_pulse_noalloc",
+    "ncpa_bb_source",
     "ncpa_bb_plan_create", "ncpa_bb_plan_set_problems", "ncpa_bb_plan_run", "ncpa_bb_plan_result",
     "ncpa_bb_plan_tuning", "ncpa_bb_plan_destroy",
 ]
@@ -154,6 +156,8 @@ def load():
                                        dp, vp]),
         "ncpa_bb_pulse": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, dp, dp, dp, C.c_double, C.c_int,
                                     vp]),
+        "ncpa_bb_pulse_noalloc": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, dp, dp, dp, C.c_double, C.c_int,
+                                            vp]),
         "ncpa_bb_source": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_int, C.c_double, dp, C.c_int, dp,
                                      dp]),
         "ncpa_bb_plan_create": (C.c_int, [vp, P(Grid), P(Problem), P(BbJob), P(vp)]),

@@ -52,3 +52,27 @@ def test_null_arguments_return_codes():
     mx, rms, cnt = ctypes.c_double(), ctypes.c_double(), ctypes.c_size_t()
     assert lib.ncpa_field_compare(None, None, None, 6, 10, -140.0, ctypes.byref(mx), ctypes.byref(rms),
                                   ctypes.byref(cnt)) == _lib.NCPA_ERR_ARG
+
+
+def test_noalloc_pulse_calls_no_allocator():
+    """ncpa_bb_pulse_noalloc and the helpers it calls name no allocating routine, and host code
+    reaches the fixed array only through the address hipGetSymbolAddress returned."""
+    src = open(os.path.join(ROOT, "ncpaprop-release_amd", "csrc", "broadband.hip")).read()
+
+    def body(signature):
+        i = src.index(signature)
+        j = src.index("\n}\n", i)
+        return src[i:j]
+
+    forbidden = ("pool_alloc", "hipMalloc", "hipFree", "hipHostMalloc", "hipExtMalloc", "hipMallocAsync",
+                 "hipEventCreate", "TimedLaunch", "upload_tmp", "launch_fft(")
+    for sig in ("int ncpa_bb_pulse_noalloc(", "int pulse_scratch_setup(", "int bb_prepare_fft(",
+                "void bb_launch_arg(", "void bb_launch_out("):
+        b = body(sig)
+        hits = [f for f in forbidden if re.search(r"(?<![A-Za-z0-9_])" + re.escape(f), b)]
+        assert not hits, (sig, hits)
+    b = body("int ncpa_bb_pulse_noalloc(")
+    assert "bb_launch_fft(ctx, d_arg, d_spec, nfft, (int)n_tr, -1, d_work, false)" in b   # untimed form
+    # host code names the device array only inside hipGetSymbolAddress
+    uses = [ln for ln in src.splitlines() if re.search(r"\bscratch\b", ln) and "//" not in ln.split("scratch")[0]]
+    assert all("__device__" in ln or "HIP_SYMBOL(scratch)" in ln for ln in uses), uses
