@@ -243,18 +243,25 @@ def tl2d_rows(nz):
 
 
 def tl2d(k_pert, psi_rows, psi_s, dr, n_r, lossless=False, chunk=1024):
-    """ModeSolver::getTLoss2D field (ModeSolver.cpp:524-544): p[r, row]."""
+    """ModeSolver::getTLoss2D field (ModeSolver.cpp:524-544) at r = dr, 2 dr, ..: p[r, row]."""
+    r = (np.arange(n_r) + 1) * dr
+    return r, tl2d_at(k_pert, psi_rows, psi_s, r, lossless, chunk)
+
+
+def tl2d_at(k_pert, psi_rows, psi_s, r, lossless=False, chunk=1024):
+    """The getTLoss2D sum at the ranges r [m] (any subset of the reference's range steps):
+    p[r, row] = 1000 expov8pi sum_m psi_m(zs) psi_m(row) e^{i k_m r} / sqrt(k_m) / sqrt(r)."""
     k = np.asarray(k_pert, dtype=np.complex128)
     kk = k.real if lossless else k
-    r = (np.arange(n_r) + 1) * dr
-    out = np.empty((n_r, psi_rows.shape[0]), dtype=np.complex128)
+    r = np.asarray(r, dtype=np.float64)
+    out = np.empty((r.size, psi_rows.shape[0]), dtype=np.complex128)
     c = np.asarray(psi_s) / np.sqrt(kk.astype(np.complex128))
-    for s in range(0, n_r, chunk):
+    for s in range(0, r.size, chunk):
         rr = r[s:s + chunk]
         E = np.exp(1j * np.outer(rr, kk)) * c[None, :]
         out[s:s + chunk] = E @ psi_rows.T
     out *= (1000.0 * EXPOV8PI / np.sqrt(r))[:, None]
-    return r, out
+    return out
 
 
 def tl_db(p):
