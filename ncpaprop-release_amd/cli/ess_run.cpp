@@ -178,7 +178,11 @@ FILE *open_or_throw(const std::string &path, const char *mode) {
 
 // --use_attn_file: "altitude[km] attenuation[Np/m]" lines interpolated onto the profile nodes
 // with the Steffen spline (Atmosphere1D::read_attenuation_from_file, Atmosphere1D.cpp:307-424).
-// Outside the file's altitude range the end values are held.
+// A table that starts above the profile's lowest node or ends below its highest is first
+// extended by a node at that altitude carrying the table's end value (the reference's intent;
+// its copies move bytes instead of doubles, Atmosphere1D.cpp:368-392), so the end values are
+// held outside the table and the Steffen slopes at the table's end nodes are those of the
+// extended table.
 std::vector<double> attenuation_at_nodes(const std::string &path, const std::vector<double> &z_nodes_km) {
     std::ifstream in(path);
     if (!in) throw Fail("cannot open attenuation file " + path);
@@ -199,6 +203,16 @@ std::vector<double> attenuation_at_nodes(const std::string &path, const std::vec
         aa.push_back(a);
     }
     if (za.size() < 2) throw Fail("attenuation file " + path + " needs at least two lines");
+    for (size_t i = 1; i < za.size(); ++i)
+        if (!(za[i] > za[i - 1])) throw Fail("attenuation file " + path + ": altitudes must increase");
+    if (!z_nodes_km.empty() && za.front() > z_nodes_km.front()) {
+        za.insert(za.begin(), z_nodes_km.front());
+        aa.insert(aa.begin(), aa.front());
+    }
+    if (!z_nodes_km.empty() && za.back() < z_nodes_km.back()) {
+        za.push_back(z_nodes_km.back());
+        aa.push_back(aa.back());
+    }
     std::vector<double> out(z_nodes_km.size());
     auto yf = [&aa](int k) { return aa[k]; };
     for (size_t i = 0; i < out.size(); ++i) out[i] = ncpa::steffen_eval(za.data(), (int)za.size(), yf, z_nodes_km[i]);

@@ -223,6 +223,40 @@ class Steffen:
         return np.where(inside, d1, 0.0)
 
 
+def read_attenuation_file(path):
+    """Atmosphere1D::read_attenuation_from_file (Atmosphere1D.cpp:307-367): lines that do not
+    start with '#' hold 'altitude[km] attenuation[Np/m]', split at blanks or commas.
+    Returns (z_km, alpha)."""
+    zs, aa = [], []
+    with open(path) as fh:
+        for line in fh:
+            t = line.strip()
+            if not t or t[0] == "#":
+                continue
+            f = [x for x in t.replace(",", " ").split() if x]
+            if len(f) != 2:
+                raise ValueError("Atmosphere1D - Error parsing attenuation line:\n" + line +
+                                 "Must be formatted as:\naltitude  attenuation")
+            zs.append(float(f[0]))
+            aa.append(float(f[1]))
+    return np.array(zs), np.array(aa)
+
+
+def attenuation_file_nodes(path, z_nodes_km):
+    """_ALPHA_ at the profile nodes from an attenuation file (Atmosphere1D.cpp:368-414): the
+    table is extended by a node at the profile's lowest (highest) altitude carrying its first
+    (last) value when it starts above (ends below) the profile, then Steffen-interpolated at
+    the nodes.  (The reference's extension copies bytes instead of doubles; the build
+    implements the evident intent, which holds the end values.)"""
+    z, a = read_attenuation_file(path)
+    z_nodes_km = np.asarray(z_nodes_km, dtype=np.float64)
+    if z[0] > z_nodes_km[0]:
+        z, a = np.concatenate([[z_nodes_km[0]], z]), np.concatenate([[a[0]], a])
+    if z[-1] < z_nodes_km[-1]:
+        z, a = np.concatenate([z, [z_nodes_km[-1]]]), np.concatenate([a, [a[-1]]])
+    return Steffen(z, a).get(z_nodes_km)
+
+
 # --------------------------------------------------------------------------------------
 # Sutherland-Bass attenuation (src/atmosphere/AtmosphericModel.cpp:86-219)
 # --------------------------------------------------------------------------------------

@@ -310,6 +310,23 @@ def format_phase_speeds(freq, k_pert):
                    for j, kp in enumerate(k_pert))
 
 
+def modal_starter(psi, i_s, k_pert, dz, freq):
+    """ESSModeSolver::getModalStarter (ESSModeSolver.cpp:670-723): rows j = 0, z_cnd, 2 z_cnd ..
+    with z_cnd = int((340/f)/10/dz); starter(z_j) = pi sqrt(k0) sum_m psi_m(zs) psi_m(z_j) /
+    sqrt(Re k_m), k0 = 2 pi f/340.  Returns (z [m], starter [rows]) (real: the reference's
+    complex sum has a zero imaginary part)."""
+    k0 = 2 * PI * freq / 340.0
+    z_cnd = int((340.0 / freq) / 10 / dz)
+    rows = np.arange(0, psi.shape[0], z_cnd)
+    s = psi[rows] @ (psi[i_s] / np.sqrt(np.real(k_pert)))
+    return rows * dz, s * PI * math.sqrt(k0)
+
+
+def format_modal_starter(z_m, starter):
+    """'%10.3f   %16.12e   %16.12e' per row and a closing blank line (ESSModeSolver.cpp:718-720)."""
+    return "".join("%10.3f   %16.12e   %16.12e\n" % (z / 1000.0, v, 0.0) for z, v in zip(z_m, starter)) + "\n"
+
+
 def group_speeds(psi, ceffz, dz, freq, k_pert):
     """writePhaseAndGroupSpeeds (ESSModeSolver.cpp:735-745): (v_phase, v_group)."""
     omega = 2 * PI * freq
@@ -328,13 +345,14 @@ class Solution:
 
 def solve(profile_path=None, nodes=None, freq=0.1, azimuth=90.0, Nz=20000, maxheight_km=150.0,
           zground_km=0.0, zs_km=0.0, zr_km=0.0, turnoff_wkb=False, lamb_wave_bc=False,
-          wvnum_filter=None, alpha_freq=None, vectors=True, grid=None):
+          wvnum_filter=None, alpha_freq=None, vectors=True, grid=None, attn_nodes=None):
     """Modes for one (profile, f, azimuth), as ESSModeSolver::solve (ESSModeSolver.cpp:320-382)."""
     if nodes is None:
         nodes = AtmosphereNodes(read_profile(profile_path))
     if grid is None:
         grid = EssGrid(nodes, Nz, maxheight_km, zground_km)
-    grid.set_attenuation(freq if alpha_freq is None else alpha_freq)
+    # attn_nodes: _ALPHA_ at the profile nodes from --use_attn_file (ESSModeSolver.cpp:224-228)
+    grid.set_attenuation(freq if alpha_freq is None else alpha_freq, attn_nodes)
     zs, zr = zs_km * 1000.0, zr_km * 1000.0
     adm = admittance(grid, lamb_wave_bc)
     diag, ceffz, k_min, k_max, used = modal_trace(grid, freq, azimuth, zs, zr, turnoff_wkb, adm)
