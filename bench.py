@@ -31,9 +31,15 @@ reference's rule) with a Gaussian-windowed 0.5 Hz source given as a spectrum.
                     [--workload field|broadband|serving|precision|wmod|broadband-wmod]
                     [--schedule pipeline|streams] [--streams S] [--no-cpu]
 
-Prints one JSON line.  cpu_baseline: the FP64 oracle (scipy LAPACK bisection + inverse
-iteration, numpy sums; one thread per process) on a bounded sample of the same workload, run
-in parallel worker processes on this host's cores and scaled to the workload's unit.
+The default run (field workload) then adds the rest of the metric: "latency" (p50/p99 of the
+continuous-batching server at the production precision under Poisson arrivals at 100 and 500
+requests/s) and "broadband_job" (the batched ModBB job and its CPU comparison);
+--headline-only skips them.
+
+Prints one JSON line.  cpu_baseline (field and broadband): the oracle's C restatement of the
+reference loops (oracle/ess_port.c, gcc -O2, FP64) on the host cores that go with one GPU
+(16), one solve per thread, on a bounded sample of the workload scaled to its unit; the
+wide-angle workload times the Python oracle in worker processes instead.
 """
 import argparse
 import ctypes
@@ -201,12 +207,84 @@ def cpu_baseline(workload, n_workers):
     return out
 
 
+HOST_CPU_SHARE = 16   # host cores that go with one GPU on the GPU boxes (their stated CPU share)
+
+
 def cpu_cores():
+    """Threads for the CPU baseline: every core this process may use, capped at the host CPU
+    share that goes with one GPU on the GPU boxes (nproc there counts the whole host)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, min(HOST_CPU_SHARE, n))
+
+
+def host_info():
+    """nproc, the affinity count and the CPU model of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"nproc": os.cpu_count(), "affinity": aff, "cpu_model": model}
+
+
+# ------------------------------------------------------------------------------------------
+# CPU baseline (the oracle's C restatement of the reference loops, one job per thread)
+# ------------------------------------------------------------------------------------------
+PORT_FIELD_RANGES = 4     # 2-D field ranges timed per solve (of NR; scaled by NR / this)
+
+
+def port_baseline(workload, n_threads):
+    """Times oracle/_build/ess_port (FP64 C, gcc -O2: Sturm bisection + inverse iteration,
+    doPerturb, getTLoss1D / getTLoss2D with a complex exp and sqrt per term, as the reference
+    loops) on n_threads threads, one job per thread; returns the aggregate rate and the sample."""
+    port = os.path.join(ROOT, "oracle", "_build", "ess_port")
+    if not os.path.exists(port):
+        sys.stderr.write("bench.py: %s is not built (run __graft_entry__.build())\n" % port)
+        return None
+    tmp = tempfile.mkdtemp(prefix="ncpa_cpu_")
+    if workload == "field":
+        paths = write_profiles(tmp)
+        jobs = ["%s:%g:%g" % (paths[i % len(paths)], AZIMUTHS[(i * 3) % len(AZIMUTHS)], FREQ)
+                for i in range(n_threads)]
+        opts = ["--Nz", str(NZ), "--nr", str(NR), "--dr", str(MAXRANGE_M / NR), "--rows", str(FIELD_ROWS),
+                "--stride", str(FIELD_STRIDE), "--field_ranges", str(PORT_FIELD_RANGES)]
+    else:
+        path = write_profiles(tmp, [0])[0]
+        # frequencies spread uniformly over the band so the mean cost matches the band's
+        idx = [int((i + 0.5) * BB_NF / n_threads) + 1 for i in range(n_threads)]
+        jobs = ["%s:%g:%.17g" % (path, AZIMUTHS[i % len(AZIMUTHS)], fi * BB_FSTEP) for i, fi in enumerate(idx)]
+        # compute_modal_sum at the 16 receiver ranges: one complex exp and sqrt per (mode,
+        # range), the per-term work of the 1-D TL loop at 16 ranges
+        opts = ["--Nz", str(NZ), "--nr", str(len(BB_RANGES_KM)), "--dr", "50000"]
+    cmd = [port, "bench", "--threads", str(n_threads)] + opts
+    for j in jobs:
+        cmd += ["--job", j]
+    t = time.perf_counter()
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    wall = time.perf_counter() - t
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        return None
+    out = json.loads(r.stdout)
+    per = []
+    for j in out["jobs"]:
+        setup, eig, tl1d, tl2d = j["times"]
+        per.append(setup + eig + tl1d + (tl2d * NR / PORT_FIELD_RANGES if workload == "field" else 0.0))
+    mean = sum(per) / len(per)
+    parts = [sum(j["times"][k] for j in out["jobs"]) / len(per) for k in range(4)]
+    return {"workers": n_threads, "wall_s": wall, "mean_solve_s": mean, "rate": n_threads / mean,
+            "parts": parts, "jobs": len(per), "host": host_info()}
 
 
 # ------------------------------------------------------------------------------------------
@@ -443,9 +521,12 @@ def field_main(args, cpu):
     # headline throughput: the two-stage pipeline (default) or independent batches in flight
     # on several streams
     res = run_field(lanes[:1], args, chosen)
+    plain_multi = None
     if args.schedule == "pipeline":
         contractor = ncpamodes.Context(0)
         multi = run_field_pipelined(ctx, contractor, problems, grid, args, chosen)
+        # the FP64 contraction in the same two-stage schedule: the precision's own speed-up
+        plain_multi = run_field_pipelined(ctx, contractor, problems, grid, args, 0) if chosen != 0 else multi
         contractor.close()
     else:
         multi = run_field(lanes, args, chosen) if len(lanes) > 1 else res
@@ -478,7 +559,7 @@ def field_main(args, cpu):
     else:
         avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
         achieved = FLOPS_PER_STURM_STEP * res["sturm_steps"] / avg_s / 1e12
-        roof = {"kernel": "k_modes<2>", "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f64"],
+        roof = {"kernel": "k_modes<2>", "bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f64"],
                 "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"], "traffic": measured_traffic("k_modes<2>"),
                 "algorithmic": "3 FP64 flops per Sturm step (row x sweep); FP64 VALU, MI355X FP64 peak"}
     modes_avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
@@ -517,7 +598,12 @@ def field_main(args, cpu):
         "tl_check_db": {PREC_NAMES[p]: {"max": e[0], "rms": e[1], "points": e[2]} for p, e in errs.items()},
         "tl_check_scope": "all %d solves of the step, every field point with TL_fp64 > -140 dB" % P,
         "fp64_plain": {"value": P * args.steps / plain["elapsed"], "unit": "solves/s",
-                       "ms_per_step": plain["elapsed"] / args.steps * 1e3},
+                       "ms_per_step": plain["elapsed"] / args.steps * 1e3,
+                       "note": "FP64 contraction, one stream (compare single_stream)"},
+        "fp64_plain_pipelined": None if plain_multi is None else {
+            "value": P * args.steps / plain_multi["elapsed"], "unit": "solves/s",
+            "ms_per_step": plain_multi["elapsed"] / args.steps * 1e3,
+            "note": "FP64 contraction in the headline's two-stage schedule (compare value)"},
         "single_stream": {"value": P * args.steps / res["elapsed"], "unit": "solves/s",
                           "ms_per_step": res["elapsed"] / args.steps * 1e3,
                           "note": "stage split and roofline are measured in this one-stream region"},
@@ -527,11 +613,15 @@ def field_main(args, cpu):
                     "tflops": FLOPS_PER_STURM_STEP * res["sturm_steps"] / modes_avg_s / 1e12},
         "cpu_baseline": None if cpu is None else {
             "value": cpu["rate"], "unit": "solves/s", "cores": cpu["workers"], "kind": "port",
-            "sample": "%d concurrent 1-thread processes, each one of the benchmark's 1 Hz solves "
-                      "(FP64 oracle: scipy LAPACK stebz/stein eigenpairs, numpy 1-D TL at 10000 ranges, "
-                      "2-D field on 1500 depths x 100 ranges scaled x100 to 10000 ranges); "
-                      "mean %.1f s per solve, %.1f s wall" % (cpu["workers"], cpu["mean_solve_s"],
-                                                               cpu["wall_s"]),
+            "sample": "the oracle's C restatement of the reference loops (gcc -O2, FP64: Sturm bisection + "
+                      "inverse iteration, doPerturb, getTLoss1D at all 10000 ranges, getTLoss2D with a complex "
+                      "exp and sqrt per (range, depth, mode) term on 1500 depths x %d ranges, scaled x%d to "
+                      "10000 ranges) on %d threads, one of the benchmark's 1 Hz solves per thread; mean "
+                      "%.1f s per solve (eigen %.2f s, 1-D TL %.2f s, 2-D field %.0f s), %.1f s wall"
+                      % (PORT_FIELD_RANGES, NR // PORT_FIELD_RANGES, cpu["workers"], cpu["mean_solve_s"],
+                         cpu["parts"][1], cpu["parts"][2], cpu["parts"][3] * NR / PORT_FIELD_RANGES,
+                         cpu["wall_s"]),
+            "host": cpu["host"],
         },
     }
     return line
@@ -608,20 +698,23 @@ def broadband_main(args, cpu):
         "roofline": {"kernel": "k_modes<%d> (%s)" % (tuning["ms_passes"], "one block per work-list slot"
                                                      if tuning["grid_blocks"] == 0 else
                                                      "persistent, %d blocks" % tuning["grid_blocks"]),
-                     "bound": "mfma",
+                     "bound": "valu",
                      "achieved": achieved, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS["f64"],
                      "traffic": measured_traffic("k_modes<2> (8192-problem batch)"),
-                     "algorithmic": "3 FP64 flops per Sturm step (row x sweep); FP64 VALU"},
+                     "algorithmic": "3 FP64 flops per Sturm step (row x sweep); FP64 VALU (peak: the FP64 "
+                                    "vector rate, equal to the FP64 MFMA rate on MI355X)"},
         "cpu_baseline": None,
     }
     if cpu is not None:
-        per_job_s = cpu["mean_solve_s"] * P / cpu["workers"] + cpu["fft_s"]
         line["cpu_baseline"] = {
-            "value": P / per_job_s, "unit": "dispersion solves/s", "cores": cpu["workers"], "kind": "port",
-            "sample": "%d concurrent 1-thread processes, each one dispersion solve at a frequency spread "
-                      "over the band (FP64 oracle eigenpairs + compute_modal_sum at 16 ranges), plus the "
-                      "128 NFFT-%d pulse FFTs once; scaled to the %d-solve job" % (cpu["workers"], nfft, P)}
+            "value": cpu["rate"], "unit": "dispersion solves/s", "cores": cpu["workers"], "kind": "port",
+            "sample": "the oracle's C restatement of the reference loops (gcc -O2, FP64) on %d threads, one "
+                      "dispersion solve per thread at frequencies spread uniformly over the band (Sturm "
+                      "bisection + inverse iteration, doPerturb, a complex exp and sqrt per (mode, range) "
+                      "term at 16 ranges); the 128 pulse FFTs of a job (~1 ms on the GPU) are not timed; "
+                      "mean %.2f s per solve, %.1f s wall" % (cpu["workers"], cpu["mean_solve_s"], cpu["wall_s"]),
+            "host": cpu["host"]}
         line["speedup_vs_cpu_all_cores"] = value / line["cpu_baseline"]["value"]
     return line
 
@@ -701,7 +794,7 @@ def wmod_main(args, cpu):
         "stage_ms": {k: v[0] for k, v in fams.items()},
         "ess_same_batch": {"solves_per_s": P * args.steps / ess_elapsed,
                            "wide_over_ess_time": elapsed / ess_elapsed},
-        "roofline": {"kernel": "k_wmodes<2, 256, true>", "bound": "mfma", "achieved": achieved,
+        "roofline": {"kernel": "k_wmodes<2, 256, true>", "bound": "valu", "achieved": achieved,
                      "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"],
                      "traffic": measured_traffic("k_wmodes<2, 256, true>"),
                      "algorithmic": "6 FP64 flops per row per sweep (the row value a_i(k) = (P - kQ)^2 "
@@ -787,7 +880,7 @@ def broadband_wmod_main(args):
                                "%d; eager (one batched solve per job)" % nfft,
                    "global_batch": P, "modes_per_step": int(total_modes), "parallelism": "single GPU"},
         "stage_ms": {k: v[0] for k, v in fams.items()},
-        "roofline": {"kernel": "k_wmodes<2, 128, true>", "bound": "mfma", "achieved": achieved,
+        "roofline": {"kernel": "k_wmodes<2, 128, true>", "bound": "valu", "achieved": achieved,
                      "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"],
                      "traffic": None, "algorithmic": "6 FP64 flops per row per sweep, as the wmod workload"},
         "cpu_baseline": None,
@@ -919,6 +1012,33 @@ def serving_main(args):
     }
 
 
+def serving_short():
+    """p50/p99 request latency of the continuous-batching server (production precision BF16x6F,
+    3 device contexts) on two short Poisson streams, and its capacity with the stream queued."""
+    from ncpamodes import serve
+    n_profiles = 64
+    srv = serve.Server(n_workers=3, precision=6, max_batch=64)
+    srv.warm(range(n_profiles))
+    warm = serve.make_requests(64, 1000.0, seed=99, n_profiles=n_profiles)
+    srv.serve(warm, realtime=False)
+    cap = serve.make_requests(1000, 500.0, seed=1, n_profiles=n_profiles)
+    for r in cap:
+        r.t_arrival = 0.0
+    capacity = len(cap) / srv.serve(cap, realtime=False)
+    out = {"unit": "ms", "capacity_per_s": capacity, "precision": "bf16x6f",
+           "workload": "Poisson arrivals; per request a seeded profile (jet amplitudes +-40%, heights +-5 km, "
+                       "64-profile pool resident), f log-uniform 0.05-2 Hz, azimuth U(0,360), max range "
+                       "U(200,1500) km, 80% 1-D TL / 20% 2-D field; 3 device contexts, batches <= 64"}
+    for rate, n in ((100.0, 400), (500.0, 1000)):
+        reqs = serve.make_requests(n, rate, seed=int(rate), n_profiles=n_profiles)
+        w = srv.serve(reqs, realtime=True)
+        st = serve.latency_stats(reqs)
+        st["offered_per_s"], st["achieved_per_s"], st["requests"] = rate, n / w, n
+        out["%g_per_s" % rate] = st
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -934,6 +1054,8 @@ def main():
                     help="field workload with --schedule streams: batches in flight on this many streams")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run the plain copy kernel over MIB MiB first (byte-counter calibration)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="field workload: skip the latency and broadband legs that follow the headline")
     ap.add_argument("--cpu-worker", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker:
@@ -953,9 +1075,27 @@ def main():
     if args.workload == "broadband-wmod":
         print(json.dumps(broadband_wmod_main(args)), flush=True)
         return 0
-    cpu = None if args.no_cpu else cpu_baseline(args.workload, cpu_cores())
+    cores = cpu_cores()
+    if args.no_cpu:
+        cpu = None
+    elif args.workload in ("field", "broadband"):
+        cpu = port_baseline(args.workload, cores)
+    else:
+        cpu = cpu_baseline(args.workload, cores)
     if args.workload == "field":
         line = field_main(args, cpu)
+        if not args.headline_only:
+            # the rest of the metric in the same run: request latency under Poisson arrivals
+            # and the batched broadband job with its CPU comparison
+            line["latency"] = serving_short()
+            bb_args = argparse.Namespace(**vars(args))
+            bb_args.steps, bb_args.warmup = 5, 1
+            bb_cpu = None if args.no_cpu else port_baseline("broadband", cores)
+            bbl = broadband_main(bb_args, bb_cpu)
+            line["broadband_job"] = {k: bbl[k] for k in ("value", "unit", "ms_per_step", "config", "roofline",
+                                                         "cpu_baseline", "plan") if k in bbl}
+            if "speedup_vs_cpu_all_cores" in bbl:
+                line["broadband_job"]["speedup_vs_cpu_all_cores"] = bbl["speedup_vs_cpu_all_cores"]
     elif args.workload == "wmod":
         line = wmod_main(args, cpu)
     else:

@@ -60,6 +60,14 @@ int ncpa_ctx_set_timing(ncpa_ctx_t *ctx, int enable);
 int ncpa_ctx_kernel_time(ncpa_ctx_t *ctx, const char *kernel, double *total_ms, int *launches);
 int ncpa_ctx_reset_timing(ncpa_ctx_t *ctx);
 
+/* Device memory of the context (no counterpart in the reference): live = bytes its
+ * allocator has handed out (requests rounded up to 256 B) plus the buffers of
+ * ncpa_dev_alloc still allocated; peak = the largest live since the last call with
+ * reset_peak != 0 (which starts a new window at the current live); cached = bytes held in
+ * the allocator's free blocks.  Any pointer may be NULL. */
+int ncpa_ctx_memory(ncpa_ctx_t *ctx, int reset_peak, size_t *live_bytes, size_t *peak_bytes,
+                    size_t *cached_bytes);
+
 /* Ordering between contexts (no counterpart in the reference, whose solver is one
  * sequential loop; these let a caller pipeline the stages of consecutive batches over two
  * contexts, e.g. eigenpairs of batch i+1 on one stream while the 2-D field of batch i is
@@ -165,6 +173,14 @@ int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_
                    int n_problems, ncpa_modes_t **out);
 void ncpa_modes_free(ncpa_modes_t *modes);
 
+/* The mode count of every problem (host array of n_problems) without the modes: the grid,
+ * getModalTrace's interval and the Sturm counts at its ends, i.e. the number of modes
+ * ncpa_ess_solve would return (ESSModeSolver.cpp:366-377, whose printed count comes from its
+ * own, differently rounded, getNumberOfModes).  No counterpart in the reference: it sizes a
+ * batch's device memory before the batch is admitted. */
+int ncpa_ess_count(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems,
+                   int n_problems, int *counts);
+
 /* Mode counts per problem (host array of n_problems) and total. */
 int ncpa_modes_counts(ncpa_modes_t *modes, int *counts, int *total);
 /* Per-problem interval [k_min, k_max] actually used (host arrays, may be NULL). */
@@ -223,10 +239,12 @@ int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *modes, const double *dev_psi_row
  * to the staged operands (buffers of ctx's pool); ncpa_field2d_contract queues the
  * contraction on any context that has waited for an event recorded after the staging
  * (ncpa_event_*), writing dev_out as ncpa_field2d does; ncpa_field_stage_free returns the
- * buffers to the staging context, which must not run before the contraction has completed
- * (wait for an event recorded after it, or synchronise).  The eigenvector rows are read by
- * both calls (FP64 contraction: by the contraction only) and must stay valid until the
- * contraction completes. */
+ * buffers to the staging context's pool, whose stream first waits (on the device, no host
+ * wait) for the last contraction queued on another context, so its later work cannot reuse
+ * them early; free a stage from the staging context's host thread, after its last
+ * ncpa_field2d_contract call.  The eigenvector rows are read by both calls (FP64
+ * contraction: by the contraction only) and must stay valid until the contraction
+ * completes. */
 typedef struct ncpa_field_stage ncpa_field_stage_t;
 int ncpa_field2d_stage(ncpa_ctx_t *ctx, ncpa_modes_t *modes, const double *dev_psi_rows, int nrows,
                        double dr, int n_r, int lossless, int precision, ncpa_field_stage_t **stage);

@@ -18,8 +18,11 @@ void *DevicePool::get(size_t bytes) {
     auto it = free_.lower_bound(bytes);
     if (it != free_.end() && it->first <= 2 * bytes) {
         void *p = it->second;
-        live_[p] = it->first;
+        live_[p] = {it->first, bytes};
+        cached_ -= it->first;
         free_.erase(it);
+        live_req_ += bytes;
+        note();
         return p;
     }
     void *p = nullptr;
@@ -27,13 +30,16 @@ void *DevicePool::get(size_t bytes) {
         // drop cached blocks and retry once
         for (auto &kv : free_) (void)hipFree(kv.second);
         free_.clear();
+        cached_ = 0;
         (void)hipGetLastError();
         if (hipMalloc(&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
             return nullptr;
         }
     }
-    live_[p] = bytes;
+    live_[p] = {bytes, bytes};
+    live_req_ += bytes;
+    note();
     return p;
 }
 
@@ -41,15 +47,25 @@ void DevicePool::put(void *p) {
     if (!p) return;
     auto it = live_.find(p);
     if (it == live_.end()) return;
-    free_.emplace(it->second, p);
+    free_.emplace(it->second.first, p);
+    cached_ += it->second.first;
+    live_req_ -= it->second.second;
     live_.erase(it);
 }
+
+void DevicePool::add_external(size_t bytes) {
+    ext_ += bytes;
+    note();
+}
+
+void DevicePool::remove_external(size_t bytes) { ext_ -= std::min(ext_, bytes); }
 
 void DevicePool::release_all() {
     for (auto &kv : free_) (void)hipFree(kv.second);
     free_.clear();
     for (auto &kv : live_) (void)hipFree(kv.first);
     live_.clear();
+    live_req_ = cached_ = 0;
 }
 
 int set_error(ncpa_ctx *ctx, int code, const std::string &msg) {
@@ -219,6 +235,18 @@ int ncpa_dev_alloc(ncpa_ctx_t *ctx, size_t bytes, void **dev_ptr) {
     *dev_ptr = nullptr;
     NCPA_HIP(ctx, hipSetDevice(ctx->device));
     NCPA_HIP(ctx, hipMalloc(dev_ptr, bytes ? bytes : 1));
+    ctx->dev_allocs[*dev_ptr] = bytes;
+    ctx->pool.add_external(bytes);
+    return NCPA_OK;
+}
+
+int ncpa_ctx_memory(ncpa_ctx_t *ctx, int reset_peak, size_t *live_bytes, size_t *peak_bytes,
+                    size_t *cached_bytes) {
+    if (!ctx) return NCPA_ERR_ARG;
+    if (live_bytes) *live_bytes = ctx->pool.live();
+    if (peak_bytes) *peak_bytes = ctx->pool.peak();
+    if (cached_bytes) *cached_bytes = ctx->pool.cached();
+    if (reset_peak) ctx->pool.reset_peak();
     return NCPA_OK;
 }
 
@@ -227,6 +255,11 @@ int ncpa_dev_free(ncpa_ctx_t *ctx, void *dev_ptr) {
     if (!dev_ptr) return NCPA_OK;
     NCPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     NCPA_HIP(ctx, hipFree(dev_ptr));
+    auto it = ctx->dev_allocs.find(dev_ptr);
+    if (it != ctx->dev_allocs.end()) {
+        ctx->pool.remove_external(it->second);
+        ctx->dev_allocs.erase(it);
+    }
     return NCPA_OK;
 }
 

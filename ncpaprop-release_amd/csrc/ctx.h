@@ -1,6 +1,7 @@
 // Context, profile store, caching allocator and per-kernel timing (host side).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <utility>
@@ -24,16 +25,28 @@ struct ProfileDev {
 };
 
 // Stream-ordered caching allocator: every use of a cached block happens on the context
-// stream, so a block returned to the pool can be handed out again immediately.
+// stream (a block another context used is put back only behind a wait for that use, see
+// ncpa_field_stage_free), so a block returned to the pool can be handed out again at once.
+// Accounting: live() = bytes handed out (requests rounded to 256 B) plus the device buffers
+// the context allocated outside the pool (ncpa_dev_alloc), peak() its largest value since
+// reset_peak(), cached() = bytes held in free blocks.
 class DevicePool {
 public:
     ~DevicePool();
     void *get(size_t bytes);
     void put(void *p);
     void release_all();
+    void add_external(size_t bytes);
+    void remove_external(size_t bytes);
+    size_t live() const { return live_req_ + ext_; }
+    size_t peak() const { return peak_; }
+    size_t cached() const { return cached_; }
+    void reset_peak() { peak_ = live(); }
 private:
+    void note() { peak_ = std::max(peak_, live()); }
     std::multimap<size_t, void *> free_;
-    std::map<void *, size_t> live_;
+    std::map<void *, std::pair<size_t, size_t>> live_;   // block bytes, requested bytes
+    size_t live_req_ = 0, ext_ = 0, peak_ = 0, cached_ = 0;
 };
 
 }  // namespace ncpa
@@ -47,6 +60,7 @@ struct ncpa_ctx {
     std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> events;
     std::vector<ncpa::ProfileDev> profiles;
     ncpa::DevicePool pool;
+    std::map<void *, size_t> dev_allocs;   // ncpa_dev_alloc buffers and their sizes
     int ms_passes = 2;     // multisection passes of the eigenvalue kernel (autotuned by plans)
 };
 

@@ -1050,6 +1050,42 @@ int ncpa_ess_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_
     return NCPA_OK;
 }
 
+int ncpa_ess_count(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems, int n_problems,
+                   int *counts) {
+    if (!ctx || !problems || n_problems <= 0 || !counts)
+        return set_error(ctx, NCPA_ERR_ARG, "ncpa_ess_count: bad arguments");
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    NCPA_HIP(ctx, hipSetDevice(ctx->device));
+    const int Nz = grid->Nz, P = n_problems;
+    std::vector<ProbDev> hp(P);
+    int max_nodes = 0;
+    if ((rc = build_problems(ctx, grid, problems, P, hp.data(), &max_nodes))) return rc;
+    ProbDev *d_probs = pool_alloc<ProbDev>(ctx, P);
+    double *d_nodes = pool_alloc<double>(ctx, 2 * size_t(P) * max_nodes);
+    double *A = pool_alloc<double>(ctx, size_t(P) * Nz), *W = pool_alloc<double>(ctx, size_t(P) * Nz);
+    double *CE = pool_alloc<double>(ctx, size_t(P) * Nz);
+    IntervalOut *d_iv = pool_alloc<IntervalOut>(ctx, P);
+    std::vector<IntervalOut> iv(P);
+    auto done = [&](int code) {
+        for (void *q : {(void *)d_probs, (void *)d_nodes, (void *)A, (void *)W, (void *)CE, (void *)d_iv})
+            ctx->pool.put(q);
+        return code;
+    };
+    if (!d_probs || !d_nodes || !A || !W || !CE || !d_iv)
+        return done(set_error(ctx, NCPA_ERR_ALLOC, "ncpa_ess_count: device allocation failed"));
+    hipError_t e = hipMemcpyAsync(d_probs, hp.data(), sizeof(ProbDev) * P, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        launch_solve_front(ctx->stream, d_probs, P, max_nodes, *grid, d_nodes, A, W, CE, d_iv);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(iv.data(), d_iv, sizeof(IntervalOut) * P, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return done(hip_error(ctx, e, "ncpa_ess_count"));
+    for (int p = 0; p < P; ++p) counts[p] = std::max(0, iv[p].n_hi - iv[p].n_lo);
+    return done(NCPA_OK);
+}
+
 void ncpa_modes_free(ncpa_modes_t *m) {
     if (!m) return;
     release_modes(m);

@@ -773,12 +773,23 @@ struct ncpa_field_stage {
     void *bh = nullptr;
     float *ah = nullptr;
     size_t lda = 0, ldb = 0;
+    // the last contraction of this stage on another context's stream: the owner's stream
+    // waits for it before the blocks go back to the owner's pool
+    hipEvent_t foreign_done = nullptr;
 };
 
 namespace {
 
 void stage_release(ncpa_field_stage *st) {
     ncpa_ctx *c = st->owner;
+    if (st->foreign_done) {
+        // later users of these blocks run on the owner's stream: order them after the
+        // contraction that ran on the other context (no host wait)
+        (void)hipSetDevice(c->device);
+        (void)hipStreamWaitEvent(c->stream, st->foreign_done, 0);
+        (void)hipEventDestroy(st->foreign_done);
+        st->foreign_done = nullptr;
+    }
     c->pool.put(st->d_sp);
     c->pool.put(st->Ahat);
     c->pool.put(st->Bhat);
@@ -993,7 +1004,14 @@ int ncpa_field2d_stage(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_r
 int ncpa_field2d_contract(ncpa_ctx_t *ctx, ncpa_field_stage_t *stage, void *dev_out) {
     if (!ctx || !stage || !dev_out) return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d_contract: bad arguments");
     NCPA_HIP(ctx, hipSetDevice(ctx->device));
-    return contract(ctx, stage, dev_out);
+    const int rc = contract(ctx, stage, dev_out);
+    if (rc != NCPA_OK || ctx == stage->owner) return rc;
+    // a contraction on another context's stream reads the owner's blocks: mark its end so
+    // that ncpa_field_stage_free returns them only behind it
+    if (!stage->foreign_done)
+        NCPA_HIP(ctx, hipEventCreateWithFlags(&stage->foreign_done, hipEventDisableTiming));
+    NCPA_HIP(ctx, hipEventRecord(stage->foreign_done, ctx->stream));
+    return NCPA_OK;
 }
 
 void ncpa_field_stage_free(ncpa_field_stage_t *stage) {

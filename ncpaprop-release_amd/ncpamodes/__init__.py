@@ -81,6 +81,13 @@ class Context:
     def reset_timing(self):
         self.check(self.lib.ncpa_ctx_reset_timing(self.h))
 
+    def memory(self, reset_peak=False):
+        """(live, peak, cached) device bytes of this context's allocations (ncpa_ctx_memory)."""
+        live, peak, cached = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        self.check(self.lib.ncpa_ctx_memory(self.h, 1 if reset_peak else 0, C.byref(live), C.byref(peak),
+                                            C.byref(cached)))
+        return live.value, peak.value, cached.value
+
     # device memory -------------------------------------------------------------------
     def empty(self, n, dtype=np.float64):
         return DeviceArray(self, n, dtype)
@@ -117,6 +124,13 @@ class Context:
         h = C.c_void_p()
         self.check(self.lib.ncpa_ess_solve(self.h, C.byref(grid), arr, len(problems), C.byref(h)))
         return Modes(self, h, len(problems), grid)
+
+    def count(self, grid, problems):
+        """Mode counts ncpa_ess_solve would return for each problem (no modes computed)."""
+        arr = (Problem * len(problems))(*problems)
+        out = (C.c_int * len(problems))()
+        self.check(self.lib.ncpa_ess_count(self.h, C.byref(grid), arr, len(problems), out))
+        return np.array(out[:], dtype=np.int64)
 
     def solve_wide(self, grid, problems):
         """Wide-angle (WMod) solve of a batch; the modes' grid spacing is the matrix

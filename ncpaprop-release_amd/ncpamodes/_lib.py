@@ -35,13 +35,13 @@ NCPA_SRC = {"impulse": 0, "pulse1": 1, "pulse2": 2, "spectrum": 3, "waveform": 4
 # Every symbol include/ncpamodes.h declares (checked by tests/test_abi.py).
 EXPORTS = [
     "ncpa_ctx_create", "ncpa_ctx_destroy", "ncpa_ctx_error", "ncpa_ctx_synchronize",
-    "ncpa_ctx_stream", "ncpa_ctx_set_timing", "ncpa_ctx_kernel_time", "ncpa_ctx_reset_timing",
+    "ncpa_ctx_stream", "ncpa_ctx_set_timing", "ncpa_ctx_kernel_time", "ncpa_ctx_reset_timing", "ncpa_ctx_memory",
     "ncpa_event_record", "ncpa_event_wait", "ncpa_event_synchronize", "ncpa_event_destroy",
     "ncpa_dev_alloc", "ncpa_dev_free", "ncpa_dev_upload", "ncpa_dev_download", "ncpa_dev_copy",
     "ncpa_profile_upload", "ncpa_profile_set_attenuation", "ncpa_profile_clear",
     "ncpa_profile_read", "ncpa_profile_nodes", "ncpa_profile_admittance",
     "ncpa_grid_make", "ncpa_profile_sample",
-    "ncpa_ess_solve", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
+    "ncpa_ess_solve", "ncpa_ess_count", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
     "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d",
     "ncpa_field2d_stage", "ncpa_field2d_contract", "ncpa_field_stage_free", "ncpa_field_compare", "ncpa_ess_eigen",
     "ncpa_wmod_solve", "ncpa_wmod_eigen",
@@ -106,6 +106,7 @@ def load():
         "ncpa_ctx_set_timing": (C.c_int, [vp, C.c_int]),
         "ncpa_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
         "ncpa_ctx_reset_timing": (C.c_int, [vp]),
+        "ncpa_ctx_memory": (C.c_int, [vp, C.c_int, P(C.c_size_t), P(C.c_size_t), P(C.c_size_t)]),
         "ncpa_event_record": (C.c_int, [vp, P(vp)]),
         "ncpa_event_wait": (C.c_int, [vp, vp]),
         "ncpa_event_synchronize": (C.c_int, [vp]),
@@ -124,6 +125,7 @@ def load():
         "ncpa_grid_make": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, P(Grid)]),
         "ncpa_profile_sample": (C.c_int, [vp, C.c_int, P(Grid), dp, dp, dp, dp]),
         "ncpa_ess_solve": (C.c_int, [vp, P(Grid), P(Problem), C.c_int, P(vp)]),
+        "ncpa_ess_count": (C.c_int, [vp, P(Grid), P(Problem), C.c_int, ip]),
         "ncpa_modes_free": (None, [vp]),
         "ncpa_modes_counts": (C.c_int, [vp, ip, ip]),
         "ncpa_modes_interval": (C.c_int, [vp, dp, dp]),

@@ -27,6 +27,16 @@ Pinning (what checks each part; see tests/test_oracle_*.py):
                                        and the reference tloss_1d goldens (x1000 scaling).
   * broadband synthesis             -- numpy.fft == FFTW_FORWARD, closed-form pulses, the
                                        reference source_spectrum golden.
+  * WKB ground-to-ground cut-off    -- hand-worked 20-row grids (stated k_max), the
+                                       closed-form tunnelling integral of a linear k_eff^2,
+                                       both bypass branches (test_oracle_wkb.py).
+  * group speeds, modal starter,    -- constant-c closed form and d omega/dk of the discrete
+    attenuation file                   dispersion relation; the lossless field at the source
+                                       range; Steffen on the end-extended table
+                                       (test_oracle_options.py).
+  * C restatement (ess_port.c)      -- the same path in plain C (the timed CPU baseline);
+                                       checked against this package's functions on the same
+                                       inputs (test_oracle_port.py).
   * wide-angle modes (wmod.py)      -- the reference WMod goldens wtloss_1d{,.lossless}.nm,
                                        the reference's 2N x 2N linearisation solved densely,
                                        the constant-c / constant-Mach closed form, and the
