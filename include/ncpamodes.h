@@ -122,6 +122,14 @@ typedef struct ncpa_profile_info {
  * NULL or "" (the header is then read from `path`). */
 int ncpa_profile_read(ncpa_ctx_t *ctx, const char *path, const char *header_path, int *profile_id,
                       ncpa_profile_info_t *info);
+/* The same reader without a context or device: parses and checks the profile and fills
+ * *info (may be NULL).  NCPA_ERR_ARG with the reference's message (Atmosphere1D.cpp:85-285,
+ * ESSModeSolver.cpp:176-210: the messages thrown there) in msg[msg_len] for a missing file, a
+ * malformed header or data line, duplicate keys, unknown units, missing U/V/T/P/RHO,
+ * altitudes that do not increase or non-finite values.  The command lines call it before
+ * any device work. */
+int ncpa_profile_check(const char *path, const char *header_path, ncpa_profile_info_t *info, char *msg,
+                       int msg_len);
 /* Host copies of a profile's SI node arrays (each n_nodes long; any pointer may be NULL):
  * z, c0, wind speed, wind direction [deg CW from N], T, P, rho. */
 int ncpa_profile_nodes(ncpa_ctx_t *ctx, int profile_id, double *z, double *c0, double *ws,

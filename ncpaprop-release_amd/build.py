@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libncpamodes.so")
 SOURCES = ["csrc/ctx.cpp", "csrc/ess.hip", "csrc/wide.hip", "csrc/field2d.hip", "csrc/broadband.hip", "csrc/plan.cpp",
-           "host/atmosphere.cpp", "host/dispersion.cpp"]
+           "host/atmosphere.cpp", "host/profile_parse.cpp", "host/dispersion.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "ess_run.h"
 #include "ncpamodes.h"
 #include "params.h"
 
@@ -167,6 +168,11 @@ int bb_run(const Options &p) {
             n_in = (int)rows.size() / 3;
         }
         const double fmx = (double)NFFT * f_step;
+        // the outputs, before any device work (BroadbandPropagator.cpp:377-414 and
+        // ModalBroadbandPropagator.cpp:200-230 write them unchecked)
+        for (const std::string &f : {std::string("source_waveform_input.dat"), std::string("source_spectrum_input.dat"),
+                                     out_file})
+            check_writable(f);
 
         Dev dev;
         dev.check(ncpa_ctx_create(0, nullptr, &dev.ctx));

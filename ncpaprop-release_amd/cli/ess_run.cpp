@@ -7,6 +7,8 @@
 // rewrites for every problem ("w") hold the last problem's data, appended files hold all of them.
 #include "ess_run.h"
 
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -246,9 +248,52 @@ void write_atmosphere(const std::string &path, const ncpa_profile_info_t &info, 
 
 }  // namespace
 
+// A path the run will write: its directory must be writable, and the file too if it exists
+// (the reference fopen()s its outputs only when it writes them and does not check the result,
+// ModeSolver.cpp:319-323; here an unwritable output stops the run before any device work).
+void check_writable(const std::string &path) {
+    const size_t slash = path.find_last_of('/');
+    const std::string dir = slash == std::string::npos ? "." : (slash == 0 ? "/" : path.substr(0, slash));
+    if (::access(dir.c_str(), W_OK) != 0)
+        throw std::runtime_error("cannot open " + path + " for writing (directory " + dir + ")");
+    if (::access(path.c_str(), F_OK) == 0 && ::access(path.c_str(), W_OK) != 0)
+        throw std::runtime_error("cannot open " + path + " for writing");
+}
+
+namespace {
+
+// Everything the run reads or writes, checked on the host before a device context exists:
+// the profile (the reference's own messages), the attenuation table and the output paths.
+void preflight(const EssRun &r) {
+    char msg[1024];
+    ncpa_profile_info_t info{};
+    if (ncpa_profile_check(r.atmosfile.c_str(), r.atmosheaderfile.c_str(), &info, msg, (int)sizeof msg) != NCPA_OK)
+        throw Fail(msg);
+    if (!r.usrattfile.empty()) (void)attenuation_at_nodes(r.usrattfile, {});
+    const char *ext = r.wide ? ".wnm" : ".nm";
+    std::vector<std::string> out;
+    for (int ll = 0; ll < (r.write_lossless ? 2 : 1); ++ll) {
+        if (r.Nby2Dprop) out.push_back(r.user_tag + (ll ? "Nby2D_tloss_1d.lossless" : "Nby2D_tloss_1d") + ext);
+        else out.push_back(r.user_tag + (ll ? "tloss_1d.lossless" : "tloss_1d") + ext);
+        if (r.write_2D_TLoss && !r.Nby2Dprop) out.push_back(r.user_tag + (ll ? "tloss_2d.lossless" : "tloss_2d") + ext);
+    }
+    if (!r.Nby2Dprop) {
+        if (r.write_phase_speeds) out.push_back(r.user_tag + "phasespeeds" + ext);
+        if (r.write_speeds) out.push_back(r.user_tag + "speeds.nm");
+        if (r.write_modes) out.push_back(r.user_tag + "mode_0" + ext);
+        if (r.write_atmosphere) out.push_back(r.user_tag + "atm_profile" + ext);
+        if (!r.modstartfile.empty()) out.push_back(r.user_tag + r.modstartfile);
+        if (!r.dispersion_file.empty()) out.push_back(r.dispersion_file);
+    }
+    for (const std::string &f : out) check_writable(f);
+}
+
+}  // namespace
+
 int ess_run(const EssRun &r) {
     Session s;
     try {
+        preflight(r);
         s.check(ncpa_ctx_create(0, nullptr, &s.ctx));
         int pid = -1;
         ncpa_profile_info_t info{};

@@ -36,4 +36,8 @@ EssRun ess_from_options(const Options &p);
 // Returns 0 on success; prints the error and returns non-zero otherwise.
 int ess_run(const EssRun &r);
 
+// Throws std::runtime_error("cannot open <path> ...") unless the run can write `path`: its
+// directory is writable and so is the file if it exists.
+void check_writable(const std::string &path);
+
 }  // namespace ncpacli

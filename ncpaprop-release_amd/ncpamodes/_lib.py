@@ -39,7 +39,7 @@ EXPORTS = [
     "ncpa_event_record", "ncpa_event_wait", "ncpa_event_synchronize", "ncpa_event_destroy",
     "ncpa_dev_alloc", "ncpa_dev_free", "ncpa_dev_upload", "ncpa_dev_download", "ncpa_dev_copy",
     "ncpa_profile_upload", "ncpa_profile_set_attenuation", "ncpa_profile_clear",
-    "ncpa_profile_read", "ncpa_profile_nodes", "ncpa_profile_admittance",
+    "ncpa_profile_read", "ncpa_profile_check", "ncpa_profile_nodes", "ncpa_profile_admittance",
     "ncpa_grid_make", "ncpa_profile_sample",
     "ncpa_ess_solve", "ncpa_ess_count", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
     "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d",
@@ -120,6 +120,7 @@ def load():
         "ncpa_profile_set_attenuation": (C.c_int, [vp, C.c_int, dp]),
         "ncpa_profile_clear": (C.c_int, [vp]),
         "ncpa_profile_read": (C.c_int, [vp, C.c_char_p, C.c_char_p, ip, P(ProfileInfo)]),
+        "ncpa_profile_check": (C.c_int, [C.c_char_p, C.c_char_p, P(ProfileInfo), C.c_char_p, C.c_int]),
         "ncpa_profile_nodes": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
         "ncpa_profile_admittance": (C.c_int, [vp, C.c_int, C.c_double, dp]),
         "ncpa_grid_make": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, P(Grid)]),
