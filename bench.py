@@ -643,8 +643,8 @@ def broadband_main(args, cpu):
     S, _ = bb.source(ctx, "spectrum", f, nfft, -1.0, np.stack([f, bb_source_spectrum(f).real,
                                                                 bb_source_spectrum(f).imag], axis=1))
     nset = len(AZIMUTHS)
-    # the job as one graph: batched dispersion (persistent eigenvalue launch, autotuned),
-    # transfer functions, pulse synthesis
+    # the job as one graph: batched dispersion (eigenvalue launch form and multisection depth
+    # autotuned), transfer functions, pulse synthesis
     t = time.perf_counter()
     plan = bb.BroadbandPlan(ctx, grid, problems, nset, f, S, ranges, np.ones(nset), BB_MAXCEL, nfft, tune=True)
     plan_s = time.perf_counter() - t
@@ -674,7 +674,7 @@ def broadband_main(args, cpu):
     ctx.close()
     assert plan_modes == total_modes, (plan_modes, total_modes)
     P = len(problems)
-    modes_s = tuning["modes_ms"] / 1e3    # tuned persistent k_modes, HIP events on the library stream
+    modes_s = tuning["modes_ms"] / 1e3    # the tuned k_modes launch, HIP events on the library stream
     achieved = FLOPS_PER_STURM_STEP * sturm / modes_s / 1e12
     value = P * args.steps / elapsed
     line = {
@@ -688,7 +688,7 @@ def broadband_main(args, cpu):
         "config": {"workload": "ModBB job: dispersion at 1024 frequencies (2/1024 - 2 Hz) x 8 azimuths, "
                                "Nz=20000, + transfer functions at 16 ranges (200-1000 km) + pulse "
                                "synthesis NFFT %d, Gaussian-windowed 0.5 Hz source spectrum; one step = one "
-                               "replay of the job's HIP graph (persistent eigenvalue launch)" % nfft,
+                               "replay of the job's HIP graph (eigenvalue launch form picked by the plan's autotuner)" % nfft,
                    "global_batch": P, "modes_per_step": int(total_modes), "nfft": nfft,
                    "parallelism": "single GPU"},
         "eager_stage_ms": {k: v[0] for k, v in fams.items()},
