@@ -313,13 +313,15 @@ def field_gate(ctx, grid, problems):
     return errs, chosen
 
 
-def issued_mfma_flops(prec, counts):
+def issued_mfma_flops(prec, counts, kernel=""):
     """MFMA flops one split contraction issues: 2 x (rows padded to 128) x (2 ranges padded to
-    128) x (sum of modes padded to 32) x products; None for the FP64 / FP32 forms."""
+    the block width: 64 for k_gemm_fold, else 128) x (sum of modes padded to 32) x products;
+    None for the FP64 / FP32 forms."""
     if prec not in PREC_KERNEL:
         return None
     mp = -(-FIELD_ROWS // 128) * 128
-    np_ = -(-2 * NR // 128) * 128
+    bw = 64 if kernel.startswith("k_gemm_fold<") else 128
+    np_ = -(-2 * NR // bw) * bw
     kp = sum(-(-int(k) // 32) * 32 for k in counts)
     return 2.0 * mp * np_ * kp * PRODUCTS[prec]
 
@@ -395,13 +397,41 @@ def run_field(lanes, args, prec):
     res = {"elapsed": elapsed, "fams": fams, "total_modes": int(last.total),
            "sturm_steps": float((iters + 1.0).sum()) * NZ,
            "field_flops": 4.0 * FIELD_ROWS * last.total * NR, "streams": L,
-           "field_mfma_flops": issued_mfma_flops(prec, last.counts)}
+           "field_mfma_flops": issued_mfma_flops(prec, last.counts,
+                                                 lanes[0][0].field_variant_info(prec, lanes[0][0].field_variant(prec))[0])}
     last.free()
     for st in lane_state:
         for b in st["tl"] + [st.get("rows"), st.get("out")]:
             if b is not None:
                 b.free()
     return res
+
+
+def tune_contraction(ctx, contractors, problems, grid, precs):
+    """Contraction autotuner (ncpa_field_tune), untimed: on one solve of the step's batch,
+    every launch variant of each precision is timed on each context that will contract, and
+    that context keeps the fastest.  Returns {precision name: {"ms": [...], "best": i}} for the
+    first context (the variants agree to the bit, so the choice changes speed only)."""
+    dr = MAXRANGE_M / NR
+    m = ctx.solve(grid, problems)
+    rows = ctx.empty(FIELD_ROWS * max(m.total, 1))
+    ctx.check(ctx.lib.ncpa_modes_rows(ctx.h, m.h, 0, FIELD_STRIDE, FIELD_ROWS, ctypes.c_void_p(rows.ptr)))
+    report = {}
+    for prec in precs:
+        stage = m.field2d_stage(rows, FIELD_ROWS, dr, NR, prec)
+        for i, c in enumerate(contractors):
+            if c is not ctx:
+                ready = ctx.record()
+                c.wait(ready)
+                ready.free()
+            best, ms = stage.tune(c, reps=3)
+            if i == 0:
+                report[PREC_NAMES[prec]] = {"ms": [round(t, 3) for t in ms], "best": best}
+        stage.free()
+    ctx.synchronize()
+    rows.free()
+    m.free()
+    return report
 
 
 def run_field_pipelined(solver, contractor, problems, grid, args, prec, depth=2):
@@ -520,10 +550,14 @@ def field_main(args, cpu):
     # one stream: the stage split and the roofline (each kernel alone on the GPU); then the
     # headline throughput: the two-stage pipeline (default) or independent batches in flight
     # on several streams
+    contractor = ncpamodes.Context(0) if args.schedule == "pipeline" else None
+    tuned = tune_contraction(ctx, [ctx] + [c for c, _, _ in lanes[1:]] + ([contractor] if contractor else []),
+                             problems, grid, sorted({chosen, 0}))
+    # the launch variant each precision's contraction runs with on the single-stream context
+    kvariant = {prec: ctx.field_variant_info(prec, ctx.field_variant(prec)) for prec in {chosen, 0}}
     res = run_field(lanes[:1], args, chosen)
     plain_multi = None
     if args.schedule == "pipeline":
-        contractor = ncpamodes.Context(0)
         multi = run_field_pipelined(ctx, contractor, problems, grid, args, chosen)
         # the FP64 contraction in the same two-stage schedule: the precision's own speed-up
         plain_multi = run_field_pipelined(ctx, contractor, problems, grid, args, 0) if chosen != 0 else multi
@@ -544,8 +578,8 @@ def field_main(args, cpu):
         avg_s = fams["field2d"][0] / max(fams["field2d"][1], 1) / 1e3
         achieved = res["field_flops"] / avg_s / 1e12
         peak = PEAK_TFLOPS[PREC_PEAK[chosen]]
-        kname = PREC_KERNEL.get(chosen, "k_gemm_x4<%s>" % ("double" if chosen == 0 else "float"))
-        roof = {"kernel": "%s (2-D field contraction, %s)" % (kname, PREC_NAMES[chosen]),
+        kname, krg = kvariant[chosen]
+        roof = {"kernel": "%s (2-D field contraction, %s; row groups per XCD class %d)" % (kname, PREC_NAMES[chosen], krg),
                 "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": measured_traffic(kname),
                 "algorithmic": "4 x depths x modes x ranges real flops per problem (real x complex)"}
@@ -609,6 +643,9 @@ def field_main(args, cpu):
                           "note": "stage split and roofline are measured in this one-stream region"},
         "stage_ms_per_step": per_step,
         "roofline": roof,
+        "contraction_tuning": {"note": "ncpa_field_tune on one solve of the batch before timing: ms per launch "
+                                       "variant (block tile, LDS stages, row tiles per XCD); the variants agree "
+                                       "to the bit", **tuned},
         "k_modes": {"avg_ms": modes_avg_s * 1e3,
                     "tflops": FLOPS_PER_STURM_STEP * res["sturm_steps"] / modes_avg_s / 1e12},
         "cpu_baseline": None if cpu is None else {

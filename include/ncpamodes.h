@@ -259,6 +259,26 @@ int ncpa_field2d_stage(ncpa_ctx_t *ctx, ncpa_modes_t *modes, const double *dev_p
 int ncpa_field2d_contract(ncpa_ctx_t *ctx, ncpa_field_stage_t *stage, void *dev_out);
 void ncpa_field_stage_free(ncpa_field_stage_t *stage);
 
+/* Contraction variants and their autotuner (no counterpart in the reference).  A
+ * precision's contraction has n_variants launch variants (block tile, LDS pipeline depth,
+ * assignment of row tiles to the 8 XCDs); within a precision they compute every output
+ * element with the same operations in the same order, so they agree to the bit and differ
+ * only in speed.  Each context keeps one variant per precision (initially 0), used by
+ * ncpa_field2d and ncpa_field2d_contract on that context.  ncpa_field_tune times every
+ * variant of the stage's precision on the staged batch (one warm-up and `reps` contractions
+ * each, HIP events on ctx's stream, writing dev_out), keeps the fastest for ctx and returns
+ * its index in *best (may be NULL) and the average milliseconds per contraction of the
+ * first ms_len variants in ms (may be NULL).  Like ncpa_field2d_contract it may run on a
+ * context other than the stage's; it waits for its own timings on the host. */
+int ncpa_field_variants(int precision, int *n_variants);
+/* The kernel a variant launches (a static string) and its row groups per XCD class (the
+ * XCDs split the row tiles into that many groups; 0: the kernel's own block order). */
+int ncpa_field_variant_info(int precision, int variant, const char **kernel, int *row_groups);
+int ncpa_field_set_variant(ncpa_ctx_t *ctx, int precision, int variant);
+int ncpa_field_get_variant(ncpa_ctx_t *ctx, int precision, int *variant);
+int ncpa_field_tune(ncpa_ctx_t *ctx, ncpa_field_stage_t *stage, void *dev_out, int reps, int *best, double *ms,
+                    int ms_len);
+
 /* Transmission-loss agreement of two fields of n_points complex values (device buffers laid
  * out as ncpa_field2d writes them): dev_ref complex128, dev_test complex128 if test_precision
  * is NCPA_PREC_FP64 and complex64 otherwise.  Over the points with 10 log10 |p_ref|^2 >

@@ -124,6 +124,36 @@ __global__ void k_amat(const double *__restrict__ psi, size_t n, float *__restri
         ah[i] = (float)psi[i];
 }
 
+// word 3 of a raw buffer resource on gfx950 (32-bit data format, no swizzle, no stride)
+constexpr int RSRC_W3 = 0x00020000;
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Block id -> (M tile, N tile).  Blocks are dealt to the 8 XCDs round-robin (id % 8).  The M
+// tiles fall into row_groups groups (1, 2, 4 or 8) and the XCDs into as many classes: XCD x
+// takes group x % row_groups and every (8 / row_groups)-th N tile from x / row_groups, its
+// slots running over the group's M tiles first, so consecutive blocks on an XCD share a B
+// tile and the XCD's L2 holds 1 / row_groups of A plus the B tiles in flight.  Grid:
+// 8 x ceil(tiles_m / row_groups) x ceil(n_tiles / (8 / row_groups)) blocks.
+__device__ __forceinline__ void xcd_tile(int bid, int tiles_m, int row_groups, int &tm, int &tn) {
+    const int per = (tiles_m + row_groups - 1) / row_groups;
+    const int xcd = bid & 7, slot = bid >> 3;
+    tm = (xcd % row_groups) * per + slot % per;
+    tn = (slot / per) * (8 / row_groups) + xcd / row_groups;
+}
+
+// s_waitcnt immediate (gfx9 encoding) that waits until at most n vector memory operations
+// are outstanding and leaves the export and LDS/scalar counters alone
+constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+
+// 16 bytes per lane from base + voff + soff (bytes; zero beyond `bytes`) into LDS at
+// dst + 16 lane, through a raw buffer resource over [base, base + bytes).
+__device__ __forceinline__ void buffer_load_lds16(const void *base, unsigned bytes, lds_void *dst, unsigned voff,
+                                                  unsigned soff) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(__builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, RSRC_W3),
+                                             dst, 16, voff, soff, 0, 0);
+}
+
 struct GemmProb {
     size_t a_off;   // column offset into A (first mode of the problem)
     size_t b_off;   // row offset into B
@@ -180,6 +210,119 @@ __global__ void __launch_bounds__(256) k_gemm_x4(const GemmProb *__restrict__ gp
     T *Cp = C + P.c_off;
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
+            for (int v = 0; v < 4; ++v) {
+                const int rr = (sizeof(T) == 8) ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+                const int r = m0 + wm + a * 16 + rr;
+                const int c = n0 + wn + b * 16 + (lane & 15);
+                if (r < M && c < N) Cp[(size_t)r * ldc + c] = acc[a][b][v];
+            }
+}
+
+// The FP64 / FP32 contraction in 128 x 128 blocks: 4 waves of 64 x 64 (4 x 4 MFMA 16x16x4
+// tiles), KS modes per k-step, operands straight from global memory into two LDS stages
+// (buffer loads into LDS, 16 B per lane), the next k-step's loads in flight while the current
+// one runs.  Each output element is summed over k in the order of k_gemm_x4 (one MFMA per 4
+// modes, ascending), so the two kernels agree to the bit.
+//   A (rows x modes, row-major, ld lda): a resource over the whole array.  Columns past the
+//     problem's K read the next problem's (finite) values, rows past M read zero.
+//   B (modes x columns, ld ldb): a resource over the problem's K rows only, so the rows
+//     past K read zero and cancel A's extra columns exactly.  Columns past N only reach
+//     outputs that are not stored.
+// LDS rows of A hold KS elements in 16-byte groups, the group g of row r stored at
+// g ^ f(r), so that 16 rows read at one k fall on distinct banks.
+template <class T, int KS>
+__global__ void __launch_bounds__(256, 2) k_gemm_x4t(const GemmProb *__restrict__ gp, const T *__restrict__ A,
+                                                     size_t lda, unsigned a_bytes, const T *__restrict__ B,
+                                                     size_t ldb, T *__restrict__ C, size_t ldc, int M, int N,
+                                                     int tiles_m, int tiles_n, int row_groups) {
+    typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
+    constexpr int G = 16 / (int)sizeof(T);   // elements per 16-byte group
+    constexpr int AG = KS / G;               // groups per A row
+    constexpr int AR = 64 / AG;              // A rows per load instruction
+    constexpr int AI = 128 / AR / 4;         // A load instructions per wave and k-step
+    constexpr int BG = 128 / G;              // groups per B row
+    constexpr int BR = 64 / BG;              // B rows per load instruction
+    constexpr int BI = KS / BR / 4;          // B load instructions per wave and k-step
+    constexpr int RB = 256 / (KS * (int)sizeof(T));   // A rows per 256 bytes (one pass of the banks)
+    static_assert(AI >= 1 && BI >= 1 && AG * AR == 64 && BG * BR == 64, "tile geometry");
+    typedef T ATile[128][KS];
+    typedef T BTile[KS][128];
+    __shared__ __attribute__((aligned(16))) ATile As0, As1;
+    __shared__ __attribute__((aligned(16))) BTile Bs0, Bs1;
+    const GemmProb P = gp[blockIdx.z];
+    int tm, tn;
+    xcd_tile(blockIdx.x, tiles_m, row_groups, tm, tn);
+    if (tn >= tiles_n || tm >= tiles_m) return;
+    const int m0 = tm * 128, n0 = tn * 128;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    acc_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    const int K = P.K;
+    const T *Bp = B + P.b_off * ldb;
+    const unsigned b_bytes = (unsigned)((size_t)K * ldb * sizeof(T));
+    unsigned voff_a[AI], voff_b[BI];
+#pragma unroll
+    for (int j = 0; j < AI; ++j) {
+        const int row = (wave * AI + j) * AR + lane / AG, gs = lane % AG;
+        const int g = gs ^ ((row / RB) % AG);
+        voff_a[j] = (unsigned)(((size_t)row * lda + (size_t)g * G) * sizeof(T));
+    }
+#pragma unroll
+    for (int j = 0; j < BI; ++j) {
+        const int kr = (wave * BI + j) * BR + lane / BG, cg = lane % BG;
+        voff_b[j] = (unsigned)(((size_t)kr * ldb + (size_t)cg * G) * sizeof(T));
+    }
+    auto issue = [&](ATile &Ax, BTile &Bx, int k0) {
+        const unsigned so_a = (unsigned)(((size_t)m0 * lda + P.a_off + k0) * sizeof(T));
+        const unsigned so_b = (unsigned)(((size_t)k0 * ldb + n0) * sizeof(T));
+#pragma unroll
+        for (int j = 0; j < AI; ++j)
+            buffer_load_lds16(A, a_bytes, (lds_void *)&Ax[(wave * AI + j) * AR][0], voff_a[j], so_a);
+#pragma unroll
+        for (int j = 0; j < BI; ++j)
+            buffer_load_lds16(Bp, b_bytes, (lds_void *)&Bx[(wave * BI + j) * BR][0], voff_b[j], so_b);
+    };
+    auto step = [&](ATile &Ac, BTile &Bc, ATile &An, BTile &Bn, int k0) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (k0 + KS < K) issue(An, Bn, k0 + KS);
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 4) {
+            const int k = kk + (lane >> 4);
+            T af[4], bf[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int r = wm + a * 16 + (lane & 15);
+                af[a] = Ac[r][(((k / G) ^ ((r / RB) % AG)) * G) + k % G];
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bf[b] = Bc[k][wn + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    if constexpr (sizeof(T) == 8)
+                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+                    else
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+                }
+        }
+    };
+    if (K > 0) issue(As0, Bs0, 0);
+    for (int k0 = 0; k0 < K; k0 += 2 * KS) {
+        step(As0, Bs0, As1, Bs1, k0);
+        if (k0 + KS < K) step(As1, Bs1, As0, Bs0, k0 + KS);
+    }
+    T *Cp = C + P.c_off;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int rr = (sizeof(T) == 8) ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
                 const int r = m0 + wm + a * 16 + rr;
@@ -501,6 +644,22 @@ __device__ __forceinline__ f4 part_products(const u4 (&af)[S], const u4 (&bf)[S]
     return mfma16<F16>(af[0], bf[0], acc);
 }
 
+// The order of part_products as a table: product i multiplies A part a(i) by B part b(i);
+// first_a(i) / first_b(i): the A / B parts in the order the products first need them.
+template <int S, int NP>
+struct PartOrder {
+    // S = 3: lo*hi, mid*mid, hi*lo, mid*hi, hi*mid, hi*hi; S = 2, NP = 4: lo*lo, lo*hi, hi*lo,
+    // hi*hi; S = 2, NP = 3: lo*hi, hi*lo, hi*hi (part 0 = hi)
+    __device__ static constexpr int a(int i) {
+        return S == 3 ? (i < 3 ? 2 - i : (i == 3 ? 1 : 0)) : (NP == 4 ? (i < 2 ? 1 : 0) : (i == 0 ? 1 : 0));
+    }
+    __device__ static constexpr int b(int i) {
+        return S == 3 ? (i < 3 ? i : (i == 4 ? 1 : 0)) : (NP == 4 ? (i == 0 || i == 2 ? 1 : 0) : (i == 1 ? 1 : 0));
+    }
+    __device__ static constexpr int first_a(int i) { return S - 1 - i; }
+    __device__ static constexpr int first_b(int i) { return S == 2 && NP == 4 ? 1 - i : i; }
+};
+
 // Plain split contraction: FP32 accumulation over all modes.  Block tile 128 x 128, 4 waves
 // of 64 x 64; operands staged global -> registers -> LDS, the next k-step fetched into
 // registers while the current one runs from LDS.
@@ -511,17 +670,12 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
                                                        size_t split_stride_a, size_t split_stride_b,
                                                        float *__restrict__ C, size_t ldc, int M, int N,
                                                        int tiles_m, int n_tiles_n, const int *__restrict__ SA, int Mp,
-                                                       const int *__restrict__ SB, int nr_pad) {
+                                                       const int *__restrict__ SB, int nr_pad, int row_groups) {
     __shared__ __attribute__((aligned(16))) unsigned short As[S][TBM][TLD];
     __shared__ __attribute__((aligned(16))) unsigned short Bs[S][TBN][TLD];
     const SplitProb P = sp[blockIdx.z];
-    // Blocks are dealt to the 8 XCDs round-robin (id % 8).  XCD x takes M-tile half x % 2 and
-    // every fourth N tile from x / 2; on the XCD a B tile is used by the half's M tiles one
-    // after another, so an L2 holds half of the problem's A and the B tiles in flight.
-    const int tmh = (tiles_m + 1) / 2;                     // M tiles per half
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int tm = (xcd & 1) * tmh + slot % tmh;
-    const int tn = (slot / tmh) * 4 + (xcd >> 1);
+    int tm, tn;
+    xcd_tile(blockIdx.x, tiles_m, row_groups, tm, tn);
     if (tn >= n_tiles_n || tm >= tiles_m) return;
     const int m0 = tm * TBM, n0 = tn * TBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -582,7 +736,6 @@ __global__ void __launch_bounds__(256, 2) k_gemm_split(const SplitProb *__restri
                        F16 ? SA + (size_t)blockIdx.z * Mp : nullptr, F16 ? SB + (size_t)blockIdx.z * nr_pad : nullptr);
 }
 
-typedef __attribute__((address_space(3))) void lds_void;
 
 // FP64-folded split contraction: each k-step's part products start from zero in FP32 and
 // are added to FP64 accumulators, so FP32 sums run over the 32 modes of one k-step only
@@ -603,18 +756,15 @@ __global__ void __launch_bounds__(256, 2) k_gemm_fold(const SplitProb *__restric
                                                       size_t split_stride_a, size_t split_stride_b,
                                                       float *__restrict__ C, size_t ldc, int M, int N,
                                                       int tiles_m, int n_tiles_n, const int *__restrict__ SA, int Mp,
-                                                      const int *__restrict__ SB, int nr_pad) {
+                                                      const int *__restrict__ SB, int nr_pad, int row_groups) {
     constexpr int BN = 64;
     typedef unsigned short ATile[TBM][TKS];
     typedef unsigned short BTile[BN][TKS];
     __shared__ __attribute__((aligned(16))) ATile As0[S], As1[S];
     __shared__ __attribute__((aligned(16))) BTile Bs0[S], Bs1[S];
     const SplitProb P = sp[blockIdx.z];
-    // XCD mapping as in k_gemm_split
-    const int tmh = (tiles_m + 1) / 2;
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int tm = (xcd & 1) * tmh + slot % tmh;
-    const int tn = (slot / tmh) * 4 + (xcd >> 1);
+    int tm, tn;
+    xcd_tile(blockIdx.x, tiles_m, row_groups, tm, tn);
     if (tn >= n_tiles_n || tm >= tiles_m) return;
     const int m0 = tm * TBM, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -630,23 +780,35 @@ __global__ void __launch_bounds__(256, 2) k_gemm_fold(const SplitProb *__restric
     const unsigned short *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
     const int Kp = P.Kp;
     const int crow = lane >> 2, gs = lane & 3;
+    // Buffer resources over each part of the problem's operand blocks (loads outside them
+    // return zero).  Buffer loads rather than global ones: a FLAT-encoded load into LDS makes
+    // the compiler wait for every LDS read at once (lgkmcnt(0)), buffer loads let each
+    // fragment read be waited for on its own.  Per lane the byte offset inside a k-step's
+    // tile chunk is fixed; the chunk's offset is a scalar that advances by 8 KB per k-step.
+    const unsigned a_bytes = (unsigned)tiles_m * TBM * Kp * 2u;
+    const unsigned b_bytes = (unsigned)((n_tiles_n + 1) / 2) * TBM * Kp * 2u;
+    unsigned voff_a[2], voff_b;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int row = (2 * wave + j) * 16 + crow;
+        voff_a[j] = (unsigned)(row * TKS + 8 * (gs ^ ((row >> 2) & 3))) * 2u;
+    }
+    {
+        const int row = wave * 16 + crow;
+        voff_b = (unsigned)(((tn & 1) * 64 + row) * TKS + 8 * (gs ^ ((row >> 2) & 3))) * 2u;
+    }
     // k-step k0 into one stage: A = 8 chunks of 16 rows (2 per wave), B = 4 chunks (1 per wave)
     auto issue = [&](ATile *Ax, BTile *Bx, int k0) {
+        const int kt = k0 / TKS, nkt = Kp / TKS;
+        const unsigned so_a = (unsigned)(tm * nkt + kt) * (TBM * TKS * 2u);
+        const unsigned so_b = (unsigned)((tn >> 1) * nkt + kt) * (TBM * TKS * 2u);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int ch = 2 * wave + j, row = ch * 16 + crow;
-                const int g = gs ^ ((row >> 2) & 3);
-                __builtin_amdgcn_global_load_lds(
-                    (const void *)&Ab[s * split_stride_a + tiled_index(m0 + row, k0 + 8 * g, Kp)],
-                    (lds_void *)&Ax[s][ch * 16][0], 16, 0, 0);
-            }
-            const int row = wave * 16 + crow;
-            const int g = gs ^ ((row >> 2) & 3);
-            __builtin_amdgcn_global_load_lds(
-                (const void *)&Bb[s * split_stride_b + tiled_index(n0 + row, k0 + 8 * g, Kp)],
-                (lds_void *)&Bx[s][wave * 16][0], 16, 0, 0);
+            for (int j = 0; j < 2; ++j)
+                buffer_load_lds16(Ab + s * split_stride_a, a_bytes, (lds_void *)&Ax[s][(2 * wave + j) * 16][0],
+                                  voff_a[j], so_a);
+            buffer_load_lds16(Bb + s * split_stride_b, b_bytes, (lds_void *)&Bx[s][wave * 16][0], voff_b, so_b);
         }
     };
     const int kq = lds_group(lane & 15, lane >> 4);
@@ -656,27 +818,144 @@ __global__ void __launch_bounds__(256, 2) k_gemm_fold(const SplitProb *__restric
         __syncthreads();
         // the other stage was last read in the previous k-step, which every wave has left
         if (k0 + TKS < Kp) issue(An, Bn, k0 + TKS);
+        // fragments in the order the products first use them
         u4 af[4][S], bfr[2][S];
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
+        for (int i = 0; i < S; ++i) {
+            const int sa = PartOrder<S, NP>::first_a(i), sb = PartOrder<S, NP>::first_b(i);
 #pragma unroll
-            for (int a = 0; a < 4; ++a) af[a][s] = *(const u4 *)&Ac[s][wm + a * 16 + (lane & 15)][kq];
+            for (int a = 0; a < 4; ++a) af[a][sa] = *(const u4 *)&Ac[sa][wm + a * 16 + (lane & 15)][kq];
 #pragma unroll
-            for (int b = 0; b < 2; ++b) bfr[b][s] = *(const u4 *)&Bc[s][wn + b * 16 + (lane & 15)][kq];
+            for (int b = 0; b < 2; ++b) bfr[b][sb] = *(const u4 *)&Bc[sb][wn + b * 16 + (lane & 15)][kq];
+        }
+        // one part product at a time over all 8 tiles (each tile's products smallest first,
+        // as part_products sums them), so dependent MFMAs are 8 instructions apart
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = mfma16<F16>(af[a][PartOrder<S, NP>::a(i)], bfr[b][PartOrder<S, NP>::b(i)], acc[a][b]);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                acc[a][b] = part_products<S, NP, F16>(af[a], bfr[b], f4{0, 0, 0, 0});
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc64[a][b][v] += (double)acc[a][b][v];
-            }
     };
     if (Kp > 0) issue(As0, Bs0, 0);   // a problem without modes has no operand block to read
     for (int k0 = 0; k0 < Kp; k0 += 2 * TKS) {
         step(As0, Bs0, As1, Bs1, k0);
         if (k0 + TKS < Kp) step(As1, Bs1, As0, Bs0, k0 + TKS);
+    }
+    store_tile<F16, 2>(C + P.c_off, ldc, M, N, m0 + wm, n0 + wn, lane, acc64,
+                       F16 ? SA + (size_t)blockIdx.z * Mp : nullptr, F16 ? SB + (size_t)blockIdx.z * nr_pad : nullptr);
+}
+
+// The folded contraction with three LDS stages: 8 waves of 64 x 32 in a 128 x 128 block,
+// one block per CU (3 x 48 KB of LDS for three parts), the loads of k-steps t + 1 and t + 2
+// in flight while k-step t runs.
+template <int S, int NP, bool F16>
+__global__ void __launch_bounds__(512, 1) k_gemm_fold3(const SplitProb *__restrict__ sp,
+                                                       const unsigned short *__restrict__ Ahat,
+                                                       const unsigned short *__restrict__ Bhat,
+                                                       size_t split_stride_a, size_t split_stride_b,
+                                                       float *__restrict__ C, size_t ldc, int M, int N,
+                                                       int tiles_m, int n_tiles_n, const int *__restrict__ SA, int Mp,
+                                                       const int *__restrict__ SB, int nr_pad, int row_groups) {
+    typedef unsigned short Tile[TBM][TKS];
+    __shared__ __attribute__((aligned(16))) Tile As0[S], As1[S], As2[S];
+    __shared__ __attribute__((aligned(16))) Tile Bs0[S], Bs1[S], Bs2[S];
+    const SplitProb P = sp[blockIdx.z];
+    int tm, tn;
+    xcd_tile(blockIdx.x, tiles_m, row_groups, tm, tn);
+    if (tn >= n_tiles_n || tm >= tiles_m) return;
+    const int m0 = tm * TBM, n0 = tn * TBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 2) * 64, wn = (wave & 3) * 32;
+    f4 acc[4][2];
+    double acc64[4][2][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc64[a][b][v] = 0.0;
+    const unsigned short *Ab = Ahat + P.a_off, *Bb = Bhat + P.b_off;
+    const int Kp = P.Kp;
+    const int crow = lane >> 2, gs = lane & 3;
+    const unsigned a_bytes = (unsigned)tiles_m * TBM * Kp * 2u;
+    const unsigned b_bytes = (unsigned)n_tiles_n * TBN * Kp * 2u;
+    // this wave's 16 rows of every A and B tile chunk (the same lane -> row / group map)
+    const int row = wave * 16 + crow;
+    const unsigned voff = (unsigned)(row * TKS + 8 * (gs ^ ((row >> 2) & 3))) * 2u;
+    auto issue = [&](Tile *Ax, Tile *Bx, int k0) {
+        const int kt = k0 / TKS, nkt = Kp / TKS;
+        const unsigned so_a = (unsigned)(tm * nkt + kt) * (TBM * TKS * 2u);
+        const unsigned so_b = (unsigned)(tn * nkt + kt) * (TBM * TKS * 2u);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            buffer_load_lds16(Ab + s * split_stride_a, a_bytes, (lds_void *)&Ax[s][wave * 16][0], voff, so_a);
+            buffer_load_lds16(Bb + s * split_stride_b, b_bytes, (lds_void *)&Bx[s][wave * 16][0], voff, so_b);
+        }
+    };
+    const int kq = lds_group(lane & 15, lane >> 4);
+    auto step = [&](Tile *Ac, Tile *Bc, Tile *An, Tile *Bn, int k0) {
+        // this wave's loads of the current stage: the loads of the next k-step (2 S per
+        // wave) may stay in flight
+        // (the builtin, not inline asm, so that the compiler's own wait insertion sees it)
+        if (k0 + TKS < Kp)
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * S));
+        else
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+        // a barrier without the fence of __syncthreads, which would wait for every load in
+        // flight; the LDS reads before it have completed (their MFMAs have consumed them)
+        asm volatile("s_barrier" ::: "memory");
+        // the stage two k-steps ahead was last read in the previous k-step, which every wave
+        // has left
+        if (k0 + 2 * TKS < Kp) issue(An, Bn, k0 + 2 * TKS);
+        u4 af[4][S], bfr[2][S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int sa = PartOrder<S, NP>::first_a(i), sb = PartOrder<S, NP>::first_b(i);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) af[a][sa] = *(const u4 *)&Ac[sa][wm + a * 16 + (lane & 15)][kq];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bfr[b][sb] = *(const u4 *)&Bc[sb][wn + b * 16 + (lane & 15)][kq];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = mfma16<F16>(af[a][PartOrder<S, NP>::a(i)], bfr[b][PartOrder<S, NP>::b(i)], acc[a][b]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc64[a][b][v] += (double)acc[a][b][v];
+    };
+    if (Kp > 0) issue(As0, Bs0, 0);   // a problem without modes has no operand block to read
+    if (TKS < Kp) issue(As1, Bs1, TKS);
+    for (int k0 = 0; k0 < Kp; k0 += 3 * TKS) {
+        step(As0, Bs0, As2, Bs2, k0);
+        if (k0 + TKS < Kp) step(As1, Bs1, As0, Bs0, k0 + TKS);
+        if (k0 + 2 * TKS < Kp) step(As2, Bs2, As1, Bs1, k0 + 2 * TKS);
     }
     store_tile<F16, 2>(C + P.c_off, ldc, M, N, m0 + wm, n0 + wn, lane, acc64,
                        F16 ? SA + (size_t)blockIdx.z * Mp : nullptr, F16 ? SB + (size_t)blockIdx.z * nr_pad : nullptr);
@@ -773,6 +1052,7 @@ struct ncpa_field_stage {
     void *bh = nullptr;
     float *ah = nullptr;
     size_t lda = 0, ldb = 0;
+    int max_k = 0;   // most modes of one problem
     // the last contraction of this stage on another context's stream: the owner's stream
     // waits for it before the blocks go back to the owner's pool
     hipEvent_t foreign_done = nullptr;
@@ -882,7 +1162,10 @@ int stage_plain(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
     st->lda = (size_t)std::max(total, 1);
     st->psi_rows = psi;
     std::vector<GemmProb> gp(P);
-    for (int p = 0; p < P; ++p) gp[p] = {(size_t)hoff[p], (size_t)hoff[p], (size_t)p * nrows * st->ldb, hcnt[p]};
+    for (int p = 0; p < P; ++p) {
+        gp[p] = {(size_t)hoff[p], (size_t)hoff[p], (size_t)p * nrows * st->ldb, hcnt[p]};
+        st->max_k = std::max(st->max_k, hcnt[p]);
+    }
     st->d_gp = pool_alloc<GemmProb>(ctx, P);
     const size_t esz = precision == PREC_FP64 ? 8 : 4;
     st->bh = ctx->pool.get(std::max<size_t>(total, 1) * st->ldb * esz);
@@ -910,42 +1193,94 @@ int stage_plain(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
     return NCPA_OK;
 }
 
-template <int S, int NP, bool F16, bool FOLD>
-void launch_split(ncpa_ctx *ctx, const ncpa_field_stage *st, float *out) {
-    // Np is a multiple of 128 (the E-staging tiles), so of either block width
-    const int tn = st->Np / (FOLD ? 64 : TBN), tm = st->Mp / TBM;
-    // 8 XCDs x ceil(tm / 2) M tiles x ceil(tn / 4) N tiles per XCD
-    dim3 g((unsigned)(8 * ((tm + 1) / 2) * ((tn + 3) / 4)), 1, (unsigned)st->P);
-    if constexpr (FOLD)
-        hipLaunchKernelGGL((k_gemm_fold<S, NP, F16>), g, dim3(256), 0, ctx->stream, st->d_sp, st->Ahat, st->Bhat,
-                           st->sa, st->sb, out, (size_t)st->N, st->M, st->N, tm, tn, st->SA, st->Mp, st->SB,
-                           st->nr_pad);
-    else
-        hipLaunchKernelGGL((k_gemm_split<S, NP, F16>), g, dim3(256), 0, ctx->stream, st->d_sp, st->Ahat, st->Bhat,
-                           st->sa, st->sb, out, (size_t)st->N, st->M, st->N, tm, tn, st->SA, st->Mp, st->SB,
-                           st->nr_pad);
+// Launch variants of the contractions: a kernel form and an XCD row grouping (xcd_tile).
+// Within one precision every variant forms each output element with the same operations in
+// the same order, so the variants agree to the bit and differ in speed only; the context
+// keeps one per precision (ncpa_field_set_variant, ncpa_field_tune).
+//   folded splits (BF16x6F, FP16x4F): form 0 = k_gemm_fold (128 x 64 blocks, 2 per CU),
+//                                     form 1 = k_gemm_fold3 (128 x 128 blocks, 3 LDS stages)
+//   plain splits: k_gemm_split (128 x 128 blocks)
+//   FP64 / FP32:  form 1 = k_gemm_x4t (128 x 128 blocks, loads into LDS), form 0 = k_gemm_x4
+//                 (64 x 64 blocks, element loads; also taken when an operand exceeds the
+//                 2 GiB that the tiled form's 32-bit offsets address)
+struct ContractVariant {
+    int form, row_groups;
+};
+constexpr ContractVariant FOLD_VARIANTS[] = {{0, 2}, {0, 4}, {0, 1}, {1, 2}, {1, 4}};
+constexpr ContractVariant SPLIT_VARIANTS[] = {{0, 2}, {0, 4}, {0, 1}};
+constexpr ContractVariant X4_VARIANTS[] = {{1, 2}, {1, 4}, {1, 1}, {0, 0}};
+
+int contract_variants(int precision) {
+    if (precision == PREC_BF16X6F || precision == PREC_FP16X4F) return (int)(sizeof FOLD_VARIANTS / sizeof FOLD_VARIANTS[0]);
+    if (precision == PREC_FP64 || precision == PREC_FP32) return (int)(sizeof X4_VARIANTS / sizeof X4_VARIANTS[0]);
+    return (int)(sizeof SPLIT_VARIANTS / sizeof SPLIT_VARIANTS[0]);
 }
 
-int contract(ncpa_ctx *ctx, const ncpa_field_stage *st, void *dev_out) {
+// blocks of an xcd_tile grid over tiles_m x tiles_n tiles
+static unsigned xcd_grid(int tiles_m, int tiles_n, int rg) {
+    return (unsigned)(8 * ((tiles_m + rg - 1) / rg) * ((tiles_n + 8 / rg - 1) / (8 / rg)));
+}
+
+template <int S, int NP, bool F16, bool FOLD>
+void launch_split(ncpa_ctx *ctx, const ncpa_field_stage *st, float *out, int variant) {
+    // Np is a multiple of 128 (the E-staging tiles), so of either block width
+    const int tm = st->Mp / TBM;
+    if constexpr (FOLD) {
+        const ContractVariant v = FOLD_VARIANTS[variant];
+        const int tn = v.form == 1 ? st->Np / TBN : (st->N + 63) / 64;   // N tiles that hold outputs
+        const dim3 g(xcd_grid(tm, tn, v.row_groups), 1, (unsigned)st->P);
+        if (v.form == 1)
+            hipLaunchKernelGGL((k_gemm_fold3<S, NP, F16>), g, dim3(512), 0, ctx->stream, st->d_sp, st->Ahat,
+                               st->Bhat, st->sa, st->sb, out, (size_t)st->N, st->M, st->N, tm, tn, st->SA, st->Mp,
+                               st->SB, st->nr_pad, v.row_groups);
+        else
+            hipLaunchKernelGGL((k_gemm_fold<S, NP, F16>), g, dim3(256), 0, ctx->stream, st->d_sp, st->Ahat,
+                               st->Bhat, st->sa, st->sb, out, (size_t)st->N, st->M, st->N, tm, tn, st->SA, st->Mp,
+                               st->SB, st->nr_pad, v.row_groups);
+    } else {
+        const ContractVariant v = SPLIT_VARIANTS[variant];
+        const int tn = st->Np / TBN;
+        const dim3 g(xcd_grid(tm, tn, v.row_groups), 1, (unsigned)st->P);
+        hipLaunchKernelGGL((k_gemm_split<S, NP, F16>), g, dim3(256), 0, ctx->stream, st->d_sp, st->Ahat, st->Bhat,
+                           st->sa, st->sb, out, (size_t)st->N, st->M, st->N, tm, tn, st->SA, st->Mp, st->SB,
+                           st->nr_pad, v.row_groups);
+    }
+}
+
+template <class T>
+void launch_x4(ncpa_ctx *ctx, const ncpa_field_stage *st, const T *A, T *out, int variant) {
+    const size_t ldc = st->ldb;
+    ContractVariant v = X4_VARIANTS[variant];
+    // the tiled form addresses its operands through 32-bit buffer offsets
+    const size_t a_bytes = (size_t)st->M * st->lda * sizeof(T);
+    const size_t b_bytes = (size_t)st->max_k * st->ldb * sizeof(T);
+    if (v.form == 1 && (a_bytes >= (size_t(1) << 31) || b_bytes >= (size_t(1) << 31))) v = ContractVariant{0, 0};
+    if (v.form == 1) {
+        const int tm = (st->M + 127) / 128, tn = (int)((st->ldb + 127) / 128);
+        const dim3 g(xcd_grid(tm, tn, v.row_groups), 1, (unsigned)st->P);
+        hipLaunchKernelGGL((k_gemm_x4t<T, 16>), g, dim3(256), 0, ctx->stream, st->d_gp, A, st->lda,
+                           (unsigned)a_bytes, (const T *)st->bh, st->ldb, out, ldc, st->M, (int)st->ldb, tm, tn,
+                           v.row_groups);
+    } else {
+        const dim3 gg((unsigned)((st->ldb + BN - 1) / BN), (unsigned)((st->M + BM - 1) / BM), (unsigned)st->P);
+        hipLaunchKernelGGL((k_gemm_x4<T, 16>), gg, dim3(256), 0, ctx->stream, st->d_gp, A, st->lda,
+                           (const T *)st->bh, st->ldb, out, ldc, st->M, (int)st->ldb);
+    }
+}
+
+int contract(ncpa_ctx *ctx, const ncpa_field_stage *st, void *dev_out, int variant) {
     TimedLaunch tl(ctx, "field2d");
     float *out = (float *)dev_out;
+    if (variant < 0) variant = ctx->field_variant[st->precision];
     switch (st->precision) {
-    case PREC_BF16X3: launch_split<2, 3, false, false>(ctx, st, out); break;
-    case PREC_BF16X6: launch_split<3, 6, false, false>(ctx, st, out); break;
-    case PREC_FP16X3: launch_split<2, 3, true, false>(ctx, st, out); break;
-    case PREC_FP16X4: launch_split<2, 4, true, false>(ctx, st, out); break;
-    case PREC_BF16X6F: launch_split<3, 6, false, true>(ctx, st, out); break;
-    case PREC_FP16X4F: launch_split<2, 4, true, true>(ctx, st, out); break;
-    default: {
-        const size_t ldc = st->ldb;
-        dim3 gg((unsigned)((st->ldb + BN - 1) / BN), (unsigned)((st->M + BM - 1) / BM), (unsigned)st->P);
-        if (st->precision == PREC_FP64)
-            hipLaunchKernelGGL((k_gemm_x4<double, 16>), gg, dim3(256), 0, ctx->stream, st->d_gp, st->psi_rows,
-                               st->lda, (const double *)st->bh, st->ldb, (double *)dev_out, ldc, st->M, (int)st->ldb);
-        else
-            hipLaunchKernelGGL((k_gemm_x4<float, 16>), gg, dim3(256), 0, ctx->stream, st->d_gp, (const float *)st->ah,
-                               st->lda, (const float *)st->bh, st->ldb, (float *)dev_out, ldc, st->M, (int)st->ldb);
-    }
+    case PREC_BF16X3: launch_split<2, 3, false, false>(ctx, st, out, variant); break;
+    case PREC_BF16X6: launch_split<3, 6, false, false>(ctx, st, out, variant); break;
+    case PREC_FP16X3: launch_split<2, 3, true, false>(ctx, st, out, variant); break;
+    case PREC_FP16X4: launch_split<2, 4, true, false>(ctx, st, out, variant); break;
+    case PREC_BF16X6F: launch_split<3, 6, false, true>(ctx, st, out, variant); break;
+    case PREC_FP16X4F: launch_split<2, 4, true, true>(ctx, st, out, variant); break;
+    case PREC_FP64: launch_x4<double>(ctx, st, st->psi_rows, (double *)dev_out, variant); break;
+    default: launch_x4<float>(ctx, st, st->ah, out, variant);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(ctx, e, "ncpa_field2d launch");
@@ -1004,7 +1339,7 @@ int ncpa_field2d_stage(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_r
 int ncpa_field2d_contract(ncpa_ctx_t *ctx, ncpa_field_stage_t *stage, void *dev_out) {
     if (!ctx || !stage || !dev_out) return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d_contract: bad arguments");
     NCPA_HIP(ctx, hipSetDevice(ctx->device));
-    const int rc = contract(ctx, stage, dev_out);
+    const int rc = contract(ctx, stage, dev_out, -1);
     if (rc != NCPA_OK || ctx == stage->owner) return rc;
     // a contraction on another context's stream reads the owner's blocks: mark its end so
     // that ncpa_field_stage_free returns them only behind it
@@ -1018,13 +1353,99 @@ void ncpa_field_stage_free(ncpa_field_stage_t *stage) {
     if (stage) stage_release(stage);
 }
 
+int ncpa_field_variants(int precision, int *n_variants) {
+    if (precision < 0 || precision > PREC_FP16X4F || !n_variants) return NCPA_ERR_ARG;
+    *n_variants = contract_variants(precision);
+    return NCPA_OK;
+}
+
+int ncpa_field_variant_info(int precision, int variant, const char **kernel, int *row_groups) {
+    if (precision < 0 || precision > PREC_FP16X4F || variant < 0 || variant >= contract_variants(precision))
+        return NCPA_ERR_ARG;
+    static const char *const split_names[8] = {"", "", "k_gemm_split<2, 3, false>", "k_gemm_split<3, 6, false>",
+                                               "k_gemm_split<2, 3, true>", "k_gemm_split<2, 4, true>", "", ""};
+    ContractVariant v;
+    const char *name;
+    if (precision == PREC_BF16X6F || precision == PREC_FP16X4F) {
+        v = FOLD_VARIANTS[variant];
+        const bool b = precision == PREC_BF16X6F;
+        name = v.form == 1 ? (b ? "k_gemm_fold3<3, 6, false>" : "k_gemm_fold3<2, 4, true>")
+                           : (b ? "k_gemm_fold<3, 6, false>" : "k_gemm_fold<2, 4, true>");
+    } else if (precision == PREC_FP64 || precision == PREC_FP32) {
+        v = X4_VARIANTS[variant];
+        const bool d = precision == PREC_FP64;
+        name = v.form == 1 ? (d ? "k_gemm_x4t<double, 16>" : "k_gemm_x4t<float, 16>")
+                           : (d ? "k_gemm_x4<double, 16>" : "k_gemm_x4<float, 16>");
+    } else {
+        v = SPLIT_VARIANTS[variant];
+        name = split_names[precision];
+    }
+    if (kernel) *kernel = name;
+    if (row_groups) *row_groups = v.row_groups;
+    return NCPA_OK;
+}
+
+int ncpa_field_set_variant(ncpa_ctx_t *ctx, int precision, int variant) {
+    if (!ctx || precision < 0 || precision > PREC_FP16X4F || variant < 0 || variant >= contract_variants(precision))
+        return set_error(ctx, NCPA_ERR_ARG, "ncpa_field_set_variant: bad arguments");
+    ctx->field_variant[precision] = variant;
+    return NCPA_OK;
+}
+
+int ncpa_field_get_variant(ncpa_ctx_t *ctx, int precision, int *variant) {
+    if (!ctx || precision < 0 || precision > PREC_FP16X4F || !variant)
+        return set_error(ctx, NCPA_ERR_ARG, "ncpa_field_get_variant: bad arguments");
+    *variant = ctx->field_variant[precision];
+    return NCPA_OK;
+}
+
+int ncpa_field_tune(ncpa_ctx_t *ctx, ncpa_field_stage_t *stage, void *dev_out, int reps, int *best, double *ms,
+                    int ms_len) {
+    if (!ctx || !stage || !dev_out || reps < 1) return set_error(ctx, NCPA_ERR_ARG, "ncpa_field_tune: bad arguments");
+    NCPA_HIP(ctx, hipSetDevice(ctx->device));
+    const int nv = contract_variants(stage->precision);
+    hipEvent_t e0, e1;
+    NCPA_HIP(ctx, hipEventCreate(&e0));
+    NCPA_HIP(ctx, hipEventCreate(&e1));
+    int pick = 0, rc = NCPA_OK;
+    double best_ms = 0.0;
+    for (int v = 0; v < nv && rc == NCPA_OK; ++v) {
+        rc = contract(ctx, stage, dev_out, v);   // warm-up
+        hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < reps && rc == NCPA_OK; ++r) rc = contract(ctx, stage, dev_out, v);
+        hipEventRecord(e1, ctx->stream);
+        float t = 0.0f;
+        if (rc == NCPA_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess) {
+            const double avg = (double)t / reps;
+            if (ms && v < ms_len) ms[v] = avg;
+            if (v == 0 || avg < best_ms) {
+                best_ms = avg;
+                pick = v;
+            }
+        } else if (rc == NCPA_OK) {
+            rc = set_error(ctx, NCPA_ERR_HIP, "ncpa_field_tune: timing failed");
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc != NCPA_OK) return rc;
+    ctx->field_variant[stage->precision] = pick;
+    if (best) *best = pick;
+    if (ctx != stage->owner) {
+        if (!stage->foreign_done)
+            NCPA_HIP(ctx, hipEventCreateWithFlags(&stage->foreign_done, hipEventDisableTiming));
+        NCPA_HIP(ctx, hipEventRecord(stage->foreign_done, ctx->stream));
+    }
+    return NCPA_OK;
+}
+
 int ncpa_field2d(ncpa_ctx_t *ctx, ncpa_modes_t *m, const double *dev_psi_rows, int nrows, double dr, int n_r,
                  int lossless, int precision, void *dev_out) {
     if (!dev_out) return set_error(ctx, NCPA_ERR_ARG, "ncpa_field2d: bad arguments");
     ncpa_field_stage *st = nullptr;
     int rc = ncpa_field2d_stage(ctx, m, dev_psi_rows, nrows, dr, n_r, lossless, precision, &st);
     if (rc != NCPA_OK) return rc;
-    rc = contract(ctx, st, dev_out);
+    rc = contract(ctx, st, dev_out, -1);
     stage_release(st);   // stream-ordered on ctx: the blocks are reused only after the contraction
     return rc;
 }

@@ -88,6 +88,27 @@ class Context:
                                             C.byref(cached)))
         return live.value, peak.value, cached.value
 
+    # contraction variants ----------------------------------------------------------------
+    def field_variants(self, precision):
+        """Number of launch variants of a precision's contraction (ncpa_field_variants)."""
+        n = C.c_int()
+        self.check(self.lib.ncpa_field_variants(precision, C.byref(n)))
+        return n.value
+
+    def field_variant_info(self, precision, variant):
+        """(kernel name, row groups) of a contraction variant (ncpa_field_variant_info)."""
+        name, rg = C.c_char_p(), C.c_int()
+        self.check(self.lib.ncpa_field_variant_info(precision, variant, C.byref(name), C.byref(rg)))
+        return name.value.decode(), rg.value
+
+    def set_field_variant(self, precision, variant):
+        self.check(self.lib.ncpa_field_set_variant(self.h, precision, variant))
+
+    def field_variant(self, precision):
+        v = C.c_int()
+        self.check(self.lib.ncpa_field_get_variant(self.h, precision, C.byref(v)))
+        return v.value
+
     # device memory -------------------------------------------------------------------
     def empty(self, n, dtype=np.float64):
         return DeviceArray(self, n, dtype)
@@ -207,6 +228,21 @@ class FieldStage:
             out = ctx.empty(self.n_out, np.float64 if self.precision == 0 else np.float32)
         ctx.check(self.lib.ncpa_field2d_contract(ctx.h, self.h, C.c_void_p(out.ptr)))
         return out
+
+    def tune(self, ctx, out=None, reps=3):
+        """Time every contraction variant of this precision on the staged batch
+        (ncpa_field_tune); ctx keeps the fastest.  Returns (best, [ms per variant])."""
+        own = out is None
+        if own:
+            out = ctx.empty(self.n_out, np.float64 if self.precision == 0 else np.float32)
+        n = ctx.field_variants(self.precision)
+        best, ms = C.c_int(), (C.c_double * n)()
+        try:
+            ctx.check(self.lib.ncpa_field_tune(ctx.h, self.h, C.c_void_p(out.ptr), reps, C.byref(best), ms, n))
+        finally:
+            if own:   # the tuner has waited for its launches
+                out.free()
+        return best.value, list(ms)
 
     def free(self):
         if self.h:

@@ -43,7 +43,9 @@ EXPORTS = [
     "ncpa_grid_make", "ncpa_profile_sample",
     "ncpa_ess_solve", "ncpa_ess_count", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
     "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d",
-    "ncpa_field2d_stage", "ncpa_field2d_contract", "ncpa_field_stage_free", "ncpa_field_compare", "ncpa_ess_eigen",
+    "ncpa_field2d_stage", "ncpa_field2d_contract", "ncpa_field_stage_free",
+    "ncpa_field_variants", "ncpa_field_variant_info", "ncpa_field_set_variant", "ncpa_field_get_variant", "ncpa_field_tune",
+    "ncpa_field_compare", "ncpa_ess_eigen",
     "ncpa_wmod_solve", "ncpa_wmod_eigen",
     "ncpa_dispersion_write", "ncpa_dispersion_read", "ncpa_dispersion_info", "ncpa_dispersion_arrays",
     "ncpa_dispersion_free",
@@ -140,6 +142,11 @@ def load():
         "ncpa_field2d_stage": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, P(vp)]),
         "ncpa_field2d_contract": (C.c_int, [vp, vp, vp]),
         "ncpa_field_stage_free": (None, [vp]),
+        "ncpa_field_variants": (C.c_int, [C.c_int, ip]),
+        "ncpa_field_variant_info": (C.c_int, [C.c_int, C.c_int, P(C.c_char_p), ip]),
+        "ncpa_field_set_variant": (C.c_int, [vp, C.c_int, C.c_int]),
+        "ncpa_field_get_variant": (C.c_int, [vp, C.c_int, ip]),
+        "ncpa_field_tune": (C.c_int, [vp, vp, vp, C.c_int, ip, dp, C.c_int]),
         "ncpa_field_compare": (C.c_int, [vp, vp, vp, C.c_int, C.c_size_t, C.c_double, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.POINTER(C.c_size_t)]),
         "ncpa_ess_eigen": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int,
