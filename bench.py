@@ -581,7 +581,7 @@ def field_main(args, cpu):
         kname, krg = kvariant[chosen]
         roof = {"kernel": "%s (2-D field contraction, %s; row groups per XCD class %d)" % (kname, PREC_NAMES[chosen], krg),
                 "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": measured_traffic(kname),
+                "frac": achieved / peak, "traffic": measured_traffic("%s rg %d" % (kname, krg)),
                 "algorithmic": "4 x depths x modes x ranges real flops per problem (real x complex)"}
         if res.get("field_mfma_flops"):
             issued = res["field_mfma_flops"] / avg_s / 1e12

@@ -266,7 +266,8 @@ void ncpa_field_stage_free(ncpa_field_stage_t *stage);
  * only in speed.  Each context keeps one variant per precision (initially 0), used by
  * ncpa_field2d and ncpa_field2d_contract on that context.  ncpa_field_tune times every
  * variant of the stage's precision on the staged batch (one warm-up and `reps` contractions
- * each, HIP events on ctx's stream, writing dev_out), keeps the fastest for ctx and returns
+ * each, HIP events on ctx's stream, writing dev_out), keeps the fastest for ctx (variant 0,
+ * the one with the least HBM traffic, unless another is more than 2 % faster) and returns
  * its index in *best (may be NULL) and the average milliseconds per contraction of the
  * first ms_len variants in ms (may be NULL).  Like ncpa_field2d_contract it may run on a
  * context other than the stage's; it waits for its own timings on the host. */

@@ -1404,33 +1404,33 @@ int ncpa_field_tune(ncpa_ctx_t *ctx, ncpa_field_stage_t *stage, void *dev_out, i
     if (!ctx || !stage || !dev_out || reps < 1) return set_error(ctx, NCPA_ERR_ARG, "ncpa_field_tune: bad arguments");
     NCPA_HIP(ctx, hipSetDevice(ctx->device));
     const int nv = contract_variants(stage->precision);
+    std::vector<double> t(nv, 0.0);
     hipEvent_t e0, e1;
     NCPA_HIP(ctx, hipEventCreate(&e0));
     NCPA_HIP(ctx, hipEventCreate(&e1));
-    int pick = 0, rc = NCPA_OK;
-    double best_ms = 0.0;
+    int rc = NCPA_OK;
     for (int v = 0; v < nv && rc == NCPA_OK; ++v) {
         rc = contract(ctx, stage, dev_out, v);   // warm-up
         hipEventRecord(e0, ctx->stream);
         for (int r = 0; r < reps && rc == NCPA_OK; ++r) rc = contract(ctx, stage, dev_out, v);
         hipEventRecord(e1, ctx->stream);
-        float t = 0.0f;
-        if (rc == NCPA_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess) {
-            const double avg = (double)t / reps;
-            if (ms && v < ms_len) ms[v] = avg;
-            if (v == 0 || avg < best_ms) {
-                best_ms = avg;
-                pick = v;
-            }
-        } else if (rc == NCPA_OK) {
+        float el = 0.0f;
+        if (rc == NCPA_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&el, e0, e1) != hipSuccess))
             rc = set_error(ctx, NCPA_ERR_HIP, "ncpa_field_tune: timing failed");
-        }
+        t[v] = (double)el / reps;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (rc != NCPA_OK) return rc;
+    int pick = 0;
+    for (int v = 1; v < nv; ++v)
+        if (t[v] < t[pick]) pick = v;
+    // variant 0 (the lowest HBM traffic of its family) stays unless another is faster by more
+    // than the launch-to-launch spread (~2 %)
+    if (t[pick] > 0.98 * t[0]) pick = 0;
     ctx->field_variant[stage->precision] = pick;
     if (best) *best = pick;
+    for (int v = 0; ms && v < std::min(nv, ms_len); ++v) ms[v] = t[v];
     if (ctx != stage->owner) {
         if (!stage->foreign_done)
             NCPA_HIP(ctx, hipEventCreateWithFlags(&stage->foreign_done, hipEventDisableTiming));

@@ -86,7 +86,12 @@ def test_tune_on_another_context(gpu_ctx, batch, prec):
         best, ms = st.tune(other, out, reps=2)
         print("precision %d: variant ms %s, best %d" % (prec, ["%.3f" % t for t in ms], best), flush=True)
         assert 0 <= best < other.field_variants(prec) and len(ms) == other.field_variants(prec)
-        assert all(t > 0 for t in ms) and ms[best] == min(ms)
+        assert all(t > 0 for t in ms)
+        # the fastest, or variant 0 when no other beats it by more than 2 %
+        if best == 0:
+            assert min(ms) >= 0.98 * ms[0]
+        else:
+            assert ms[best] == min(ms) and ms[best] < 0.98 * ms[0]
         assert other.field_variant(prec) == best and gpu_ctx.field_variant(prec) == 0
         got = st.contract(other)
         done = other.record()
