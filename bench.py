@@ -33,8 +33,10 @@ reference's rule) with a Gaussian-windowed 0.5 Hz source given as a spectrum.
 
 The default run (field workload) then adds the rest of the metric: "latency" (p50/p99 of the
 continuous-batching server at the production precision under Poisson arrivals at 100 and 500
-requests/s) and "broadband_job" (the batched ModBB job and its CPU comparison);
---headline-only skips them.
+requests/s), "broadband_job" (the batched ModBB job and its CPU comparison) and
+"precision_2hz" (the --workload precision comparison, 10 launches per precision);
+--headline-only skips them.  Before timing, every context that contracts runs the contraction
+autotuner (ncpa_field_tune) on one solve of the batch ("contraction_tuning").
 
 Prints one JSON line.  cpu_baseline (field and broadband): the oracle's C restatement of the
 reference loops (oracle/ess_port.c, gcc -O2, FP64) on the host cores that go with one GPU
@@ -1133,6 +1135,14 @@ def main():
                                                          "cpu_baseline", "plan") if k in bbl}
             if "speedup_vs_cpu_all_cores" in bbl:
                 line["broadband_job"]["speedup_vs_cpu_all_cores"] = bbl["speedup_vs_cpu_all_cores"]
+            # the mixed-precision mode sum against the FP64 contraction at 2 Hz (config 3)
+            pr_args = argparse.Namespace(**vars(args))
+            pr_args.steps, pr_args.warmup = 10, 2
+            prl = precision_main(pr_args)
+            line["precision_2hz"] = {"config": prl["config"]["workload"], "chosen": prl["dtype"],
+                                     "speedup_vs_fp64": prl["precisions"][prl["dtype"].split("+")[1]]["speedup_vs_fp64"],
+                                     "precisions": {k: {kk: v[kk] for kk in ("ms", "speedup_vs_fp64", "max_db", "rms_db")}
+                                                    for k, v in prl["precisions"].items()}}
     elif args.workload == "wmod":
         line = wmod_main(args, cpu)
     else:
