@@ -159,6 +159,15 @@ struct ModesSharedN {
 };
 using ModesShared = ModesSharedN<2 * NCPA_RING>;
 
+// Next shift of a lane whose bracket [lo, hi) holds the eigenvalues cl .. ch-1 (j among
+// them): the midpoint when j is alone, else the (j - cl + 1)-th of ch - cl evenly spaced
+// interior points, so the lanes that share a bracket cut it into ch - cl + 1 parts between
+// them instead of all testing its midpoint.
+__device__ __forceinline__ double next_shift(double lo, double hi, int cl, int ch, int j) {
+    const int k = ch - cl;
+    return k > 1 ? lo + (double)(j - cl + 1) * ((hi - lo) / (double)(k + 1)) : 0.5 * (lo + hi);
+}
+
 template <int MS, class Pen, class Sh>
 __device__ __forceinline__ void modes_item(const ModeProb &P, const Pen &pen, const ModeWork wk, Sh &sh_,
                                            const ModesOut &out) {
@@ -195,7 +204,7 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const Pen &pen, co
         blo = __shfl(lo, 0);
         bhi = __shfl(hi, last);
     }
-    double sig = 0.5 * (lo + hi);
+    double sig = next_shift(lo, hi, cl, ch, j);
     bool done = !active;
     int my_iters = MS;
     bool collapsed = false;
@@ -250,7 +259,7 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const Pen &pen, co
                     ch = cc;
                 }
             }
-            sig = 0.5 * (lo + hi);
+            sig = next_shift(lo, hi, cl, ch, j);
             double step = INFINITY;
             if (newton && q0 != 0.0 && isfinite(p0) && isfinite(q0)) {
                 const double dlt = -p0 / q0;
@@ -259,13 +268,16 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const Pen &pen, co
                 // Converged when the step is at the tolerance, or (pencils with
                 // Pen::early_accept) when two successive Newton steps show the quadratic
                 // regime (ratio below 1e-3) and predict the next step, |dlt|^3 / |dlt_prev|^2,
-                // at the tolerance: the sweep that would only confirm it is skipped.  The
-                // prediction takes the previous step as asymptotic; det T(k) of the wide-angle
-                // pencil has other zero branches close enough to break that, so it waits for
-                // the step itself.
+                // a thousandfold below the tolerance: the sweep that would only confirm it is
+                // skipped.  The prediction takes the previous step as asymptotic, which it is
+                // not when that step came from far off (the first Newton step from a wide
+                // bracket): accepted at the tolerance itself, such a lane stopped 300x the
+                // tolerance short; the factor covers it.  det T(k) of the wide-angle pencil has
+                // other zero branches close enough to break the prediction, so it waits for the
+                // step itself.
                 const double rat = adl / prev_step;   // 0 after a bisection: no estimate yet
                 if (adl <= P.tol ||
-                    (Pen::early_accept && prev_step < INFINITY && rat <= 1e-3 && adl * rat * rat <= P.tol)) {
+                    (Pen::early_accept && prev_step < INFINITY && rat <= 1e-3 && 1e3 * adl * rat * rat <= P.tol)) {
                     sig = cand;
                     done = true;
                     my_iters = MS + it + 1;
