@@ -81,3 +81,36 @@ def test_small_grids_vs_oracle(gpu_ctx, synth_profile, wide, Nz):
     assert np.allclose(k.imag, sol.k_pert.imag, rtol=1e-6, atol=1e-16)
     ref = sol.psi[sol.i_s] * sol.psi[sol.i_r]
     assert np.max(np.abs(psr - ref)) <= 1e-7 * np.max(np.abs(ref))
+
+
+def test_eigenvalues_to_working_precision(gpu_ctx, synth_profile, tmp_path):
+    """Every eigenvalue of an 8-problem 1 Hz batch (Nz 20000, ~4000 modes) within 8 tolerances
+    (8 x 8 eps ||dz^2 T||) of LAPACK's bisection, in dz^2 k^2: the early Newton acceptance must
+    not stop a lane short of working precision (accepting the predicted next step at the
+    tolerance itself left one of these eigenvalues 300 tolerances off)."""
+    import ncpamodes
+    from ncpamodes import synth
+    from oracle import ess
+    from oracle.atmos import read_profile
+    print("case eigenvalues at working precision, 1 Hz x 8", flush=True)
+    paths = [synth_profile, synth.write_profile(str(tmp_path / "P3.dat"), seed=3)]
+    pids = [gpu_ctx.read_profile(p)[0] for p in paths]
+    g = gpu_ctx.grid(pids[0])
+    azs = [0.0, 90.0, 180.0, 270.0]
+    probs = [ncpamodes.problem(pid, 1.0, az) for pid in pids for az in azs]
+    m = gpu_ctx.solve(g, probs)
+    worst = 0.0
+    for i, (path, az) in enumerate([(p, az) for p in paths for az in azs]):
+        grid = ess.EssGrid(ess.AtmosphereNodes(read_profile(path)), g.Nz, 150.0, 0.0)
+        grid.set_attenuation(1.0, None)
+        diag, _, kmin, kmax, _ = ess.modal_trace(grid, 1.0, az, 0.0, 0.0, True, 0.0)
+        w, _ = ess.eigen(diag, grid.dz, kmin, kmax, vectors=False)
+        k, _, _, _ = m.problem_modes(i)
+        assert k.size == w.size
+        h2 = grid.dz ** 2
+        tol = 8 * np.finfo(float).eps * (4.0 + h2 * max(abs(kmin ** 2), abs(kmax ** 2)))
+        err = np.max(np.abs((k ** 2).real - w)) * h2
+        worst = max(worst, err / tol)
+    m.free()
+    print("largest error %.2f tolerances" % worst, flush=True)
+    assert worst <= 8.0
