@@ -251,12 +251,26 @@ __global__ void k_counts2(const double *__restrict__ a, int n, double lo, double
 }
 
 
-// dz^2 T - s with T the effective-sound-speed operator: one diagonal array
+// dz^2 T - s with T the effective-sound-speed operator: one diagonal array.  Sweeps cover rows
+// n_eff-1 .. 0: once the block's eigenvalues are known to lie above s_min, narrow() leaves out
+// the top rows that are evanescent at every shift >= s_min (evanescent_rows: counts exact,
+// eigenvalues unchanged to rounding).  In the broadband batch that is ~23 % of the rows.
 struct EssPencil {
     static constexpr bool early_accept = true;
     const ModeProb &p;
-    __device__ __forceinline__ int count(double *ring, double s) const { return sturm_count(p.a, ring, p.n, s); }
-    __device__ __forceinline__ NewtonOut newton(double *ring, double s) const { return sturm_newton(p.a, ring, p.n, s); }
+    int n_eff;
+    __device__ __forceinline__ int count(double *ring, double s) const {
+        return sturm_count(p.a, ring, n_eff, s) + (p.n - n_eff);
+    }
+    __device__ __forceinline__ NewtonOut newton(double *ring, double s) const {
+        NewtonOut o = sturm_newton(p.a, ring, n_eff, s);
+        o.cnt += p.n - n_eff;
+        return o;
+    }
+    // (the final sweep records phi at rows srow + 1 and rrow + 1: those stay inside)
+    __device__ __forceinline__ void narrow(double s_min) {
+        n_eff = evanescent_rows(p.a, p.n, s_min - 16.0 * p.tol, 80.0, max(p.srow, p.rrow) + 2);
+    }
     // rows pass: shift of a mode with eigenvalue k2, row values a_i - s, q_0 unscaled
     __device__ __forceinline__ double shift(double k2) const { return k2 * p.h2; }
     __device__ __forceinline__ double qscale(double) const { return 1.0; }
@@ -271,7 +285,7 @@ struct EssPencil {
     // phase 3: sensitivities at the converged eigenvalue (every lane, one sweep)
     __device__ __forceinline__ void finish(const ModeProb &P, double *s_ring, double sig, bool collapsed, int nclu,
                                            int j, bool active, int my_iters, const ModesOut &out) const {
-        const FinalOut f = sturm_final(P.a, P.w, s_ring, P.n, sig, P.srow, P.rrow);
+        const FinalOut f = sturm_final(P.a, P.w, s_ring, n_eff, sig, P.srow, P.rrow);
         SVal th_s, th_r;
         theta_forward(P.a, s_ring, sig, P.srow, P.rrow, th_s, th_r);
         if (!active) return;   // (whole-wave work above is done; nothing of this lane is left)
@@ -313,7 +327,8 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob >= 0) {
         const ModeProb &P = mp[wk.prob];
-        modes_item<MS>(P, EssPencil{P}, wk, sh, out);
+        EssPencil pen{P, P.n};
+        modes_item<MS>(P, pen, wk, sh, out);
     }
 }
 
@@ -334,7 +349,8 @@ __global__ void __launch_bounds__(64) k_modes_persistent(const ModeProb *__restr
         const ModeWork wk = work[w];
         if (wk.prob >= 0) {
             const ModeProb &P = mp[wk.prob];
-            modes_item<MS>(P, EssPencil{P}, wk, sh, out);
+            EssPencil pen{P, P.n};
+            modes_item<MS>(P, pen, wk, sh, out);
         }
         __syncthreads();
     }

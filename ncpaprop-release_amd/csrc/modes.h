@@ -147,7 +147,8 @@ __device__ inline void interval_bounds(const ProbDev &pr, int Nz, double dz, con
 //
 // The search is written once for any pencil whose count of eigenvalues below the shift
 // is non-decreasing in the shift: `pen` supplies count(ring, s), newton(ring, s) (count,
-// det and d det/ds) and finish(...) (phase 3 and the outputs).  EssPencil is dz^2 T - s;
+// det and d det/ds), narrow(s_min) (told once, after the first multisection pass, that no
+// later shift lies below s_min) and finish(...) (phase 3 and the outputs).  EssPencil is dz^2 T - s;
 // the wide-angle pencil (wide.h) is dz^2 T(k) with the shift k inside the diagonal.
 template <int RING>
 struct ModesSharedN {
@@ -169,7 +170,7 @@ __device__ __forceinline__ double next_shift(double lo, double hi, int cl, int c
 }
 
 template <int MS, class Pen, class Sh>
-__device__ __forceinline__ void modes_item(const ModeProb &P, const Pen &pen, const ModeWork wk, Sh &sh_,
+__device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const ModeWork wk, Sh &sh_,
                                            const ModesOut &out) {
     double *ms_s = sh_.ms_s, *hb_lo = sh_.hb_lo, *hb_hi = sh_.hb_hi, *s_ring = sh_.ring;
     int *ms_c = sh_.ms_c, *h_list = sh_.h_list;
@@ -203,6 +204,10 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, const Pen &pen, co
         __syncthreads();
         blo = __shfl(lo, 0);
         bhi = __shfl(hi, last);
+        // every later shift of the block lies above lane 0's lower bound (to within the
+        // tolerance of a Newton step accepted just outside a bracket): the pencil may drop
+        // what no such shift needs (narrow)
+        if (pass == 0) pen.narrow(blo);
     }
     double sig = next_shift(lo, hi, cl, ch, j);
     bool done = !active;

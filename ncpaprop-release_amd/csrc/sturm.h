@@ -391,6 +391,62 @@ __device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w
     return o;
 }
 
+// Rows an eigenvalue sweep may leave out at the top of T' - s for every shift s >= s_min.
+// Returns n_eff: sweeps run rows n_eff-1 .. 0 from phi_{n_eff} = 1, phi_{n_eff+1} = 0 and add
+// n - n_eff to the count.
+//  * Counts stay exact.  Every row i >= t0 has fl(a'_i - s_min) <= -2 (the subtraction the
+//    sweep itself performs, monotone in s), so in the full recurrence from the top each pivot
+//    D_i = (a'_i - s) - 1/D_{i+1} is below -1 (D_n = a'_{n-1} - s <= -2; D_{i+1} < -1 gives
+//    -1/D_{i+1} < 1): exactly one negative pivot per row above t0, the n - n_eff added back.
+//  * The truncated start replaces the true pivot entering row n_eff - 1 by infinity.  Down the
+//    evanescent rows t0 .. n_eff-1 that difference is multiplied by 1/D_i^2 per row, where
+//    |D_i| tracks the growth factor r_i of the decaying solution, r_i + 1/r_i = 2 + x_i,
+//    x_i = s - a'_i - 2: ln(1/D_i^2) ~ -2 acosh(1 + x_i/2).  The cut is placed where that sum,
+//    counted upwards from t0, first reaches `decay` (80: e^-80, so that even a WKB estimate off
+//    by a factor two leaves the pivot at t0 exact to far below rounding).  Eigenvalues then
+//    move by less than rounding; the left-out rows carry eigenvector components below
+//    e^(-decay/2) of those at t0.
+//  * n_eff >= min_keep (rows the caller records), and n when no cut qualifies.
+// Whole-wave call (64 lanes), reads a' straight from global memory: ~(n - t0) / 64 loads per
+// lane, a fraction of one sweep.
+__device__ __forceinline__ int evanescent_rows(const double *a, int n, double s_min, double decay, int min_keep) {
+    const int lane = (int)(threadIdx.x & 63);
+    // t0: one above the highest row that is not evanescent at s_min
+    int t0 = 0;
+    for (int top = n - 1; top >= 0; top -= 64) {
+        const int i = top - lane;
+        const bool osc = i >= 0 && !(a[i] - s_min <= -2.0);   // (NaN counts as oscillatory)
+        const unsigned long long m = __ballot(osc);
+        if (m) {
+            t0 = top - (__ffsll((unsigned long long)m) - 1) + 1;
+            break;
+        }
+    }
+    int cut = n;   // highest row kept
+    double carry = 0.0;
+    for (int base = t0; base < n; base += 64) {
+        const int i = base + lane;
+        double d = 0.0;
+        if (i < n) {
+            const double y = 0.5 * (-2.0 - (a[i] - s_min));   // x_i / 2 >= 0
+            d = 2.0 * log1p(y + sqrt(y * (y + 2.0)));          // 2 acosh(1 + y)
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double v = __shfl_up(d, o);
+            if (lane >= o) d += v;
+        }
+        const unsigned long long m = __ballot(carry + d >= decay);
+        if (m) {
+            cut = base + __ffsll((unsigned long long)m) - 1;
+            break;
+        }
+        carry += __shfl(d, 63);
+    }
+    const int n_eff = cut < n ? cut + 1 : n;
+    return min(n, max(n_eff, min_keep));
+}
+
 // forward continuants theta_{s-1}, theta_{r-1} (theta_{-1} = 1); rows 0 .. max(s,r)-1
 __device__ __forceinline__ void theta_forward(const double *a, double *ring, double s, int srow, int rrow, SVal &th_s,
                                               SVal &th_r) {

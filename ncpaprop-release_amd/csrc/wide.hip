@@ -101,6 +101,8 @@ struct WidePencil {
 
     __device__ __forceinline__ int count(double *ring, double k) const { return s.count(ring, k); }
     __device__ __forceinline__ NewtonOut newton(double *ring, double k) const { return s.newton(ring, k); }
+    // (every row is swept: the wide-angle sweeps leave none out)
+    __device__ __forceinline__ void narrow(double) {}
 
     // phase 3 at the converged k: psi = u / sqrt(dz (1 + k^2)), absorption u^T W u / (1 + k^2)
     __device__ __forceinline__ void finish(const ModeProb &P, double *ring, double k, bool collapsed, int nclu,
@@ -226,7 +228,7 @@ __global__ void __launch_bounds__(64) k_wmodes(const ModeProb *__restrict__ mp, 
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob < 0) return;
     const ModeProb &P = mp[wk.prob];
-    const WidePencil<T, PQ> pen{wide_sweeps<T, PQ>(P, wk.prob, BC, plane, PQp, beta)};
+    WidePencil<T, PQ> pen{wide_sweeps<T, PQ>(P, wk.prob, BC, plane, PQp, beta)};
     modes_item<MS>(P, pen, wk, sh, out);
 }
 
