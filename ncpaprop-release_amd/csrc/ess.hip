@@ -271,8 +271,12 @@ struct EssPencil {
     __device__ __forceinline__ void narrow(double s_min) {
         n_eff = evanescent_rows(p.a, p.n, s_min - 16.0 * p.tol, 80.0, max(p.srow, p.rrow) + 2);
     }
-    // rows pass: shift of a mode with eigenvalue k2, row values a_i - s, q_0 unscaled
+    // rows pass: shift of a mode with eigenvalue k2, row values a_i - s, q_0 unscaled; rows
+    // swept for modes whose shifts are >= s_min (the source row stays inside)
     __device__ __forceinline__ double shift(double k2) const { return k2 * p.h2; }
+    __device__ __forceinline__ int rows_kept(double s_min) const {
+        return evanescent_rows(p.a, p.n, s_min - 16.0 * p.tol, 80.0, p.srow + 2);
+    }
     __device__ __forceinline__ double qscale(double) const { return 1.0; }
     template <class Step, class ChunkEnd>
     __device__ __forceinline__ void up(double *ring, int m, double s, Step step, ChunkEnd ce) const {
@@ -485,7 +489,7 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
-    rows_item(P, EssPencil{P}, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec);
+    rows_item(P, EssPencil{P, P.n}, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec);
 }
 __device__ __forceinline__ double log2abs_s(double v, int e) { return log2(fabs(v)) + (double)e; }
 

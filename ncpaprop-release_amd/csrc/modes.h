@@ -147,8 +147,8 @@ __device__ inline void interval_bounds(const ProbDev &pr, int Nz, double dz, con
 //
 // The search is written once for any pencil whose count of eigenvalues below the shift
 // is non-decreasing in the shift: `pen` supplies count(ring, s), newton(ring, s) (count,
-// det and d det/ds), narrow(s_min) (told once, after the first multisection pass, that no
-// later shift lies below s_min) and finish(...) (phase 3 and the outputs).  EssPencil is dz^2 T - s;
+// det and d det/ds), narrow(s_min) (told after each multisection pass that no later shift
+// lies below s_min) and finish(...) (phase 3 and the outputs).  EssPencil is dz^2 T - s;
 // the wide-angle pencil (wide.h) is dz^2 T(k) with the shift k inside the diagonal.
 template <int RING>
 struct ModesSharedN {
@@ -206,8 +206,8 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
         bhi = __shfl(hi, last);
         // every later shift of the block lies above lane 0's lower bound (to within the
         // tolerance of a Newton step accepted just outside a bracket): the pencil may drop
-        // what no such shift needs (narrow)
-        if (pass == 0) pen.narrow(blo);
+        // what no such shift needs (narrow; after every pass, as the bound rises)
+        pen.narrow(blo);
     }
     double sig = next_shift(lo, hi, cl, ch, j);
     bool done = !active;
@@ -338,7 +338,10 @@ __device__ __forceinline__ void rows_item(const ModeProb &P, const Pen &pen, con
     const bool active = j < P.n_hi;
     const size_t o = (size_t)P.out_off + (size_t)(active ? (j - P.n_lo) : 0);
     const double s = active ? pen.shift(k2[o]) : 0.5 * (P.lo + P.hi);   // the pencil's shift
-    const int n = P.n;
+    // rows n_eff .. n-1 lie in the evanescent top of every mode of the block (lane 0 holds its
+    // lowest eigenvalue): their components are below e^-40 of those at the top of the
+    // oscillatory part and are written as zeros; the sweeps start at row n_eff - 1
+    const int n = pen.rows_kept(__shfl(s, 0));
     const size_t src = (size_t)ra.nrec * ra.ld + o;   // slot of the source row
     const int last_rec = ra.row0 + (ra.nrec - 1) * ra.stride;
     // forward: theta_k for k = 0 .. n-2, recorded at row k+1
@@ -352,6 +355,11 @@ __device__ __forceinline__ void rows_item(const ModeProb &P, const Pen &pen, con
             next = ra.nrec > 1 ? ra.row0 + ra.stride : INT_MAX;
         }
         if (P.srow == 0 && active) { TH[src] = 1.0; THE[src] = 0; }
+        for (int k = slot, row = next; k < ra.nrec; ++k, row += ra.stride)   // rows left out
+            if (row >= n && active) {
+                TH[(size_t)k * ra.ld + o] = 0.0;
+                THE[(size_t)k * ra.ld + o] = 0;
+            }
         pen.up(
             s_ring, n - 1, s,
             [&](double am, int k) {
@@ -382,6 +390,12 @@ __device__ __forceinline__ void rows_item(const ModeProb &P, const Pen &pen, con
         double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
         int ex = 0;
         int slot = ra.nrec - 1, next = last_rec;
+        for (; slot >= 0 && next >= n; --slot, next -= ra.stride)   // rows left out
+            if (active) {
+                PH[(size_t)slot * ra.ld + o] = 0.0;
+                PHE[(size_t)slot * ra.ld + o] = 0;
+            }
+        if (slot < 0) next = INT_MIN;
         if (next == n - 1) {
             if (active) { PH[(size_t)slot * ra.ld + o] = 1.0; PHE[(size_t)slot * ra.ld + o] = 0; }
             --slot;
