@@ -27,6 +27,8 @@ __host__ __device__ inline double sgn1(double v) { return v < 0.0 ? -1.0 : 1.0; 
 // as AtmosphericProperty1D::get does (src/atmosphere/AtmosphericProperty1D.cpp:116-126).
 template <class YF>
 __host__ __device__ inline double steffen_eval_at(const double *z, int n, YF yf, double x, int k);
+template <class ZF, class YF>
+__host__ __device__ inline double steffen_eval_zf(ZF z, int n, YF yf, double x, int k);
 
 template <class YF>
 __host__ __device__ inline double steffen_eval(const double *z, int n, YF yf, double x) {
@@ -35,9 +37,21 @@ __host__ __device__ inline double steffen_eval(const double *z, int n, YF yf, do
     return steffen_eval_at(z, n, yf, x, bsearch_interval(z, n, x));
 }
 
-// the same, on interval k = bsearch_interval(z, n, x) found by the caller (x inside the nodes)
-template <class YF>
-__host__ __device__ inline double steffen_eval_at(const double *z, int n, YF yf, double x, int k) {
+// node abscissae read through an accessor (global arrays, or a window staged in LDS)
+struct PtrNodes {
+    const double *z;
+    __host__ __device__ double operator[](int k) const { return z[k]; }
+};
+
+// Hermite coefficients of Steffen's cubic on interval k: y(x) = y + dx (yp + dx (b + dx a)),
+// dx = x - z_k.  Split from the evaluation so that a kernel can form them once per interval
+// and evaluate many samples with the same arithmetic.
+struct SteffenCoef {
+    double y, yp, b, a;
+};
+
+template <class ZF, class YF>
+__host__ __device__ inline SteffenCoef steffen_coef(ZF z, int n, YF yf, int k) {
     const double hk = z[k + 1] - z[k];
     const double yk = yf(k), yk1 = yf(k + 1);
     const double sk = (yk1 - yk) / hk;
@@ -58,10 +72,27 @@ __host__ __device__ inline double steffen_eval_at(const double *z, int n, YF yf,
         const double p = (sk * hp + sp * hk) / (hk + hp);
         ypk1 = (sgn1(sk) + sgn1(sp)) * fmin(fabs(sk), fmin(fabs(sp), 0.5 * fabs(p)));
     }
-    const double a = (ypk + ypk1 - 2.0 * sk) / hk / hk;
-    const double b = (3.0 * sk - 2.0 * ypk - ypk1) / hk;
-    const double dx = x - z[k];
-    return yk + dx * (ypk + dx * (b + dx * a));
+    SteffenCoef c;
+    c.y = yk;
+    c.yp = ypk;
+    c.a = (ypk + ypk1 - 2.0 * sk) / hk / hk;
+    c.b = (3.0 * sk - 2.0 * ypk - ypk1) / hk;
+    return c;
+}
+
+__host__ __device__ inline double steffen_poly(const SteffenCoef &c, double dx) {
+    return c.y + dx * (c.yp + dx * (c.b + dx * c.a));
+}
+
+// the same, on interval k = bsearch_interval(z, n, x) found by the caller (x inside the nodes)
+template <class ZF, class YF>
+__host__ __device__ inline double steffen_eval_zf(ZF z, int n, YF yf, double x, int k) {
+    return steffen_poly(steffen_coef(z, n, yf, k), x - z[k]);
+}
+
+template <class YF>
+__host__ __device__ inline double steffen_eval_at(const double *z, int n, YF yf, double x, int k) {
+    return steffen_eval_zf(PtrNodes{z}, n, yf, x, k);
 }
 
 // Steffen first derivative (AtmosphericProperty1D.cpp:128-136: zero outside the range)

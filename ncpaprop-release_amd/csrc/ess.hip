@@ -167,43 +167,109 @@ __device__ __forceinline__ double spline_at(const double *zn, int n, const doubl
     return steffen_eval_at(zn, n, ArrNode{y}, z, k);
 }
 
-__global__ void k_setup_grid(const ProbDev *__restrict__ probs, int Nz, double z_min, double dz,
-                             const double *__restrict__ alpha_nodes, const double *__restrict__ ceff_nodes,
-                             int ldn, double *__restrict__ A, double *__restrict__ W,
-                             double *__restrict__ CE) {
+// One row of dz^2 T, the absorption weight and c_eff at z = z_min + i dz from the spline
+// values of c_eff, alpha, P and rho at the row (`at(f)`).
+template <class At>
+__device__ __forceinline__ void setup_row(const ProbDev &pr, double dz, int i, At at, double *__restrict__ A,
+                                          double *__restrict__ W, double *__restrict__ CE, size_t o) {
+    const double h2 = dz * dz;
+    const double ceff = at(0);
+    // row of dz^2 T: -2 + dz^2 diag, with diag = (omega/c_eff)^2 + the ground term on row 0
+    // (ESSModeSolver.cpp:544-559, EigenEngine.cpp:57-79)
+    double diag = h2 * ((pr.omega / ceff) * (pr.omega / ceff));
+    if (i == 0) diag = 1.0 / (dz * pr.admittance + 1) + diag;
+    const double a = -2.0 + diag;
+    const double alpha = at(1);
+    const double Pz = at(2);
+    const double Rz = at(3);
+    const double cT = sqrt(1.4 * Pz / Rz);
+    A[o] = a;
+    // u^2 (omega/c_T) alpha 2 == dz psi^2 (omega/c_T) alpha 2 with psi = u/sqrt(dz)
+    W[o] = (pr.omega / cT) * alpha * 2.0;
+    CE[o] = ceff;
+}
+
+// Grid samples of every problem: block (x, p) takes rows [1024 x, 1024 x + 1024) of problem p.
+// The Steffen coefficients of the node intervals those rows fall in (~80 for 0.1 km nodes and
+// Nz = 20000) are formed once per block into LDS, with the node abscissae: one binary search
+// in global memory per block instead of one per row, and per row a search in LDS and three
+// fma per property instead of a spline evaluation with eight divisions.  A window of more
+// than NCPA_SETUP_IV intervals (nodes much finer than the grid) falls back to evaluating each
+// row from global memory.  The coefficients are those steffen_eval_zf forms, so both paths
+// give the same samples.  Values outside the nodes are the end values
+// (AtmosphericProperty1D.cpp:116-126).
+#define NCPA_SETUP_ROWS 1024
+#define NCPA_SETUP_IV 128
+__global__ void __launch_bounds__(256) k_setup_grid(const ProbDev *__restrict__ probs, int Nz, double z_min,
+                                                    double dz, const double *__restrict__ alpha_nodes,
+                                                    const double *__restrict__ ceff_nodes, int ldn,
+                                                    double *__restrict__ A, double *__restrict__ W,
+                                                    double *__restrict__ CE) {
+    __shared__ SteffenCoef s_cf[4][NCPA_SETUP_IV];   // c_eff, alpha, P, rho per interval
+    __shared__ double s_z[NCPA_SETUP_IV + 1];
+    __shared__ int s_k[2];
     const int p = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Nz) return;
+    const int r0 = blockIdx.x * NCPA_SETUP_ROWS, r1 = min(Nz, r0 + NCPA_SETUP_ROWS);
+    const ProbDev pr = probs[p];
+    const int n = pr.n_nodes;
+    const double *zn = pr.prof + PF_Z * n;
+    const double *yg[4] = {ceff_nodes + size_t(p) * ldn, alpha_nodes + size_t(p) * ldn, pr.prof + PF_P * n,
+                           pr.prof + PF_RHO * n};
+    const double z0 = zn[0], z1 = zn[n - 1];
+    if (threadIdx.x == 0) s_k[0] = bsearch_interval(zn, n, z_min + r0 * dz);
+    if (threadIdx.x == 64) s_k[1] = bsearch_interval(zn, n, z_min + (r1 - 1) * dz);
+    __syncthreads();
+    const int k_lo = s_k[0], k_hi = s_k[1], ni = k_hi - k_lo + 1;
+    const size_t off = size_t(p) * Nz;
+    if (ni <= NCPA_SETUP_IV) {
+        for (int t = threadIdx.x; t < 4 * ni; t += blockDim.x) {
+            const int f = t / ni, k = k_lo + t % ni;
+            s_cf[f][k - k_lo] = steffen_coef(PtrNodes{zn}, n, ArrNode{yg[f]}, k);
+        }
+        for (int t = threadIdx.x; t <= ni; t += blockDim.x) s_z[t] = zn[min(k_lo + t, n - 1)];
+        __syncthreads();
+        for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+            const double z = z_min + i * dz;
+            // largest k in [k_lo, k_hi] with z_k <= z (what bsearch_interval returns for z)
+            int lo = 0, hi = ni;
+            while (hi > lo + 1) {
+                const int mid = (hi + lo) >> 1;
+                if (s_z[mid] > z) hi = mid; else lo = mid;
+            }
+            const double dx = z - s_z[lo];
+            setup_row(pr, dz, i,
+                      [&](int f) {
+                          if (z > z1) return yg[f][n - 1];
+                          if (z < z0) return yg[f][0];
+                          return steffen_poly(s_cf[f][lo], dx);
+                      },
+                      A, W, CE, off + i);
+        }
+    } else {
+        for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+            const double z = z_min + i * dz;
+            const int k = bsearch_interval(zn, n, z);
+            setup_row(pr, dz, i, [&](int f) { return spline_at(zn, n, yg[f], z, k); }, A, W, CE, off + i);
+        }
+    }
+}
+
+// The last row of each problem's matrix takes diag[Nz-2] (the stale value[1] of the
+// assembly loop, EigenEngine.cpp:57-71): one thread per problem, after k_setup_grid.
+__global__ void k_setup_last_row(const ProbDev *__restrict__ probs, int P, int Nz, double z_min, double dz,
+                                 const double *__restrict__ ceff_nodes, int ldn, double *__restrict__ A) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || Nz < 2) return;
     const ProbDev pr = probs[p];
     const int n = pr.n_nodes;
     const double *zn = pr.prof + PF_Z * n;
     const double *cn = ceff_nodes + size_t(p) * ldn;
     const double h2 = dz * dz;
-    const double z = z_min + i * dz;
-    const int k = bsearch_interval(zn, n, z);   // one interval for every property of the row
-    const double ceff = spline_at(zn, n, cn, z, k);
-    // row of dz^2 T: -2 + dz^2 diag, with diag = (omega/c_eff)^2 + the ground term on row 0
-    // (ESSModeSolver.cpp:544-559, EigenEngine.cpp:57-79)
-    double diag = h2 * ((pr.omega / ceff) * (pr.omega / ceff));
-    if (i == 0) diag = 1.0 / (dz * pr.admittance + 1) + diag;
-    double a = -2.0 + diag;
-    if (i == Nz - 1) {
-        // last row takes diag[Nz-2] (stale value[1], EigenEngine.cpp:57-71)
-        const double zp = z_min + (Nz - 2) * dz;
-        const double cp = spline_at(zn, n, cn, zp, bsearch_interval(zn, n, zp));
-        double dp = h2 * ((pr.omega / cp) * (pr.omega / cp));
-        if (Nz - 2 == 0) dp = 1.0 / (dz * pr.admittance + 1) + dp;
-        a = -2.0 + dp;
-    }
-    const double alpha = spline_at(zn, n, alpha_nodes + size_t(p) * ldn, z, k);
-    const double Pz = spline_at(zn, n, pr.prof + PF_P * n, z, k);
-    const double Rz = spline_at(zn, n, pr.prof + PF_RHO * n, z, k);
-    const double cT = sqrt(1.4 * Pz / Rz);
-    const size_t o = size_t(p) * Nz + i;
-    A[o] = a;
-    // u^2 (omega/c_T) alpha 2 == dz psi^2 (omega/c_T) alpha 2 with psi = u/sqrt(dz)
-    W[o] = (pr.omega / cT) * alpha * 2.0;
-    CE[o] = ceff;
+    const double zp = z_min + (Nz - 2) * dz;
+    const double cp = spline_at(zn, n, cn, zp, bsearch_interval(zn, n, zp));
+    double dp = h2 * ((pr.omega / cp) * (pr.omega / cp));
+    if (Nz - 2 == 0) dp = 1.0 / (dz * pr.admittance + 1) + dp;
+    A[size_t(p) * Nz + Nz - 1] = -2.0 + dp;
 }
 
 // ------------------------------------------------------------------------------------
@@ -780,9 +846,11 @@ void launch_setup(hipStream_t st, const ProbDev *probs, int P, int max_nodes, co
                   double *d_nodes, double *A, double *W, double *CE) {
     double *ceff_nodes = d_nodes + (size_t)P * max_nodes;
     launch_node_tables(st, probs, P, max_nodes, d_nodes);
-    dim3 gs((grid.Nz + 255) / 256, P);
+    dim3 gs((grid.Nz + NCPA_SETUP_ROWS - 1) / NCPA_SETUP_ROWS, P);
     hipLaunchKernelGGL(k_setup_grid, gs, dim3(256), 0, st, probs, grid.Nz, grid.z_min, grid.dz, d_nodes, ceff_nodes,
                        max_nodes, A, W, CE);
+    hipLaunchKernelGGL(k_setup_last_row, dim3((P + 255) / 256), dim3(256), 0, st, probs, P, grid.Nz, grid.z_min,
+                       grid.dz, ceff_nodes, max_nodes, A);
 }
 
 void launch_solve_front(hipStream_t st, const ProbDev *probs, int P, int max_nodes, const ncpa_grid_t &grid,

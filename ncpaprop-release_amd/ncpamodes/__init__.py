@@ -338,6 +338,12 @@ class Modes:
         self.ctx.check(self.ctx.lib.ncpa_modes_iterations(self.h, it.ctypes.data_as(C.POINTER(C.c_int))))
         return it[:self.total]
 
+    def ceff(self, p):
+        """c_eff [m/s] on the grid rows of problem p (ncpa_modes_ceff)."""
+        out = np.empty(self.grid.Nz)
+        self.ctx.check(self.ctx.lib.ncpa_modes_ceff(self.h, int(p), _dp(out)))
+        return out
+
     def problem_modes(self, p):
         k, psr, ps, pr = self.download()
         s = slice(self.offsets[p], self.offsets[p + 1])
