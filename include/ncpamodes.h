@@ -60,6 +60,13 @@ int ncpa_ctx_set_timing(ncpa_ctx_t *ctx, int enable);
 int ncpa_ctx_kernel_time(ncpa_ctx_t *ctx, const char *kernel, double *total_ms, int *launches);
 int ncpa_ctx_reset_timing(ncpa_ctx_t *ctx);
 
+/* How the eigenvalue sweeps of this context's eager solves (ncpa_ess_solve) read the matrix
+ * diagonal: 0 (default) through an LDS ring, fastest for batches of a few waves per GPU;
+ * 1 through scalar loads, for batches that put several waves on every SIMD.  Both give the
+ * same results to the bit.  (Broadband plans pick their own form in the autotuner.)  No
+ * counterpart in the reference, which has one CPU path. */
+int ncpa_ctx_set_modes_form(ncpa_ctx_t *ctx, int form);
+
 /* Device memory of the context (no counterpart in the reference): live = bytes its
  * allocator has handed out (requests rounded up to 256 B) plus the buffers of
  * ncpa_dev_alloc still allocated; peak = the largest live since the last call with
@@ -429,7 +436,8 @@ int ncpa_bb_source(ncpa_ctx_t *ctx, int kind, const double *f_vec, int n_freq, i
  * counts once to size the mode buffers and, with tune != 0, times the eigenvalue kernel's
  * variants on the batch and keeps the fastest: multisection passes 1-3 x launch form (a
  * persistent grid of 4, 8 or 15 blocks per CU pulling work from per-XCD queues, or one
- * block per work-list slot; grid_blocks = 0 reports the latter).
+ * block per work-list slot with the diagonal read through LDS or by scalar loads;
+ * grid_blocks = 0 and -1 report these two).
  * ------------------------------------------------------------------------------------------ */
 typedef struct ncpa_bb_plan ncpa_bb_plan_t;
 
@@ -464,7 +472,8 @@ int ncpa_bb_plan_result(ncpa_bb_plan_t *plan, const double **dev_wave, const dou
                         const double **dev_kim, const double **dev_psi_sr, const int **dev_offsets,
                         int *total_modes);
 /* The configuration in use and the tuner's measurements: log holds (passes, blocks, ms)
- * triples (blocks 0: one block per work-list slot); *n_log in: capacity in triples, out:
+ * triples (blocks 0 / -1: one block per work-list slot, LDS / scalar loads); *n_log in:
+ * capacity in triples, out:
  * triples measured. */
 int ncpa_bb_plan_tuning(ncpa_bb_plan_t *plan, int *ms_passes, int *grid_blocks, double *modes_ms,
                         double *log, int *n_log);

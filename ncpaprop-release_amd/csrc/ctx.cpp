@@ -161,6 +161,12 @@ int ncpa_ctx_set_timing(ncpa_ctx_t *ctx, int enable) {
     return NCPA_OK;
 }
 
+int ncpa_ctx_set_modes_form(ncpa_ctx_t *ctx, int form) {
+    if (!ctx || form < 0 || form > 1) return NCPA_ERR_ARG;
+    ctx->modes_smem = form == 1;
+    return NCPA_OK;
+}
+
 int ncpa_ctx_kernel_time(ncpa_ctx_t *ctx, const char *kernel, double *total_ms, int *launches) {
     if (!ctx || !kernel) return NCPA_ERR_ARG;
     NCPA_HIP(ctx, hipStreamSynchronize(ctx->stream));

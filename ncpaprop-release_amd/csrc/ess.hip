@@ -321,15 +321,18 @@ __global__ void k_counts2(const double *__restrict__ a, int n, double lo, double
 // n_eff-1 .. 0: once the block's eigenvalues are known to lie above s_min, narrow() leaves out
 // the top rows that are evanescent at every shift >= s_min (evanescent_rows: counts exact,
 // eigenvalues unchanged to rounding).  In the broadband batch that is ~23 % of the rows.
-struct EssPencil {
+// SMEM: the count and Newton sweeps read the diagonal through scalar loads instead of the
+// LDS ring (sweep_down_smem; the final sweep keeps the ring).
+template <bool SMEM = false>
+struct EssPencilT {
     static constexpr bool early_accept = true;
     const ModeProb &p;
     int n_eff;
     __device__ __forceinline__ int count(double *ring, double s) const {
-        return sturm_count(p.a, ring, n_eff, s) + (p.n - n_eff);
+        return sturm_count<SMEM>(p.a, ring, n_eff, s) + (p.n - n_eff);
     }
     __device__ __forceinline__ NewtonOut newton(double *ring, double s) const {
-        NewtonOut o = sturm_newton(p.a, ring, n_eff, s);
+        NewtonOut o = sturm_newton<SMEM>(p.a, ring, n_eff, s);
         o.cnt += p.n - n_eff;
         return o;
     }
@@ -379,6 +382,8 @@ struct EssPencil {
     }
 };
 
+using EssPencil = EssPencilT<false>;
+
 // Two launch forms of the eigenvalue kernel, both over the XCD-interleaved work list
 // (item 8t + x belongs to XCD x; workgroups go to the XCDs round-robin, block b -> b % 8):
 //   k_modes             one block per item, block b takes item b.  The item count comes from
@@ -388,7 +393,7 @@ struct EssPencil {
 //   k_modes_persistent  a fixed grid whose blocks pull items 8t + x, t = queue[x]++, until
 //                       the device count is exhausted.  (Its loop costs registers: ~200
 //                       VGPRs against ~120 for k_modes; the plan's autotuner times both.)
-template <int MS>
+template <int MS, bool SMEM = false>
 __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
                                               int n_work, const int *__restrict__ n_work_dev, ModesOut out) {
     __shared__ ModesShared sh;
@@ -397,7 +402,7 @@ __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, c
     const ModeWork wk = work[blockIdx.x];
     if (wk.prob >= 0) {
         const ModeProb &P = mp[wk.prob];
-        EssPencil pen{P, P.n};
+        EssPencilT<SMEM> pen{P, P.n};
         modes_item<MS>(P, pen, wk, sh, out);
     }
 }
@@ -811,7 +816,16 @@ __global__ void __launch_bounds__(256) k_tl1d(const TlProb *__restrict__ tp, con
 namespace ncpa {
 // k_modes with MS multisection passes (1..3, chosen by the autotuner; 2 by default)
 void launch_k_modes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, int n_work,
-                    const int *n_work_dev, int *queue, const ModesOut &mo) {
+                    const int *n_work_dev, int *queue, const ModesOut &mo, bool smem) {
+    if (smem && !queue) {
+        if (ms <= 1)
+            hipLaunchKernelGGL((k_modes<1, true>), grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, mo);
+        else if (ms == 2)
+            hipLaunchKernelGGL((k_modes<2, true>), grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, mo);
+        else
+            hipLaunchKernelGGL((k_modes<3, true>), grid, dim3(64), 0, st, mp, work, n_work, n_work_dev, mo);
+        return;
+    }
     if (queue) {
         if (ms <= 1)
             hipLaunchKernelGGL(k_modes_persistent<1>, grid, dim3(64), 0, st, mp, work, n_work_dev, queue, mo);
@@ -975,7 +989,7 @@ int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, con
                             m->d_BC, (size_t)P * Nz, m->d_PQ, m->d_beta, mo);
         else
             launch_k_modes(ctx->ms_passes, dim3((unsigned)work.size()), ctx->stream, m->d_mp, m->d_work,
-                           (int)work.size(), nullptr, nullptr, mo);
+                           (int)work.size(), nullptr, nullptr, mo, ctx->modes_smem);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(ctx, e, "k_modes launch");

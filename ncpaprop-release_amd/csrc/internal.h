@@ -85,7 +85,7 @@ void launch_plan_build(hipStream_t st, const ProbDev *probs, const IntervalOut *
 // queue != nullptr: the persistent form (grid = resident blocks); else one block per item,
 // n_work from the host or, when n_work_dev is given, from the device (grid = capacity)
 void launch_k_modes(int ms, dim3 grid, hipStream_t st, const ModeProb *mp, const ModeWork *work, int n_work,
-                    const int *n_work_dev, int *queue, const ModesOut &mo);
+                    const int *n_work_dev, int *queue, const ModesOut &mo, bool smem = false);
 int build_problems(ncpa_ctx *ctx, const ncpa_grid_t *grid, const ncpa_problem_t *problems, int n_problems,
                    ProbDev *hp, int *max_nodes);
 // attenuation and c_eff at the profile nodes: d_nodes[0 .. P ldn) and [P ldn .. 2 P ldn)

@@ -90,14 +90,15 @@ void enqueue_front(ncpa_bb_plan *pl, int cap_modes, int cap_work) {
 }
 
 // blocks > 0: persistent form with that grid; 0: one block per work-list slot (grid =
-// capacity, blocks past the device count exit)
+// capacity, blocks past the device count exit), diagonal through the LDS ring; -1: the same
+// with the count and Newton sweeps reading the diagonal by scalar loads
 void enqueue_modes(ncpa_bb_plan *pl, int ms, int blocks) {
     if (blocks > 0)
         launch_k_modes(ms, dim3((unsigned)blocks), pl->ctx->stream, pl->mp, pl->work, 0, &pl->meta->n_work,
                        pl->meta->queue, modes_out(pl));
     else
         launch_k_modes(ms, dim3((unsigned)pl->cap_work), pl->ctx->stream, pl->mp, pl->work, 0, &pl->meta->n_work,
-                       nullptr, modes_out(pl));
+                       nullptr, modes_out(pl), blocks < 0);
 }
 
 int enqueue_all(ncpa_bb_plan *pl) {
@@ -135,14 +136,15 @@ int overflow_error(ncpa_bb_plan *pl, const PlanMeta &m) {
 // Times every (multisection passes, launch form) candidate on the plan's own batch (front
 // stages once, then k_plan_build + k_modes per candidate; k_plan_build resets the queues)
 // and keeps the fastest.  Launch forms: persistent grids of 4, 8 and 15 blocks per CU (one
-// wave per SIMD, two, the LDS limit) and one block per work-list slot (blocks = 0).
+// wave per SIMD, two, the LDS limit) and one block per work-list slot with the diagonal
+// through the LDS ring (blocks = 0) or through scalar loads (blocks = -1).
 int autotune(ncpa_bb_plan *pl, int n_cu) {
     hipStream_t st = pl->ctx->stream;
     hipEvent_t e0, e1;
     NCPA_HIP(pl->ctx, hipEventCreate(&e0));
     NCPA_HIP(pl->ctx, hipEventCreate(&e1));
     const int ms_cand[] = {1, 2, 3};
-    const int per_cu[] = {0, 4, 8, 15};
+    const int per_cu[] = {0, -1, 4, 8, 15};   // (-1: scalar-load sweeps, one block per slot)
     double best = 1e300;
     int rc = NCPA_OK;
     enqueue_front(pl, pl->cap_modes, pl->cap_work);
@@ -150,7 +152,7 @@ int autotune(ncpa_bb_plan *pl, int n_cu) {
     for (int ms : ms_cand)
         for (int k : per_cu) {
             if (rc) break;
-            const int blocks = n_cu * k;
+            const int blocks = k < 0 ? -1 : n_cu * k;
             launch_plan_build(st, pl->probs, pl->iv, pl->P, pl->grid.Nz, pl->grid.dz, pl->A, pl->W, pl->cap_modes,
                               pl->cap_work, pl->n_sets, pl->n_freq, pl->mp, pl->offsets, pl->work, pl->mbeg,
                               pl->mcnt, pl->meta);

@@ -271,12 +271,73 @@ __device__ __forceinline__ void sweep_up(const double *a, double *ring, int m, S
     }
 }
 
+// ---- the diagonal through the scalar data cache into SGPRs ---------------------------------
+// The alternative to the LDS ring for launches that fill the chip (several waves per SIMD):
+// the rows are loaded by scalar loads (the address is wave-uniform, the array read-only for
+// the kernel's lifetime: the constant address space), so each row's value reaches the VALU as
+// an SGPR operand with no LDS traffic.  Scalar loads return out of order, so the only wait
+// the compiler can place for a chunk is one for every scalar load in flight; left to itself
+// it issues the next chunk's loads and then waits for all of them before the current chunk's
+// first row.  Here the order is fixed: wait for the current chunk (explicit lgkmcnt(0), which
+// the compiler's wait insertion sees), then issue the next chunk's loads, then step through
+// the current chunk while they are in flight, with scheduling barriers between the three.
+typedef const __attribute__((address_space(4))) double *cdptr;
+
+__device__ __forceinline__ void smem_chunk(double (&v)[NCPA_CHUNK], cdptr p) {
+#pragma unroll
+    for (int u = 0; u < NCPA_CHUNK; ++u) v[u] = p[u];
+}
+
+// rows n-1 .. 0 as sweep_down (the n % 16 top rows first, then 16-row chunks downwards)
+template <class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_down_smem(const double *a_, int n, Step step, ChunkEnd chunk_end) {
+    if (n <= 0) return;
+    const cdptr a = (cdptr)a_;
+    const int rem = n % NCPA_CHUNK;
+    if (rem) {
+        for (int i = n - 1; i >= n - rem; --i) step(a[i], i);
+        chunk_end(n - rem, rem);
+    }
+    int c = n - rem - NCPA_CHUNK;   // first row of the current chunk
+    if (c < 0) return;
+    // One loop body with no branch between a chunk's loads and their use (the copy into
+    // `cur` at its end): split over two bodies, the loads were sunk past the exit test into
+    // the body that uses them, right behind their own wait.
+    double cur[NCPA_CHUNK];
+    smem_chunk(cur, a + c);
+    for (;;) {
+        wait_lds();   // lgkmcnt(0): the current chunk has arrived
+        __builtin_amdgcn_sched_barrier(0);
+        double nxt[NCPA_CHUNK];
+        smem_chunk(nxt, a + (c >= NCPA_CHUNK ? c - NCPA_CHUNK : c));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = NCPA_CHUNK - 1; u >= 0; --u) step(cur[u], c + u);
+        chunk_end(c, NCPA_CHUNK);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < NCPA_CHUNK; ++u) cur[u] = nxt[u];
+        if (c < NCPA_CHUNK) break;
+        c -= NCPA_CHUNK;
+    }
+}
+
+// rows n-1 .. 0 through the LDS ring (SMEM false) or scalar loads (SMEM true)
+template <bool SMEM, class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_down_via(const double *a, double *ring, int n, Step step, ChunkEnd chunk_end) {
+    if constexpr (SMEM)
+        sweep_down_smem(a, n, step, chunk_end);
+    else
+        sweep_down(a, ring, n, step, chunk_end);
+}
+
 // count of eigenvalues below s (full-wave entry; s may differ per lane)
+template <bool SMEM = false>
 __device__ __forceinline__ int sturm_count(const double *a, double *ring, int n, double s) {
     double p1 = 1.0, p2 = 0.0;
     int cnt = 0;
     unsigned bits = 0u;   // bit 0: sign of the newest continuant; starts with phi_n = +1
-    sweep_down(
+    sweep_down_via<SMEM>(
         a, ring, n,
         [&](double ai, int) {
             const double p0 = fma(ai - s, p1, -p2);
@@ -300,11 +361,12 @@ struct NewtonOut {
 };
 
 // count plus det and d det/ds (Newton step -p0/q0)
+template <bool SMEM = false>
 __device__ __forceinline__ NewtonOut sturm_newton(const double *a, double *ring, int n, double s) {
     double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
     int cnt = 0;
     unsigned bits = 0u;
-    sweep_down(
+    sweep_down_via<SMEM>(
         a, ring, n,
         [&](double ai, int) {
             const double am = ai - s;

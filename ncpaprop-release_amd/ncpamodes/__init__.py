@@ -78,6 +78,10 @@ class Context:
         self.check(self.lib.ncpa_ctx_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def set_modes_form(self, form):
+        """Eigenvalue sweeps of eager solves: 0 diagonal through LDS, 1 through scalar loads."""
+        self.check(self.lib.ncpa_ctx_set_modes_form(self.h, int(form)))
+
     def reset_timing(self):
         self.check(self.lib.ncpa_ctx_reset_timing(self.h))
 

@@ -36,6 +36,7 @@ NCPA_SRC = {"impulse": 0, "pulse1": 1, "pulse2": 2, "spectrum": 3, "waveform": 4
 EXPORTS = [
     "ncpa_ctx_create", "ncpa_ctx_destroy", "ncpa_ctx_error", "ncpa_ctx_synchronize",
     "ncpa_ctx_stream", "ncpa_ctx_set_timing", "ncpa_ctx_kernel_time", "ncpa_ctx_reset_timing", "ncpa_ctx_memory",
+    "ncpa_ctx_set_modes_form",
     "ncpa_event_record", "ncpa_event_wait", "ncpa_event_synchronize", "ncpa_event_destroy",
     "ncpa_dev_alloc", "ncpa_dev_free", "ncpa_dev_upload", "ncpa_dev_download", "ncpa_dev_copy",
     "ncpa_profile_upload", "ncpa_profile_set_attenuation", "ncpa_profile_clear",
@@ -106,6 +107,7 @@ def load():
         "ncpa_ctx_synchronize": (C.c_int, [vp]),
         "ncpa_ctx_stream": (vp, [vp]),
         "ncpa_ctx_set_timing": (C.c_int, [vp, C.c_int]),
+        "ncpa_ctx_set_modes_form": (C.c_int, [vp, C.c_int]),
         "ncpa_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, dp, ip]),
         "ncpa_ctx_reset_timing": (C.c_int, [vp]),
         "ncpa_ctx_memory": (C.c_int, [vp, C.c_int, P(C.c_size_t), P(C.c_size_t), P(C.c_size_t)]),

@@ -44,6 +44,7 @@ def test_null_arguments_return_codes():
     assert lib.ncpa_event_record(None, ctypes.byref(out)) == _lib.NCPA_ERR_ARG
     assert lib.ncpa_event_wait(None, None) == _lib.NCPA_ERR_ARG
     assert lib.ncpa_event_synchronize(None) == _lib.NCPA_ERR_ARG
+    assert lib.ncpa_ctx_set_modes_form(None, 1) == _lib.NCPA_ERR_ARG
     lib.ncpa_event_destroy(None)
     lib.ncpa_field_stage_free(None)
     assert lib.ncpa_field2d_contract(None, None, None) == _lib.NCPA_ERR_ARG
