@@ -534,6 +534,7 @@ def field_main(args, cpu):
     tmp = tempfile.mkdtemp(prefix="ncpa_bench_")
     paths = write_profiles(tmp)
     ctx = ncpamodes.Context(0)
+    ctx.set_modes_form(args.modes_form)
     if args.copy_calibration:
         copy_calibration(ctx, args.copy_calibration)
     pids = [ctx.read_profile(p)[0] for p in paths]
@@ -546,6 +547,7 @@ def field_main(args, cpu):
     n_lanes = args.streams if args.schedule == "streams" else 1
     for _ in range(n_lanes - 1):
         c2 = ncpamodes.Context(0)
+        c2.set_modes_form(args.modes_form)
         p2 = [c2.read_profile(p)[0] for p in paths]
         g2 = c2.grid(p2[0], Nz=NZ)
         lanes.append((c2, [ncpamodes.problem(pid, FREQ, az) for pid in p2 for az in AZIMUTHS], g2))
@@ -624,6 +626,7 @@ def field_main(args, cpu):
             "field_depths": FIELD_ROWS,
             "modes_per_step": res["total_modes"],
             "field_precision": PREC_NAMES[chosen],
+            "eigen_sweeps": "diagonal via scalar loads" if args.modes_form else "diagonal via LDS ring",
             "device_streams": multi["streams"],
             "schedule": ("two-stage pipeline: eigenpairs, 1-D TL, rows and field operand staging of step i "
                          "on one stream while step i-1's field contraction runs on the other"
@@ -1093,6 +1096,8 @@ def main():
                     help="field workload with --schedule streams: batches in flight on this many streams")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run the plain copy kernel over MIB MiB first (byte-counter calibration)")
+    ap.add_argument("--modes-form", type=int, choices=[0, 1], default=0,
+                    help="field workload: eigenvalue sweeps read the diagonal through LDS (0) or scalar loads (1)")
     ap.add_argument("--headline-only", action="store_true",
                     help="field workload: skip the latency and broadband legs that follow the headline")
     ap.add_argument("--cpu-worker", help=argparse.SUPPRESS)

@@ -358,9 +358,9 @@ struct EssPencilT {
     // phase 3: sensitivities at the converged eigenvalue (every lane, one sweep)
     __device__ __forceinline__ void finish(const ModeProb &P, double *s_ring, double sig, bool collapsed, int nclu,
                                            int j, bool active, int my_iters, const ModesOut &out) const {
-        const FinalOut f = sturm_final(P.a, P.w, s_ring, n_eff, sig, P.srow, P.rrow);
+        const FinalOut f = sturm_final<SMEM>(P.a, P.w, s_ring, n_eff, sig, P.srow, P.rrow);
         SVal th_s, th_r;
-        theta_forward(P.a, s_ring, sig, P.srow, P.rrow, th_s, th_r);
+        theta_forward<SMEM>(P.a, s_ring, sig, P.srow, P.rrow, th_s, th_r);
         if (!active) return;   // (whole-wave work above is done; nothing of this lane is left)
         const double mult = collapsed ? (double)nclu : 1.0;
         const double us2 = adjugate_product(th_s, f.phi_s1, f.q0, f.ex, 0) / mult;
@@ -396,7 +396,8 @@ using EssPencil = EssPencilT<false>;
 template <int MS, bool SMEM = false>
 __global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
                                               int n_work, const int *__restrict__ n_work_dev, ModesOut out) {
-    __shared__ ModesShared sh;
+    // (the scalar-load sweeps stage nothing in LDS: ~2 KB per block instead of ~10 KB)
+    __shared__ ModesSharedN<SMEM ? 1 : 2 * NCPA_RING> sh;
     const int nw = n_work_dev ? *n_work_dev : n_work;
     if (blockIdx.x >= (unsigned)nw) return;
     const ModeWork wk = work[blockIdx.x];

@@ -219,15 +219,23 @@ __device__ __forceinline__ void sweep_down2(const double *a, const double *w, do
             top = n - rem;
         }
         for (int cb = top - NCPA_CHUNK; cb >= base; cb -= NCPA_CHUNK) {
-            double va[NCPA_CHUNK], vw[NCPA_CHUNK];
-            lds_chunk(va, s + (cb - base));
-            lds_chunk(vw, t + (cb - base));
-            if (rec(cb)) {
+            const bool r = rec(cb);
+            // each array read 8 rows at a time (32 VGPRs for both, not 64)
 #pragma unroll
-                for (int u = NCPA_CHUNK - 1; u >= 0; --u) step_rec(va[u], vw[u], cb + u);
-            } else {
+            for (int h = NCPA_CHUNK / 8 - 1; h >= 0; --h) {
+                double va[8], vw[8];
 #pragma unroll
-                for (int u = NCPA_CHUNK - 1; u >= 0; --u) step(va[u], vw[u], cb + u);
+                for (int u = 0; u < 8; ++u) {
+                    va[u] = s[cb - base + 8 * h + u];
+                    vw[u] = t[cb - base + 8 * h + u];
+                }
+                if (r) {
+#pragma unroll
+                    for (int u = 7; u >= 0; --u) step_rec(va[u], vw[u], cb + 8 * h + u);
+                } else {
+#pragma unroll
+                    for (int u = 7; u >= 0; --u) step(va[u], vw[u], cb + 8 * h + u);
+                }
             }
             chunk_end(cb, NCPA_CHUNK);
         }
@@ -322,6 +330,90 @@ __device__ __forceinline__ void sweep_down_smem(const double *a_, int n, Step st
     }
 }
 
+// rows 0 .. m-1 ascending as sweep_up (16-row chunks, then the m % 16 last rows), the
+// diagonal by scalar loads in the order of sweep_down_smem
+template <class Step, class ChunkEnd>
+__device__ __forceinline__ void sweep_up_smem(const double *a_, int m, Step step, ChunkEnd chunk_end) {
+    if (m <= 0) return;
+    const cdptr a = (cdptr)a_;
+    const int nfull = m / NCPA_CHUNK;
+    if (nfull > 0) {
+        int c = 0;
+        double cur[NCPA_CHUNK];
+        smem_chunk(cur, a);
+        for (;;) {
+            wait_lds();
+            __builtin_amdgcn_sched_barrier(0);
+            double nxt[NCPA_CHUNK];
+            smem_chunk(nxt, a + (c + NCPA_CHUNK < nfull * NCPA_CHUNK ? c + NCPA_CHUNK : c));
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < NCPA_CHUNK; ++u) step(cur[u], c + u);
+            chunk_end(c, NCPA_CHUNK);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < NCPA_CHUNK; ++u) cur[u] = nxt[u];
+            c += NCPA_CHUNK;
+            if (c >= nfull * NCPA_CHUNK) break;
+        }
+    }
+    const int endc = nfull * NCPA_CHUNK;
+    for (int i = endc; i < m; ++i) step(a[i], i);
+    if (m > endc) chunk_end(endc, m - endc);
+}
+
+// two arrays (diagonal and absorption weights) by scalar loads, as sweep_down2: the n % 16
+// top rows with step_rec, then 16-row chunks downwards, each read as two 8-row halves (the
+// next half's loads in flight while the current one is stepped), step_rec where rec(lo)
+template <class Step, class StepRec, class RecPred, class ChunkEnd>
+__device__ __forceinline__ void sweep_down2_smem(const double *a_, const double *w_, int n, Step step,
+                                                 StepRec step_rec, RecPred rec, ChunkEnd chunk_end) {
+    if (n <= 0) return;
+    const cdptr a = (cdptr)a_, w = (cdptr)w_;
+    const int rem = n % NCPA_CHUNK;
+    if (rem) {
+        for (int i = n - 1; i >= n - rem; --i) step_rec(a[i], w[i], i);
+        chunk_end(n - rem, rem);
+    }
+    const int c = n - rem - NCPA_CHUNK;   // lowest full chunk start is 0; chunks are 16-aligned
+    if (c < 0) return;
+    int h = c + 8;                        // first row of the current 8-row half
+    double ca[8], cw[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        ca[u] = a[h + u];
+        cw[u] = w[h + u];
+    }
+    for (;;) {
+        wait_lds();   // lgkmcnt(0): the current half has arrived
+        __builtin_amdgcn_sched_barrier(0);
+        const int hn = h >= 8 ? h - 8 : h;
+        double na[8], nw[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            na[u] = a[hn + u];
+            nw[u] = w[hn + u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (rec(h & ~(NCPA_CHUNK - 1))) {
+#pragma unroll
+            for (int u = 7; u >= 0; --u) step_rec(ca[u], cw[u], h + u);
+        } else {
+#pragma unroll
+            for (int u = 7; u >= 0; --u) step(ca[u], cw[u], h + u);
+        }
+        if ((h & (NCPA_CHUNK - 1)) == 0) chunk_end(h, NCPA_CHUNK);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            ca[u] = na[u];
+            cw[u] = nw[u];
+        }
+        if (h < 8) break;
+        h -= 8;
+    }
+}
+
 // rows n-1 .. 0 through the LDS ring (SMEM false) or scalar loads (SMEM true)
 template <bool SMEM, class Step, class ChunkEnd>
 __device__ __forceinline__ void sweep_down_via(const double *a, double *ring, int n, Step step, ChunkEnd chunk_end) {
@@ -407,6 +499,7 @@ struct FinalOut {
 };
 
 // backward sweep with absorption derivative and the records phi_{s+1}, phi_{r+1}
+template <bool SMEM = false>
 __device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w, double *ring2, int n, double s,
                                                 int srow, int rrow) {
     double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0, e1 = 0.0, e2 = 0.0;
@@ -426,26 +519,28 @@ __device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w
         e2 = e1;
         e1 = e0;
     };
-    sweep_down2(
-        a, w, ring2, n, step,
-        [&](double ai, double wi, int i) {
-            step(ai, wi, i);
-            if (i == srow + 1) o.phi_s1 = SVal{p1, ex};
-            if (i == rrow + 1) o.phi_r1 = SVal{p1, ex};
-        },
-        [&](int lo) {
-            return (unsigned)(srow + 1 - lo) < (unsigned)NCPA_CHUNK || (unsigned)(rrow + 1 - lo) < (unsigned)NCPA_CHUNK;
-        },
-        [&](int, int) {
-            const int k = pow2_exponent(p1);
-            p1 = ldexp(p1, -k);
-            p2 = ldexp(p2, -k);
-            q1 = ldexp(q1, -k);
-            q2 = ldexp(q2, -k);
-            e1 = ldexp(e1, -k);
-            e2 = ldexp(e2, -k);
-            ex += k;
-        });
+    auto step_rec = [&](double ai, double wi, int i) {
+        step(ai, wi, i);
+        if (i == srow + 1) o.phi_s1 = SVal{p1, ex};
+        if (i == rrow + 1) o.phi_r1 = SVal{p1, ex};
+    };
+    auto rec = [&](int lo) {
+        return (unsigned)(srow + 1 - lo) < (unsigned)NCPA_CHUNK || (unsigned)(rrow + 1 - lo) < (unsigned)NCPA_CHUNK;
+    };
+    auto chunk_end = [&](int, int) {
+        const int k = pow2_exponent(p1);
+        p1 = ldexp(p1, -k);
+        p2 = ldexp(p2, -k);
+        q1 = ldexp(q1, -k);
+        q2 = ldexp(q2, -k);
+        e1 = ldexp(e1, -k);
+        e2 = ldexp(e2, -k);
+        ex += k;
+    };
+    if constexpr (SMEM)
+        sweep_down2_smem(a, w, n, step, step_rec, rec, chunk_end);
+    else
+        sweep_down2(a, w, ring2, n, step, step_rec, rec, chunk_end);
     o.p0 = p1;
     o.q0 = q1;
     o.e0 = e1;
@@ -510,6 +605,7 @@ __device__ __forceinline__ int evanescent_rows(const double *a, int n, double s_
 }
 
 // forward continuants theta_{s-1}, theta_{r-1} (theta_{-1} = 1); rows 0 .. max(s,r)-1
+template <bool SMEM = false>
 __device__ __forceinline__ void theta_forward(const double *a, double *ring, double s, int srow, int rrow, SVal &th_s,
                                               SVal &th_r) {
     th_s = SVal{1.0, 0};
@@ -517,21 +613,23 @@ __device__ __forceinline__ void theta_forward(const double *a, double *ring, dou
     const int top = srow > rrow ? srow : rrow;
     double t1 = 1.0, t2 = 0.0;
     int ex = 0;
-    sweep_up(
-        a, ring, top,
-        [&](double ak, int k) {
-            const double t0 = fma(ak - s, t1, -t2);
-            t2 = t1;
-            t1 = t0;
-            if (k == srow - 1) th_s = SVal{t1, ex};
-            if (k == rrow - 1) th_r = SVal{t1, ex};
-        },
-        [&](int, int) {
-            const int kk = pow2_exponent(t1);
-            t1 = ldexp(t1, -kk);
-            t2 = ldexp(t2, -kk);
-            ex += kk;
-        });
+    auto step = [&](double ak, int k) {
+        const double t0 = fma(ak - s, t1, -t2);
+        t2 = t1;
+        t1 = t0;
+        if (k == srow - 1) th_s = SVal{t1, ex};
+        if (k == rrow - 1) th_r = SVal{t1, ex};
+    };
+    auto chunk_end = [&](int, int) {
+        const int kk = pow2_exponent(t1);
+        t1 = ldexp(t1, -kk);
+        t2 = ldexp(t2, -kk);
+        ex += kk;
+    };
+    if constexpr (SMEM)
+        sweep_up_smem(a, top, step, chunk_end);
+    else
+        sweep_up(a, ring, top, step, chunk_end);
 }
 
 // u_lo u_hi = -(-1)^(hi-lo) theta_{lo-1} phi_{hi+1} / q_0

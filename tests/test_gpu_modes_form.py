@@ -2,7 +2,8 @@
 same modes to the bit: the per-row arithmetic is the same, only the source of a_i differs.
 
 Grids of 20000 rows (whole 16-row chunks), 7001 (9 top rows before the first chunk) and 100
-(4 top rows, then six chunks), frequencies from a few modes to ~500 per problem, several problems per
+(4 top rows, then six chunks), sources and receivers on the ground and aloft (the final sweep
+records rows inside a chunk, the forward sweep climbs to them), frequencies from a few modes to ~500 per problem, several problems per
 launch; the scalar form is also checked against the oracle at 1 Hz.
 """
 import numpy as np
@@ -27,13 +28,17 @@ def solve_both(ctx, g, probs):
     return out
 
 
-@pytest.mark.parametrize("Nz,freqs", [(20000, (0.05, 0.4, 1.0, 1.7)), (7001, (0.1, 0.5)), (100, (0.3, 1.0))])
-def test_forms_bit_equal(gpu_ctx, tmp_path, Nz, freqs):
-    print("case modes form Nz=%d freqs=%s" % (Nz, freqs), flush=True)
+@pytest.mark.parametrize("Nz,freqs,zs_m,zr_m", [(20000, (0.05, 0.4, 1.0, 1.7), 0.0, 0.0),
+                                                (20000, (0.3, 1.2), 10000.0, 1000.0),   # rows recorded aloft
+                                                (7001, (0.1, 0.5), 2000.0, 0.0),
+                                                (100, (0.3, 1.0), 0.0, 0.0)])
+def test_forms_bit_equal(gpu_ctx, tmp_path, Nz, freqs, zs_m, zr_m):
+    print("case modes form Nz=%d freqs=%s zs=%g zr=%g" % (Nz, freqs, zs_m, zr_m), flush=True)
     paths = [synth.write_profile(str(tmp_path / ("P%d.dat" % s)), seed=s) for s in (0, 2)]
     pids = [gpu_ctx.read_profile(p)[0] for p in paths]
     g = gpu_ctx.grid(pids[0], Nz=Nz)
-    probs = [ncpamodes.problem(pid, f, az) for pid in pids for f in freqs for az in (30.0, 200.0)]
+    probs = [ncpamodes.problem(pid, f, az, zs_m=zs_m, zr_m=zr_m) for pid in pids for f in freqs
+             for az in (30.0, 200.0)]
     a, b = solve_both(gpu_ctx, g, probs)
     assert np.array_equal(a[0], b[0])
     assert a[1].size > 0
