@@ -336,6 +336,17 @@ struct EssPencilT {
         o.cnt += p.n - n_eff;
         return o;
     }
+    // det and d det/ds of the whole matrix without the count (the rows left out contribute
+    // one negative pivot each, which flips det's sign n - n_eff times)
+    static constexpr bool sign_counts = true;
+    __device__ __forceinline__ NewtonOut newton_nocount(double *ring, double s) const {
+        NewtonOut o = sturm_newton_nocount<SMEM>(p.a, ring, n_eff, s);
+        if ((p.n - n_eff) & 1) {
+            o.p0 = -o.p0;
+            o.q0 = -o.q0;
+        }
+        return o;
+    }
     // (the final sweep records phi at rows srow + 1 and rrow + 1: those stay inside)
     __device__ __forceinline__ void narrow(double s_min) {
         n_eff = evanescent_rows(p.a, p.n, s_min - 16.0 * p.tol, 80.0, max(p.srow, p.rrow) + 2);
@@ -393,8 +404,10 @@ using EssPencil = EssPencilT<false>;
 //   k_modes_persistent  a fixed grid whose blocks pull items 8t + x, t = queue[x]++, until
 //                       the device count is exhausted.  (Its loop costs registers: ~200
 //                       VGPRs against ~120 for k_modes; the plan's autotuner times both.)
+// (the scalar-load form asks for 6 waves per SIMD: 77 VGPRs, SGPRs spilled to VGPR lanes
+// outside the sweep loops; 5 or 7 waves measure the same, 8 would spill to scratch)
 template <int MS, bool SMEM = false>
-__global__ void __launch_bounds__(64) k_modes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
+__global__ void __launch_bounds__(64, SMEM ? 6 : 1) k_modes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
                                               int n_work, const int *__restrict__ n_work_dev, ModesOut out) {
     // (the scalar-load sweeps stage nothing in LDS: ~2 KB per block instead of ~10 KB)
     __shared__ ModesSharedN<SMEM ? 1 : 2 * NCPA_RING> sh;

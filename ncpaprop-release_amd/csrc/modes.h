@@ -155,6 +155,7 @@ struct ModesSharedN {
     double ms_s[64];
     int ms_c[64];
     double hb_lo[64], hb_hi[64];   // brackets of the unconverged eigenvalues
+    int hb_cl[64];                 // the count at each bracket's lower end
     int h_list[64];                // their lanes, in lane order
     double ring[RING];             // the staged arrays (RING doubles)
 };
@@ -173,7 +174,7 @@ template <int MS, class Pen, class Sh>
 __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const ModeWork wk, Sh &sh_,
                                            const ModesOut &out) {
     double *ms_s = sh_.ms_s, *hb_lo = sh_.hb_lo, *hb_hi = sh_.hb_hi, *s_ring = sh_.ring;
-    int *ms_c = sh_.ms_c, *h_list = sh_.h_list;
+    int *ms_c = sh_.ms_c, *h_list = sh_.h_list, *hb_cl = sh_.hb_cl;
     const int lane = (int)threadIdx.x;
     const int j = wk.j0 + lane;
     const bool active = j < P.n_hi;
@@ -227,10 +228,12 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
         if (!done) {
             hb_lo[lane] = lo;
             hb_hi[lane] = hi;
+            hb_cl[lane] = cl;
             h_list[__popcll(um & below)] = lane;
         }
         __syncthreads();
         double sh = sig;
+        int sh_cl = cl;   // count at the lower end of the bracket that holds sh
         if (done) {
             const int h = __popcll(~um & below);        // rank among the helpers
             const int idx = h % nu, kk = h / nu;
@@ -238,11 +241,20 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
             const int u = h_list[idx];
             const double a = hb_lo[u], b = hb_hi[u];
             sh = a + (double)(kk + 1) * ((b - a) / (double)(nh + 1));
+            sh_cl = hb_cl[u];
         }
         const bool newton = !done && (ch - cl == 1);
         int cnt;
         double p0 = 0.0, q0 = 0.0;
-        if (__any(newton)) {
+        if (Pen::sign_counts && __all(done || ch - cl == 1)) {
+            // every shift of the wave lies strictly inside a bracket that holds one eigenvalue
+            // (its own, or the one it helps with): the count there is cl or cl + 1, told apart
+            // by the sign of det, and the sweep needs no sign history
+            const NewtonOut o = pen.newton_nocount(s_ring, sh);
+            cnt = count_from_sign(o.p0, sh_cl);
+            p0 = o.p0;
+            q0 = o.q0;
+        } else if (__any(newton)) {
             const NewtonOut o = pen.newton(s_ring, sh);
             cnt = o.cnt;
             p0 = o.p0;

@@ -486,6 +486,45 @@ __device__ __forceinline__ NewtonOut sturm_newton(const double *a, double *ring,
     return o;
 }
 
+// det and d det/ds without the count: for the phase in which every shift of the wave lies in
+// a bracket holding a single eigenvalue, where the count follows from the sign of det alone
+// (the sign changes from phi_n = +1 down to phi_0 are even exactly when phi_0's sign bit is
+// clear, so count(s) is cl or cl + 1 by that bit): one instruction less per row.
+template <bool SMEM = false>
+__device__ __forceinline__ NewtonOut sturm_newton_nocount(const double *a, double *ring, int n, double s) {
+    double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
+    sweep_down_via<SMEM>(
+        a, ring, n,
+        [&](double ai, int) {
+            const double am = ai - s;
+            const double p0 = fma(am, p1, -p2);
+            const double q0 = fma(am, q1, -(q2 + p1));
+            p2 = p1;
+            p1 = p0;
+            q2 = q1;
+            q1 = q0;
+        },
+        [&](int, int) {
+            const int k = pow2_exponent(p1);
+            p1 = ldexp(p1, -k);
+            p2 = ldexp(p2, -k);
+            q1 = ldexp(q1, -k);
+            q2 = ldexp(q2, -k);
+        });
+    NewtonOut o;
+    o.cnt = -1;
+    o.p0 = p1;
+    o.q0 = q1;
+    return o;
+}
+
+// count of a shift inside a bracket whose lower end has count cl and that holds one
+// eigenvalue, from the sign bit of det(T' - s) there
+__device__ __forceinline__ int count_from_sign(double det, int cl) {
+    const int odd = (int)((unsigned)__double2hiint(det) >> 31);
+    return cl + ((cl ^ odd) & 1);
+}
+
 // scaled value: v * 2^e
 struct SVal {
     double v;

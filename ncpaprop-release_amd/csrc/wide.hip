@@ -103,6 +103,9 @@ struct WidePencil {
     __device__ __forceinline__ NewtonOut newton(double *ring, double k) const { return s.newton(ring, k); }
     // (every row is swept: the wide-angle sweeps leave none out)
     __device__ __forceinline__ void narrow(double) {}
+    // (counts always come from the sweep: no sign-only Newton phase)
+    static constexpr bool sign_counts = false;
+    __device__ __forceinline__ NewtonOut newton_nocount(double *ring, double k) const { return s.newton(ring, k); }
     __device__ __forceinline__ int rows_kept(double) const { return s.n; }
 
     // phase 3 at the converged k: psi = u / sqrt(dz (1 + k^2)), absorption u^T W u / (1 + k^2)
