@@ -395,9 +395,9 @@ def run_field(lanes, args, prec):
     for ctx, _, _ in lanes:
         ctx.set_timing(False)
     last = lane_state[0]["last"]
-    iters = np.abs(last.iterations()).astype(np.float64)
     res = {"elapsed": elapsed, "fams": fams, "total_modes": int(last.total),
-           "sturm_steps": float((iters + 1.0).sum()) * NZ,
+           # rows each mode's search swept (sweeps it used x rows below its block's evanescent cut)
+           "sturm_steps": float(last.sweep_rows().astype(np.float64).sum()),
            "field_flops": 4.0 * FIELD_ROWS * last.total * NR, "streams": L,
            "field_mfma_flops": issued_mfma_flops(prec, last.counts,
                                                  lanes[0][0].field_variant_info(prec, lanes[0][0].field_variant(prec))[0])}
@@ -676,6 +676,8 @@ def broadband_main(args, cpu):
     tmp = tempfile.mkdtemp(prefix="ncpa_bench_")
     path = write_profiles(tmp, [0])[0]
     ctx = ncpamodes.Context(0)
+    if args.copy_calibration:
+        copy_calibration(ctx, args.copy_calibration)
     pid = ctx.read_profile(path)[0]
     grid = ctx.grid(pid, Nz=NZ)
     f = np.arange(BB_NF + 1) * BB_FSTEP
@@ -698,7 +700,7 @@ def broadband_main(args, cpu):
     W, kb = bb.synthesize_from_modes(ctx, m, nset, f, ranges, np.ones(nset), S, BB_MAXCEL, nfft)
     ctx.synchronize()
     total_modes = m.total
-    sturm = float((np.abs(m.iterations()).astype(np.float64) + 1.0).sum()) * NZ
+    sturm = float(m.sweep_rows().astype(np.float64).sum())   # rows swept per mode, summed
     fams = {n: ctx.kernel_time(n) for n in ("setup", "interval", "modes", "bb_transfer", "fft")}
     ctx.set_timing(False)
     for b in (W,) + kb:
@@ -716,7 +718,9 @@ def broadband_main(args, cpu):
     ctx.close()
     assert plan_modes == total_modes, (plan_modes, total_modes)
     P = len(problems)
-    modes_s = tuning["modes_ms"] / 1e3    # the tuned k_modes launch, HIP events on the library stream
+    # the tuned k_modes kernel (the plan set the context's form and depth) in the eager solve of
+    # the same batch, HIP events on the library stream around its one launch
+    modes_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
     achieved = FLOPS_PER_STURM_STEP * sturm / modes_s / 1e12
     value = P * args.steps / elapsed
     line = {
@@ -737,15 +741,20 @@ def broadband_main(args, cpu):
         "plan": {"create_s": plan_s, "ms_passes": tuning["ms_passes"], "grid_blocks": tuning["grid_blocks"],
                  "k_modes_ms": tuning["modes_ms"],
                  "tuner_ms": {"%d/%d" % (int(a), int(b)): round(c, 3) for a, b, c in tuning["log"]}},
-        "roofline": {"kernel": "k_modes<%d> (%s)" % (tuning["ms_passes"], "one block per work-list slot"
-                                                     if tuning["grid_blocks"] == 0 else
-                                                     "persistent, %d blocks" % tuning["grid_blocks"]),
+        "roofline": {"kernel": "k_modes<%d%s> (%s)" % (tuning["ms_passes"], ", true" if tuning["grid_blocks"] < 0 else "",
+                                                       "one block per work-list slot, diagonal through LDS"
+                                                       if tuning["grid_blocks"] == 0 else
+                                                       "one block per work-list slot, diagonal by scalar loads"
+                                                       if tuning["grid_blocks"] < 0 else
+                                                       "persistent, %d blocks" % tuning["grid_blocks"]),
                      "bound": "valu",
                      "achieved": achieved, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS["f64"],
-                     "traffic": measured_traffic("k_modes<2> (8192-problem batch)"),
-                     "algorithmic": "3 FP64 flops per Sturm step (row x sweep); FP64 VALU (peak: the FP64 "
-                                    "vector rate, equal to the FP64 MFMA rate on MI355X)"},
+                     "traffic": measured_traffic("k_modes<%d%s> (8192-problem batch)"
+                                                 % (tuning["ms_passes"], ", true" if tuning["grid_blocks"] < 0 else "")),
+                     "algorithmic": "3 FP64 flops per Sturm step (row x sweep), counting the rows each "
+                                    "mode's sweeps covered (the evanescent top left out, ncpa_modes_sweep_rows); "
+                                    "FP64 VALU (peak: the FP64 vector rate, equal to the FP64 MFMA rate on MI355X)"},
         "cpu_baseline": None,
     }
     if cpu is not None:

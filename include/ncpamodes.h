@@ -61,10 +61,11 @@ int ncpa_ctx_kernel_time(ncpa_ctx_t *ctx, const char *kernel, double *total_ms, 
 int ncpa_ctx_reset_timing(ncpa_ctx_t *ctx);
 
 /* How the eigenvalue sweeps of this context's eager solves (ncpa_ess_solve) read the matrix
- * diagonal: 0 (default) through an LDS ring, fastest for batches of a few waves per GPU;
- * 1 through scalar loads, for batches that put several waves on every SIMD.  Both give the
- * same results to the bit.  (Broadband plans pick their own form in the autotuner.)  No
- * counterpart in the reference, which has one CPU path. */
+ * diagonal: 0 (default) through an LDS ring, fastest beside a running 2-D field contraction;
+ * 1 by scalar loads (no LDS, 6 waves per SIMD), fastest for batches that fill the GPU.  Both
+ * give the same results to the bit.  A broadband plan created with tuning sets the form (and
+ * the multisection depth) its autotuner picked.  No counterpart in the reference, which has
+ * one CPU path. */
 int ncpa_ctx_set_modes_form(ncpa_ctx_t *ctx, int form);
 
 /* Device memory of the context (no counterpart in the reference): live = bytes its
@@ -208,6 +209,10 @@ int ncpa_modes_download(ncpa_modes_t *modes, double *k_re, double *k_im, double 
 /* Sweeps each eigenvalue needed (host int[total]); negative when its bracket collapsed
  * to working precision before Newton converged. */
 int ncpa_modes_iterations(ncpa_modes_t *modes, int *iters);
+/* Rows of the tridiagonal each mode's eigenvalue search swept in total (host int[total]): the
+ * sweeps it used, each over the rows below the evanescent top its block leaves out (the
+ * work behind the eigenvalue kernel's throughput figures).  No counterpart in the reference. */
+int ncpa_modes_sweep_rows(ncpa_modes_t *modes, int *rows);
 /* Device views (valid until ncpa_modes_free): offsets[n_problems+1] (int), and per mode
  * k^2, absorption A (k_pert = sqrt(k^2 + i A)) and psi(zs) psi(zr) (double). */
 int ncpa_modes_device(ncpa_modes_t *modes, const int **dev_offsets, const double **dev_k2,

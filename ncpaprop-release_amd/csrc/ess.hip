@@ -348,6 +348,7 @@ struct EssPencilT {
         return o;
     }
     // (the final sweep records phi at rows srow + 1 and rrow + 1: those stay inside)
+    __device__ __forceinline__ int rows() const { return n_eff; }
     __device__ __forceinline__ void narrow(double s_min) {
         n_eff = evanescent_rows(p.a, p.n, s_min - 16.0 * p.tol, 80.0, max(p.srow, p.rrow) + 2);
     }
@@ -368,7 +369,8 @@ struct EssPencilT {
     }
     // phase 3: sensitivities at the converged eigenvalue (every lane, one sweep)
     __device__ __forceinline__ void finish(const ModeProb &P, double *s_ring, double sig, bool collapsed, int nclu,
-                                           int j, bool active, int my_iters, const ModesOut &out) const {
+                                           int j, bool active, int my_iters, int rows_used,
+                                           const ModesOut &out) const {
         const FinalOut f = sturm_final<SMEM>(P.a, P.w, s_ring, n_eff, sig, P.srow, P.rrow);
         SVal th_s, th_r;
         theta_forward<SMEM>(P.a, s_ring, sig, P.srow, P.rrow, th_s, th_r);
@@ -390,6 +392,7 @@ struct EssPencilT {
         out.psir[o] = (usr < 0.0) ? -pr : pr;
         out.psisr[o] = usr * P.inv_dz;
         out.iters[o] = my_iters;
+        if (out.rows) out.rows[o] = rows_used;
     }
 };
 
@@ -900,7 +903,7 @@ int check_grid(ncpa_ctx *ctx, const ncpa_grid_t *g) {
 void release_modes(ncpa_modes *m) {
     if (!m || !m->ctx) return;
     DevicePool &pl = m->ctx->pool;
-    void *bufs[] = {m->d_offsets, m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters,
+    void *bufs[] = {m->d_offsets, m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters, m->d_rows,
                     m->d_A, m->d_W, m->d_CE, m->d_mp, m->d_work, m->d_mode_prob, m->d_psis_rows, m->d_BC,
                     m->d_PQ, m->d_beta};
     for (void *b : bufs) pl.put(b);
@@ -984,11 +987,12 @@ int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, con
     m->d_psir = pool_alloc<double>(ctx, tot_alloc);
     m->d_psisr = pool_alloc<double>(ctx, tot_alloc);
     m->d_iters = pool_alloc<int>(ctx, tot_alloc);
+    m->d_rows = pool_alloc<int>(ctx, tot_alloc);
     m->d_offsets = pool_alloc<int>(ctx, P + 1);
     m->d_mp = pool_alloc<ModeProb>(ctx, P);
     m->d_work = pool_alloc<ModeWork>(ctx, std::max<size_t>(work.size(), 1));
     m->d_mode_prob = pool_alloc<int>(ctx, tot_alloc);
-    if (!m->d_k2 || !m->d_abs || !m->d_psis || !m->d_psir || !m->d_psisr || !m->d_iters || !m->d_offsets ||
+    if (!m->d_k2 || !m->d_abs || !m->d_psis || !m->d_psir || !m->d_psisr || !m->d_iters || !m->d_rows || !m->d_offsets ||
         !m->d_mp || !m->d_work || !m->d_mode_prob)
         return set_error(ctx, NCPA_ERR_ALLOC, "mode buffers: device allocation failed");
     NCPA_HIP(ctx, hipMemcpyAsync(m->d_offsets, m->offsets.data(), sizeof(int) * (P + 1), hipMemcpyHostToDevice, ctx->stream));
@@ -997,7 +1001,7 @@ int launch_modes(ncpa_ctx *ctx, ncpa_modes *m, const std::vector<int> &srow, con
         NCPA_HIP(ctx, hipMemcpyAsync(m->d_work, work.data(), sizeof(ModeWork) * work.size(), hipMemcpyHostToDevice, ctx->stream));
         NCPA_HIP(ctx, hipMemcpyAsync(m->d_mode_prob, mode_prob.data(), sizeof(int) * total, hipMemcpyHostToDevice, ctx->stream));
         TimedLaunch tl(ctx, "modes");
-        const ModesOut mo{m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters};
+        const ModesOut mo{m->d_k2, m->d_abs, m->d_psis, m->d_psir, m->d_psisr, m->d_iters, m->d_rows};
         if (m->wide)
             launch_k_wmodes(ctx->ms_passes, dim3((unsigned)work.size()), ctx->stream, m->d_mp, m->d_work,
                             m->d_BC, (size_t)P * Nz, m->d_PQ, m->d_beta, mo);
@@ -1245,6 +1249,14 @@ int ncpa_modes_download(ncpa_modes_t *m, double *k_re, double *k_im, double *psi
         if (psi_s) psi_s[i] = ps[i];
         if (psi_r) psi_r[i] = pr[i];
     }
+    return NCPA_OK;
+}
+
+int ncpa_modes_sweep_rows(ncpa_modes_t *m, int *rows) {
+    if (!m || !rows) return NCPA_ERR_ARG;
+    if (m->total == 0) return NCPA_OK;
+    NCPA_HIP(m->ctx, hipMemcpyAsync(rows, m->d_rows, m->total * sizeof(int), hipMemcpyDeviceToHost, m->ctx->stream));
+    NCPA_HIP(m->ctx, hipStreamSynchronize(m->ctx->stream));
     return NCPA_OK;
 }
 

@@ -59,6 +59,7 @@ struct ModeWork {
 struct ModesOut {
     double *k2, *absn, *psis, *psir, *psisr;
     int *iters;
+    int *rows;   // rows the sweeps of each mode's search covered (the evanescent top left out)
 };
 
 // Written on the device by k_plan_build for the persistent eigenvalue launch.
@@ -129,6 +130,7 @@ struct ncpa_modes {
     int *d_offsets = nullptr;
     double *d_k2 = nullptr, *d_abs = nullptr, *d_psis = nullptr, *d_psir = nullptr, *d_psisr = nullptr;
     int *d_iters = nullptr;
+    int *d_rows = nullptr;
     double *d_A = nullptr, *d_W = nullptr, *d_CE = nullptr;   // per-problem grid samples
     double *d_BC = nullptr;                    // wide-angle B and C rows
     double *d_PQ = nullptr, *d_beta = nullptr; // wide-angle factored rows P, Q and ground terms

@@ -148,7 +148,8 @@ __device__ inline void interval_bounds(const ProbDev &pr, int Nz, double dz, con
 // The search is written once for any pencil whose count of eigenvalues below the shift
 // is non-decreasing in the shift: `pen` supplies count(ring, s), newton(ring, s) (count,
 // det and d det/ds), narrow(s_min) (told after each multisection pass that no later shift
-// lies below s_min) and finish(...) (phase 3 and the outputs).  EssPencil is dz^2 T - s;
+// lies below s_min), rows() (rows a sweep covers now) and finish(...) (phase 3 and the
+// outputs).  EssPencil is dz^2 T - s;
 // the wide-angle pencil (wide.h) is dz^2 T(k) with the shift k inside the diagonal.
 template <int RING>
 struct ModesSharedN {
@@ -186,9 +187,11 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
     // work that bisection spends ~log2(n) sweeps on per lane.
     const int last = min(P.n_hi - 1, wk.j0 + 63) - wk.j0;   // last active lane
     double blo = lo, bhi = hi;
+    int rows_used = 0;   // rows of the sweeps this lane's search used (until it converged)
     for (int pass = 0; pass < MS; ++pass) {
         const double sh = blo + (double)(lane + 1) * ((bhi - blo) * (1.0 / 65.0));
         const int c = pen.count(s_ring, sh);
+        rows_used += pen.rows();
         ms_s[lane] = sh;
         ms_c[lane] = c;
         __syncthreads();
@@ -262,6 +265,7 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
         } else {
             cnt = pen.count(s_ring, sh);
         }
+        if (!done) rows_used += pen.rows();
         ms_s[lane] = sh;
         ms_c[lane] = cnt;
         __syncthreads();
@@ -314,7 +318,7 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
         __syncthreads();
     }
     if (!done) my_iters = -NCPA_MAXIT - 1;
-    pen.finish(P, s_ring, sig, collapsed, ch - cl, j, active, my_iters, out);
+    pen.finish(P, s_ring, sig, collapsed, ch - cl, j, active, my_iters, rows_used + pen.rows(), out);
 }
 
 // ------------------------------------------------------------------------------------

@@ -78,7 +78,7 @@ void release(ncpa_bb_plan *pl) {
 }
 
 ModesOut modes_out(const ncpa_bb_plan *pl) {
-    return ModesOut{pl->k2, pl->absn, pl->psis, pl->psir, pl->psisr, pl->iters};
+    return ModesOut{pl->k2, pl->absn, pl->psis, pl->psir, pl->psisr, pl->iters, nullptr};
 }
 
 // grids, intervals and the device work list
@@ -282,6 +282,7 @@ int ncpa_bb_plan_create(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_pro
     if (job->tune) {
         if ((rc = autotune(pl, std::max(n_cu, 1)))) return done(rc);
         ctx->ms_passes = pl->ms;   // the eager solves of this context follow the tuned depth
+        if (pl->blocks <= 0) ctx->modes_smem = pl->blocks < 0;   // and sweep form
     }
     // capture
     if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)

@@ -103,6 +103,7 @@ struct WidePencil {
     __device__ __forceinline__ NewtonOut newton(double *ring, double k) const { return s.newton(ring, k); }
     // (every row is swept: the wide-angle sweeps leave none out)
     __device__ __forceinline__ void narrow(double) {}
+    __device__ __forceinline__ int rows() const { return s.n; }
     // (counts always come from the sweep: no sign-only Newton phase)
     static constexpr bool sign_counts = false;
     __device__ __forceinline__ NewtonOut newton_nocount(double *ring, double k) const { return s.newton(ring, k); }
@@ -110,7 +111,8 @@ struct WidePencil {
 
     // phase 3 at the converged k: psi = u / sqrt(dz (1 + k^2)), absorption u^T W u / (1 + k^2)
     __device__ __forceinline__ void finish(const ModeProb &P, double *ring, double k, bool collapsed, int nclu,
-                                           int j, bool active, int my_iters, const ModesOut &out) const {
+                                           int j, bool active, int my_iters, int rows_used,
+                                           const ModesOut &out) const {
         const FinalOut f = s.final_sweep(ring, k, P.srow, P.rrow);
         SVal th_s, th_r;
         s.theta(ring, k, P.srow, P.rrow, th_s, th_r);
@@ -133,6 +135,7 @@ struct WidePencil {
         out.psir[o] = (usr < 0.0) ? -pr : pr;
         out.psisr[o] = usr * P.inv_dz;
         out.iters[o] = my_iters;
+        if (out.rows) out.rows[o] = rows_used;
     }
 
     // rows pass

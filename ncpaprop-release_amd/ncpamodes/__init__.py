@@ -342,6 +342,12 @@ class Modes:
         self.ctx.check(self.ctx.lib.ncpa_modes_iterations(self.h, it.ctypes.data_as(C.POINTER(C.c_int))))
         return it[:self.total]
 
+    def sweep_rows(self):
+        """Rows each mode's eigenvalue search swept in total (ncpa_modes_sweep_rows)."""
+        r = np.zeros(max(self.total, 1), dtype=np.int32)
+        self.ctx.check(self.ctx.lib.ncpa_modes_sweep_rows(self.h, r.ctypes.data_as(C.POINTER(C.c_int))))
+        return r[:self.total]
+
     def ceff(self, p):
         """c_eff [m/s] on the grid rows of problem p (ncpa_modes_ceff)."""
         out = np.empty(self.grid.Nz)

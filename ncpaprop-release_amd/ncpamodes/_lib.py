@@ -43,7 +43,7 @@ EXPORTS = [
     "ncpa_profile_read", "ncpa_profile_check", "ncpa_profile_nodes", "ncpa_profile_admittance",
     "ncpa_grid_make", "ncpa_profile_sample",
     "ncpa_ess_solve", "ncpa_ess_count", "ncpa_modes_free", "ncpa_modes_counts", "ncpa_modes_interval",
-    "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d",
+    "ncpa_modes_download", "ncpa_modes_iterations", "ncpa_modes_sweep_rows", "ncpa_modes_device", "ncpa_modes_ceff", "ncpa_modes_rows", "ncpa_tl1d", "ncpa_field2d",
     "ncpa_field2d_stage", "ncpa_field2d_contract", "ncpa_field_stage_free",
     "ncpa_field_variants", "ncpa_field_variant_info", "ncpa_field_set_variant", "ncpa_field_get_variant", "ncpa_field_tune",
     "ncpa_field_compare", "ncpa_ess_eigen",
@@ -136,6 +136,7 @@ def load():
         "ncpa_modes_interval": (C.c_int, [vp, dp, dp]),
         "ncpa_modes_download": (C.c_int, [vp, dp, dp, dp, dp, dp]),
         "ncpa_modes_iterations": (C.c_int, [vp, ip]),
+        "ncpa_modes_sweep_rows": (C.c_int, [vp, ip]),
         "ncpa_modes_rows": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, vp]),
         "ncpa_modes_device": (C.c_int, [vp, P(vp), P(vp), P(vp), P(vp)]),
         "ncpa_modes_ceff": (C.c_int, [vp, C.c_int, dp]),

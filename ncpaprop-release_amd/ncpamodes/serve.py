@@ -76,7 +76,7 @@ def _solve_bytes(nms, Nz, max_nodes):
     groups = [(n + 63) // 64 for n in nms]
     # XCD-balanced work list: at most ceil(groups / 8) + the largest problem's groups per XCD
     work = 8 * ((sum(groups) + 7) // 8 + max(groups + [0]))
-    persistent = (3 * _r256(8 * P * Nz) + 5 * _r256(8 * tot) + 2 * _r256(4 * tot) + _r256(4 * (P + 1))
+    persistent = (3 * _r256(8 * P * Nz) + 5 * _r256(8 * tot) + 3 * _r256(4 * tot) + _r256(4 * (P + 1))
                   + _r256(SIZEOF["ModeProb"] * P) + _r256(SIZEOF["ModeWork"] * max(work, 1)))
     transient = (_r256(SIZEOF["ProbDev"] * P) + _r256(16 * P * max_nodes) + _r256(SIZEOF["IntervalOut"] * P))
     return persistent, transient
