@@ -350,7 +350,7 @@ struct EssPencilT {
     // (the final sweep records phi at rows srow + 1 and rrow + 1: those stay inside)
     __device__ __forceinline__ int rows() const { return n_eff; }
     __device__ __forceinline__ void narrow(double s_min) {
-        n_eff = evanescent_rows(p.a, p.n, s_min - 16.0 * p.tol, 80.0, max(p.srow, p.rrow) + 2);
+        n_eff = evanescent_rows(p.a, p.n, s_min, 80.0, max(p.srow, p.rrow) + 2);
     }
     // rows pass: shift of a mode with eigenvalue k2, row values a_i - s, q_0 unscaled; rows
     // swept for modes whose shifts are >= s_min (the source row stays inside)

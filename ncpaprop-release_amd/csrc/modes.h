@@ -208,10 +208,10 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
         __syncthreads();
         blo = __shfl(lo, 0);
         bhi = __shfl(hi, last);
-        // every later shift of the block lies above lane 0's lower bound (to within the
-        // tolerance of a Newton step accepted just outside a bracket): the pencil may drop
+        // every later shift of the block lies above lane 0's lower bound, to within the
+        // tolerance of a Newton step accepted just outside a bracket: the pencil may drop
         // what no such shift needs (narrow; after every pass, as the bound rises)
-        pen.narrow(blo);
+        pen.narrow(blo - 16.0 * P.tol);
     }
     double sig = next_shift(lo, hi, cl, ch, j);
     bool done = !active;

@@ -605,13 +605,19 @@ __device__ __forceinline__ FinalOut sturm_final(const double *a, const double *w
 //  * n_eff >= min_keep (rows the caller records), and n when no cut qualifies.
 // Whole-wave call (64 lanes), reads a' straight from global memory: ~(n - t0) / 64 loads per
 // lane, a fraction of one sweep.
-__device__ __forceinline__ int evanescent_rows(const double *a, int n, double s_min, double decay, int min_keep) {
+// Generic over the row values: am(i) returns row i's diagonal entry of the shifted matrix at
+// the lower bound (a'_i - s_min here; the wide-angle pencil's a_i(k_min)); every later shift
+// must leave rows with am(i) <= -2 at or below -2.
+// Rows with am(i) <= -2 - margin count as evanescent (margin 0 where am is one rounded
+// subtraction, monotone in the shift; a small margin where am is a longer expression).
+template <class RowAt>
+__device__ __forceinline__ int evanescent_rows_at(RowAt am, int n, double decay, int min_keep, double margin = 0.0) {
     const int lane = (int)(threadIdx.x & 63);
-    // t0: one above the highest row that is not evanescent at s_min
+    // t0: one above the highest row that is not evanescent at the bound
     int t0 = 0;
     for (int top = n - 1; top >= 0; top -= 64) {
         const int i = top - lane;
-        const bool osc = i >= 0 && !(a[i] - s_min <= -2.0);   // (NaN counts as oscillatory)
+        const bool osc = i >= 0 && !(am(i) <= -2.0 - margin);   // (NaN counts as oscillatory)
         const unsigned long long m = __ballot(osc);
         if (m) {
             t0 = top - (__ffsll((unsigned long long)m) - 1) + 1;
@@ -624,8 +630,8 @@ __device__ __forceinline__ int evanescent_rows(const double *a, int n, double s_
         const int i = base + lane;
         double d = 0.0;
         if (i < n) {
-            const double y = 0.5 * (-2.0 - (a[i] - s_min));   // x_i / 2 >= 0
-            d = 2.0 * log1p(y + sqrt(y * (y + 2.0)));          // 2 acosh(1 + y)
+            const double y = 0.5 * (-2.0 - am(i));          // x_i / 2 >= 0
+            d = 2.0 * log1p(y + sqrt(y * (y + 2.0)));       // 2 acosh(1 + y)
         }
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -641,6 +647,10 @@ __device__ __forceinline__ int evanescent_rows(const double *a, int n, double s_
     }
     const int n_eff = cut < n ? cut + 1 : n;
     return min(n, max(n_eff, min_keep));
+}
+
+__device__ __forceinline__ int evanescent_rows(const double *a, int n, double s_min, double decay, int min_keep) {
+    return evanescent_rows_at([&](int i) { return a[i] - s_min; }, n, decay, min_keep);
 }
 
 // forward continuants theta_{s-1}, theta_{r-1} (theta_{-1} = 1); rows 0 .. max(s,r)-1

@@ -99,10 +99,37 @@ struct WidePencil {
     static constexpr bool early_accept = false;
     WideSweeps<T, PQ> s;
 
-    __device__ __forceinline__ int count(double *ring, double k) const { return s.count(ring, k); }
-    __device__ __forceinline__ NewtonOut newton(double *ring, double k) const { return s.newton(ring, k); }
-    // (every row is swept: the wide-angle sweeps leave none out)
-    __device__ __forceinline__ void narrow(double) {}
+    int keep = 0;      // rows the final sweep records below (source / receiver + 2)
+    int dropped = 0;   // evanescent rows left out at the top, one negative pivot each
+    __device__ __forceinline__ int count(double *ring, double k) const { return s.count(ring, k) + dropped; }
+    __device__ __forceinline__ NewtonOut newton(double *ring, double k) const {
+        NewtonOut o = s.newton(ring, k);
+        o.cnt += dropped;
+        return o;
+    }
+    // The evanescent cut of the effective-sound-speed pencil with k as the shift: a_i(k) + 2 =
+    // dz^2 (kd + k cd + k^2 md) is a concave parabola in k (md < 0 below Mach 1) whose larger
+    // root is positive, so a row with a_i(k_min) <= -2 keeps a_i(k) <= -2 for every k >= k_min
+    // and contributes one negative pivot at every later shift.  (The margin keeps rows whose
+    // value sits within rounding of -2 in the sweep.)
+    __device__ __forceinline__ void narrow(double k) {
+        const int n0 = s.n;
+        int n1;
+        if constexpr (PQ) {
+            const double K = fma(-s.h2 * k, k, -2.0);
+            n1 = evanescent_rows_at(
+                [&](int i) {
+                    const double t = fma(-k, s.Q[i], s.P[i]);
+                    return fma(t, t, K) + (i == 0 ? s.beta : 0.0);
+                },
+                n0, 80.0, keep, 1e-12);
+        } else {
+            n1 = evanescent_rows_at(
+                [&](int i) { return fma(fma(s.C[i], k, s.B[i]), k, s.A[i]); }, n0, 80.0, keep, 1e-12);
+        }
+        dropped += n0 - n1;
+        s.n = n1;
+    }
     __device__ __forceinline__ int rows() const { return s.n; }
     // (counts always come from the sweep: no sign-only Newton phase)
     static constexpr bool sign_counts = false;
@@ -227,7 +254,7 @@ __global__ void __launch_bounds__(64) k_wcounts2(const double *__restrict__ A, c
 // one block per work item (as k_modes; the wide-angle path has no persistent form), staged in
 // tiles of T rows
 template <int MS, int T, bool PQ>
-__global__ void __launch_bounds__(64) k_wmodes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
+__global__ void __launch_bounds__(64, T == NCPA_WTILE_SMALL ? 4 : 1) k_wmodes(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
                                                const double *__restrict__ BC, size_t plane,
                                                const double *__restrict__ PQp, const double *__restrict__ beta,
                                                ModesOut out) {
@@ -236,6 +263,7 @@ __global__ void __launch_bounds__(64) k_wmodes(const ModeProb *__restrict__ mp, 
     if (wk.prob < 0) return;
     const ModeProb &P = mp[wk.prob];
     WidePencil<T, PQ> pen{wide_sweeps<T, PQ>(P, wk.prob, BC, plane, PQp, beta)};
+    pen.keep = max(P.srow, P.rrow) + 2;
     modes_item<MS>(P, pen, wk, sh, out);
 }
 
