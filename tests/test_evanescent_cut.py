@@ -111,3 +111,65 @@ def test_cut_is_conservative_without_evanescent_top(profile):
     n_eff = cut_rows(a1, h2 * k_max ** 2)
     assert n_eff < a1.size
     assert cut_rows(a1, h2 * k_max ** 2, min_keep=a1.size - 3) == a1.size - 3
+
+
+def _cut_rows_margin(am, decay, margin):
+    """The rule on row values am at the bound (the wide-angle pencil's a_i(k_min) = A + k (B +
+    k C)), rows with am <= -2 - margin evanescent."""
+    n = am.size
+    osc = np.nonzero(~(am <= -2.0 - margin))[0]
+    t0 = int(osc[-1]) + 1 if osc.size else 0
+    y = 0.5 * (-2.0 - am[t0:])
+    d = 2.0 * np.log1p(y + np.sqrt(y * (y + 2.0)))
+    hit = np.nonzero(np.cumsum(d) >= decay)[0]
+    return t0 + int(hit[0]) + 1 if hit.size else n
+
+
+def test_wide_angle_cut(profile):
+    """The wide-angle pencil dz^2 T(k) = tridiag(1, A + k (B + k C), 1) with k as the shift
+    (csrc/wide.hip WidePencil::narrow): a_i(k) + 2 is concave in k with a positive larger root
+    (subsonic winds), so rows evanescent at k_min stay evanescent above it; full counts equal
+    truncated counts plus the dropped rows, and QEP eigenvalues of the truncated pencil equal
+    the full ones (oracle/wmod.py's branch crossings) to working precision."""
+    import scipy.optimize
+    from oracle import wmod
+    nodes = AtmosphereNodes(read_profile(profile))
+    grid = wmod.WGrid(nodes, Nz=20000)
+    kd, md, cd, _, _, k_min, k_max, _ = wmod.modal_trace(grid, 1.0, 270.0, 0.0, 0.0, True)
+    A, B, C = wmod.pencil(kd, md, cd, grid.dz_m)
+    n = A.size
+    assert np.all(md < 0)
+    Pf = (A, B, C)
+    n_lo, n_hi = wmod._n_negative(Pf, k_min), wmod._n_negative(Pf, k_max)
+    assert n_hi - n_lo > 200
+
+    def count(Pm, ks):
+        a = Pm[0][:, None] + ks[None, :] * (Pm[1][:, None] + ks[None, :] * Pm[2][:, None])
+        d = a[-1] - 0.0
+        cnt = (d < 0).astype(np.int64)
+        for i in range(a.shape[0] - 2, -1, -1):
+            d = a[i] - 1.0 / np.where(d == 0.0, -1e-300, d)
+            cnt += d < 0
+        return cnt
+
+    def branch(Pm, i):
+        return scipy.optimize.brentq(lambda k: wmod._mu(Pm, k, i), k_min, k_max, xtol=1e-300,
+                                     rtol=4 * np.finfo(float).eps, maxiter=200)
+
+    rng = np.random.default_rng(3)
+    tested = 0
+    for b in (64, 192, n_hi - n_lo - 64):           # blocks of 64 branches
+        k0 = branch(Pf, n_lo + b)
+        k_lo = k0 * (1 - 1e-12)
+        n_eff = _cut_rows_margin(A + k_lo * (B + k_lo * C), DECAY, 1e-12)
+        if n_eff == n:
+            continue
+        tested += 1
+        Pt = (A[:n_eff], B[:n_eff], C[:n_eff])
+        k1 = branch(Pf, n_lo + b + 1)
+        ks = np.concatenate([[k0 * (1 + 1e-10), k1 * (1 - 1e-10)], rng.uniform(k_lo, k_max, 4)])
+        assert np.array_equal(count(Pf, ks), count(Pt, ks) + (n - n_eff))
+        for i in (n_lo + b, n_lo + b + 1):
+            kf, kt = branch(Pf, i), branch(Pt, i - (n - n_eff))
+            assert abs(kt - kf) <= 1e-14 * kf, (i, kf, kt)
+    assert tested >= 1
