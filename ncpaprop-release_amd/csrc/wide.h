@@ -363,7 +363,9 @@ struct WideSweeps {
         return cnt;
     }
 
-    // count plus det dz^2 T(k) and its k-derivative (Newton step -p0/q0 in k)
+    // count plus det dz^2 T(k) and its k-derivative (Newton step -p0/q0 in k); COUNT false
+    // leaves out the sign history (the count is then taken from det's sign by the caller)
+    template <bool COUNT = true>
     __device__ __forceinline__ NewtonOut newton(double *ring, double k) const {
         double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
         int cnt = 0;
@@ -375,15 +377,17 @@ struct WideSweeps {
             auto step = [&](double a, double m) {
                 const double p0 = fma(a, p1, -p2);
                 const double q0 = fma(a, q1, fma(m, p1, -q2));
-                bits = push_sign(bits, p0);
+                if constexpr (COUNT) bits = push_sign(bits, p0);
                 p2 = p1;
                 p1 = p0;
                 q2 = q1;
                 q1 = q0;
             };
             auto chunk_end = [&](int, int m) {
-                cnt += sign_changes(bits, m);
-                bits &= 1u;
+                if constexpr (COUNT) {
+                    cnt += sign_changes(bits, m);
+                    bits &= 1u;
+                }
                 const int e = pow2_exponent(p1);
                 p1 = ldexp(p1, -e);
                 p2 = ldexp(p2, -e);
@@ -415,15 +419,17 @@ struct WideSweeps {
                 const double da = fma(k2, v[2], v[1]);
                 const double p0 = fma(am, p1, -p2);
                 const double q0 = fma(am, q1, fma(da, p1, -q2));
-                bits = push_sign(bits, p0);
+                if constexpr (COUNT) bits = push_sign(bits, p0);
                 p2 = p1;
                 p1 = p0;
                 q2 = q1;
                 q1 = q0;
             },
             [&](int, int m) {
-                cnt += sign_changes(bits, m);
-                bits &= 1u;
+                if constexpr (COUNT) {
+                    cnt += sign_changes(bits, m);
+                    bits &= 1u;
+                }
                 const int e = pow2_exponent(p1);
                 p1 = ldexp(p1, -e);
                 p2 = ldexp(p2, -e);

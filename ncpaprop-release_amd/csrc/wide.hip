@@ -131,9 +131,18 @@ struct WidePencil {
         s.n = n1;
     }
     __device__ __forceinline__ int rows() const { return s.n; }
-    // (counts always come from the sweep: no sign-only Newton phase)
-    static constexpr bool sign_counts = false;
-    __device__ __forceinline__ NewtonOut newton_nocount(double *ring, double k) const { return s.newton(ring, k); }
+    // Newton sweeps without the sign history once every bracket isolates one branch crossing:
+    // the branches of T(k) decrease strictly, so counts are monotone in k and the sign of
+    // det T(k) is (-1)^count; each dropped row flips it once
+    static constexpr bool sign_counts = true;
+    __device__ __forceinline__ NewtonOut newton_nocount(double *ring, double k) const {
+        NewtonOut o = s.template newton<false>(ring, k);
+        if (dropped & 1) {
+            o.p0 = -o.p0;
+            o.q0 = -o.q0;
+        }
+        return o;
+    }
     __device__ __forceinline__ int rows_kept(double) const { return s.n; }
 
     // phase 3 at the converged k: psi = u / sqrt(dz (1 + k^2)), absorption u^T W u / (1 + k^2)
