@@ -43,54 +43,107 @@ struct NodeWc {
     __device__ double operator()(int k) const { return ws[k] * cos(wd[k] * PI / 180.0 - az); }
 };
 
-// Steffen spline of node values yf at z on interval k; end values outside the nodes
-// (AtmosphericProperty1D.cpp:116-126)
-template <class YF>
-__device__ __forceinline__ double spline_f(const double *zn, int n, YF yf, double z, int k) {
-    if (z > zn[n - 1]) return yf(n - 1);
-    if (z < zn[0]) return yf(0);
-    return steffen_eval_at(zn, n, yf, z, k);
-}
-
 // Rows of one problem at Hgt[i] = z_min + i dz_h (the setParams grid, WModeSolver.cpp:150,
 // 202-209), matrix spacing dz_m (WModeSolver.cpp:297):
 //   kd = (omega/c0)^2, md = (wc/c0)^2 - 1, cd = -2 omega wc/c0^2 (:549-553), the ground term
 //   1/((dz adm + 1) dz^2) added to kd[0] (:568-570), A = -2 + dz^2 kd, B = dz^2 cd,
 //   C = dz^2 md; W = 2 (omega/c_T) alpha (ModeSolver.cpp:276-278); CE = c_eff (:554); the
 //   factored rows P = dz omega/c0, Q = dz wc/c0 of the counts (wide.h).
-__global__ void k_wsetup_grid(const ProbDev *__restrict__ probs, int Nz, double z_min, double dz_h, double dz_m,
-                              const double *__restrict__ alpha_nodes, const double *__restrict__ ceff_nodes,
-                              int ldn, double *__restrict__ A, double *__restrict__ BC, size_t plane,
-                              double *__restrict__ W, double *__restrict__ CE, double *__restrict__ PQ) {
+// Block (x, p) takes rows [1024 x, 1024 x + 1024) of problem p; as k_setup_grid (ess.hip), the
+// Steffen coefficients of the node intervals those rows fall in are formed once per block in
+// LDS (c0, the wind component, c_eff, alpha, P, rho), so a row costs a search over the LDS
+// abscissae and three fma per property; windows wider than NCPA_WSETUP_IV intervals evaluate
+// every row from global memory.  Both paths form the same coefficients (steffen_coef).
+#define NCPA_WSETUP_ROWS 1024
+#define NCPA_WSETUP_IV 128
+__global__ void __launch_bounds__(256) k_wsetup_grid(const ProbDev *__restrict__ probs, int Nz, double z_min,
+                                                     double dz_h, double dz_m,
+                                                     const double *__restrict__ alpha_nodes,
+                                                     const double *__restrict__ ceff_nodes, int ldn,
+                                                     double *__restrict__ A, double *__restrict__ BC, size_t plane,
+                                                     double *__restrict__ W, double *__restrict__ CE,
+                                                     double *__restrict__ PQ) {
+    __shared__ SteffenCoef s_cf[6][NCPA_WSETUP_IV];
+    __shared__ double s_z[NCPA_WSETUP_IV + 1];
+    __shared__ int s_k[2];
     const int p = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Nz) return;
+    const int r0 = blockIdx.x * NCPA_WSETUP_ROWS, r1 = min(Nz, r0 + NCPA_WSETUP_ROWS);
     const ProbDev pr = probs[p];
     const int n = pr.n_nodes;
     const double *zn = pr.prof + PF_Z * n;
-    const double z = z_min + i * dz_h;
-    const int k = bsearch_interval(zn, n, z);
-    const double c0 = spline_f(zn, n, NodeArr{pr.prof + PF_C0 * n}, z, k);
-    const double wc = spline_f(zn, n, NodeWc{pr.prof + PF_WS * n, pr.prof + PF_WD * n, pr.az_rad}, z, k);
-    const double ceff = spline_f(zn, n, NodeArr{ceff_nodes + size_t(p) * ldn}, z, k);
+    const NodeArr c0n{pr.prof + PF_C0 * n}, cen{ceff_nodes + size_t(p) * ldn}, aln{alpha_nodes + size_t(p) * ldn},
+        pn{pr.prof + PF_P * n}, rn{pr.prof + PF_RHO * n};
+    const NodeWc wcn{pr.prof + PF_WS * n, pr.prof + PF_WD * n, pr.az_rad};
+    // node value k of property f: c0, wc, c_eff, alpha, P, rho
+    auto node = [&](int f, int k) {
+        switch (f) {
+            case 0: return c0n(k);
+            case 1: return wcn(k);
+            case 2: return cen(k);
+            case 3: return aln(k);
+            case 4: return pn(k);
+            default: return rn(k);
+        }
+    };
+    const double z0 = zn[0], z1 = zn[n - 1];
+    if (threadIdx.x == 0) s_k[0] = bsearch_interval(zn, n, z_min + r0 * dz_h);
+    if (threadIdx.x == 64) s_k[1] = bsearch_interval(zn, n, z_min + (r1 - 1) * dz_h);
+    __syncthreads();
+    const int k_lo = s_k[0], k_hi = s_k[1], ni = k_hi - k_lo + 1;
+    const bool lds = ni <= NCPA_WSETUP_IV;   // block-uniform
+    if (lds) {
+        for (int t = threadIdx.x; t < 6 * ni; t += blockDim.x) {
+            const int f = t / ni, k = k_lo + t % ni;
+            s_cf[f][k - k_lo] = steffen_coef(PtrNodes{zn}, n, [&](int kk) { return node(f, kk); }, k);
+        }
+        for (int t = threadIdx.x; t <= ni; t += blockDim.x) s_z[t] = zn[min(k_lo + t, n - 1)];
+        __syncthreads();
+    }
     const double om = pr.omega;
     const double h2 = dz_m * dz_m;
-    double kd = (om / c0) * (om / c0);
-    if (i == 0) kd = (1.0 / (dz_m * pr.admittance + 1)) / h2 + kd;
-    const double md = (wc / c0) * (wc / c0) - 1;
-    const double cd = -2 * om * (wc / (c0 * c0));
-    const double alpha = spline_f(zn, n, NodeArr{alpha_nodes + size_t(p) * ldn}, z, k);
-    const double Pz = spline_f(zn, n, NodeArr{pr.prof + PF_P * n}, z, k);
-    const double Rz = spline_f(zn, n, NodeArr{pr.prof + PF_RHO * n}, z, k);
-    const double cT = sqrt(1.4 * Pz / Rz);
-    const size_t o = size_t(p) * Nz + i;
-    A[o] = -2.0 + h2 * kd;
-    BC[o] = h2 * cd;
-    BC[plane + o] = h2 * md;
-    PQ[o] = dz_m * (om / c0);   // the factored rows: P = dz omega/c0, Q = dz wc/c0
-    PQ[plane + o] = dz_m * (wc / c0);
-    W[o] = (om / cT) * alpha * 2.0;
-    CE[o] = ceff;
+    for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        const double z = z_min + i * dz_h;
+        int k;
+        double dx = 0.0;
+        if (lds) {
+            // largest interval in [k_lo, k_hi] with z_k <= z (bsearch_interval's answer)
+            int lo = 0, hi = ni;
+            while (hi > lo + 1) {
+                const int mid = (hi + lo) >> 1;
+                if (s_z[mid] > z) hi = mid; else lo = mid;
+            }
+            k = lo;
+            dx = z - s_z[lo];
+        } else {
+            k = bsearch_interval(zn, n, z);
+        }
+        // spline value: end values outside the nodes (AtmosphericProperty1D.cpp:116-126)
+        auto at = [&](int f) {
+            if (z > z1) return node(f, n - 1);
+            if (z < z0) return node(f, 0);
+            if (lds) return steffen_poly(s_cf[f][k], dx);
+            return steffen_eval_zf(PtrNodes{zn}, n, [&](int kk) { return node(f, kk); }, z, k);
+        };
+        const double c0 = at(0);
+        const double wc = at(1);
+        const double ceff = at(2);
+        double kd = (om / c0) * (om / c0);
+        if (i == 0) kd = (1.0 / (dz_m * pr.admittance + 1)) / h2 + kd;
+        const double md = (wc / c0) * (wc / c0) - 1;
+        const double cd = -2 * om * (wc / (c0 * c0));
+        const double alpha = at(3);
+        const double Pz = at(4);
+        const double Rz = at(5);
+        const double cT = sqrt(1.4 * Pz / Rz);
+        const size_t o = size_t(p) * Nz + i;
+        A[o] = -2.0 + h2 * kd;
+        BC[o] = h2 * cd;
+        BC[plane + o] = h2 * md;
+        PQ[o] = dz_m * (om / c0);   // the factored rows: P = dz omega/c0, Q = dz wc/c0
+        PQ[plane + o] = dz_m * (wc / c0);
+        W[o] = (om / cT) * alpha * 2.0;
+        CE[o] = ceff;
+    }
 }
 
 // The pencil of problem p: A and W rows from the ModeProb, B and C from the BC planes.
@@ -401,7 +454,8 @@ int ncpa_wmod_solve(ncpa_ctx_t *ctx, const ncpa_grid_t *grid, const ncpa_problem
     {
         TimedLaunch tl(ctx, "setup");
         launch_node_tables(ctx->stream, d_probs, P, max_nodes, d_nodes);
-        hipLaunchKernelGGL(k_wsetup_grid, dim3((Nz + 255) / 256, P), dim3(256), 0, ctx->stream, d_probs, Nz,
+        hipLaunchKernelGGL(k_wsetup_grid, dim3((Nz + NCPA_WSETUP_ROWS - 1) / NCPA_WSETUP_ROWS, P), dim3(256), 0,
+                           ctx->stream, d_probs, Nz,
                            grid->z_min, grid->dz, gm.dz, d_nodes, d_nodes + size_t(P) * max_nodes, max_nodes,
                            m->d_A, m->d_BC, plane, m->d_W, m->d_CE, m->d_PQ);
     }
