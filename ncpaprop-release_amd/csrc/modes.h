@@ -249,6 +249,7 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
         const bool newton = !done && (ch - cl == 1);
         int cnt;
         double p0 = 0.0, q0 = 0.0;
+        bool have_det = true;   // (wave-uniform) every lane evaluated det and d det/ds at its shift
         if (Pen::sign_counts && __all(done || ch - cl == 1)) {
             // every shift of the wave lies strictly inside a bracket that holds one eigenvalue
             // (its own, or the one it helps with): the count there is cl or cl + 1, told apart
@@ -264,27 +265,68 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
             q0 = o.q0;
         } else {
             cnt = pen.count(s_ring, sh);
+            have_det = false;
         }
         if (!done) rows_used += pen.rows();
         ms_s[lane] = sh;
         ms_c[lane] = cnt;
+        // det and its derivative at every shift of the sweep (the helpers' bracket arrays are
+        // free again: every helper has read its bracket before the sweep)
+        double *ms_p = hb_lo, *ms_q = hb_hi;
+        if (have_det) {
+            ms_p[lane] = p0;
+            ms_q[lane] = q0;
+        }
         __syncthreads();
         if (!done) {
+            int lo_l = -1, hi_l = -1;                // lanes whose shifts are the new bracket ends
             for (int l = 0; l < 64; ++l) {           // counts are non-decreasing in the shift
                 const int cc = ms_c[l];
                 const double ss = ms_s[l];
                 if (cc <= j) {
-                    if (ss > lo) { lo = ss; cl = cc; }
+                    if (ss > lo) { lo = ss; cl = cc; lo_l = l; }
                 } else if (ss < hi) {
                     hi = ss;
                     ch = cc;
+                    hi_l = l;
                 }
             }
             sig = next_shift(lo, hi, cl, ch, j);
             double step = INFINITY;
-            if (newton && q0 != 0.0 && isfinite(p0) && isfinite(q0)) {
-                const double dlt = -p0 / q0;
-                const double cand = sh + dlt;
+            // Newton from the best of three points: the lane's own shift and the two shifts (the
+            // helpers' or other lanes') that now end its bracket, all with det known; near a
+            // close neighbour the own step converges only linearly from far off, while a bracket
+            // end the helpers put next to the root starts in the quadratic regime.  The smallest
+            // step wins (the point nearest the root).
+            double dlt = INFINITY, from = sh;
+            bool own = false;
+            if (newton && have_det) {
+                if (q0 != 0.0 && isfinite(p0) && isfinite(q0)) {
+                    dlt = -p0 / q0;
+                    own = true;
+                }
+                if (ch - cl == 1) {
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int l = e ? hi_l : lo_l;
+                        if (l < 0 || l == lane) continue;
+                        const double pe = ms_p[l], qe = ms_q[l];
+                        if (qe == 0.0 || !isfinite(pe) || !isfinite(qe)) continue;
+                        const double de = -pe / qe;
+                        const double ce = ms_s[l] + de;
+                        // strictly inside the bracket, whatever the step: a bracket end may be
+                        // a neighbour lane's converged shift, whose tiny step points at the
+                        // neighbouring eigenvalue, just outside
+                        if (fabs(de) < fabs(dlt) && ce > lo && ce < hi) {
+                            dlt = de;
+                            from = ms_s[l];
+                            own = false;
+                        }
+                    }
+                }
+            }
+            if (newton && isfinite(dlt)) {
+                const double cand = from + dlt;
                 const double adl = fabs(dlt);
                 // Converged when the step is at the tolerance, or (pencils with
                 // Pen::early_accept) when two successive Newton steps show the quadratic
@@ -296,9 +338,11 @@ __device__ __forceinline__ void modes_item(const ModeProb &P, Pen &pen, const Mo
                 // tolerance short; the factor covers it.  det T(k) of the wide-angle pencil has
                 // other zero branches close enough to break the prediction, so it waits for the
                 // step itself.
+                // (the prediction needs the step from the lane's own shift, the point the
+                // previous step led to)
                 const double rat = adl / prev_step;   // 0 after a bisection: no estimate yet
                 if (adl <= P.tol ||
-                    (Pen::early_accept && prev_step < INFINITY && rat <= 1e-3 && 1e3 * adl * rat * rat <= P.tol)) {
+                    (Pen::early_accept && own && prev_step < INFINITY && rat <= 1e-3 && 1e3 * adl * rat * rat <= P.tol)) {
                     sig = cand;
                     done = true;
                     my_iters = MS + it + 1;
