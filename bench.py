@@ -598,7 +598,7 @@ def field_main(args, cpu):
         avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
         achieved = FLOPS_PER_STURM_STEP * res["sturm_steps"] / avg_s / 1e12
         roof = {"kernel": "k_modes<2>", "bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f64"],
-                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"], "traffic": measured_traffic("k_modes<2>"),
+                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f64"], "traffic": measured_traffic("k_modes<2, false>"),
                 "algorithmic": "3 FP64 flops per Sturm step (row x sweep); FP64 VALU, MI355X FP64 peak"}
     modes_avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
     line = {
