@@ -501,7 +501,9 @@ __global__ void __launch_bounds__(256) k_colscale(const SplitProb *__restrict__ 
 }
 
 // E split: block = 32 modes x 64 ranges of one problem; phasors formed in FP64, split to
-// 16-bit parts and written transposed ([column][mode]) through LDS.
+// 16-bit parts and written transposed ([column][mode]).  The transpose runs in registers
+// (lane shuffles inside each octet of modes): the block holds 5 KB of LDS, so blocks fit in
+// the 16 KB that two 72 KB contraction blocks leave on a CU.
 struct ETile {
     int p, k0;
 };
@@ -513,7 +515,6 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
                                                 double dr, int nr, int lossless, double ev_re, double ev_im,
                                                 size_t split_stride, const int *__restrict__ SB, int nr_pad,
                                                 unsigned short *__restrict__ Bhat) {
-    __shared__ __attribute__((aligned(16))) unsigned short T[S][128][TKS + 8];
     __shared__ double s_sc[64];
     const ETile t = tiles[blockIdx.y];
     const SplitProb P = sp[t.p];
@@ -564,10 +565,14 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
         s_m8[mloc][1] = pi;
     }
     __syncthreads();
-    double zr = s_z[threadIdx.x >> 5][mloc][0], zi = s_z[threadIdx.x >> 5][mloc][1];
+    const int w = (int)threadIdx.x >> 5;
+    double zr = s_z[w][mloc][0], zi = s_z[w][mloc][1];
     const double mr = s_m8[mloc][0], mi = s_m8[mloc][1];
+    // wd[s][q]: part s of the mode's value at range w + 8q, real part in the low half
+    unsigned wd[S][8];
+#pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const int il = (threadIdx.x >> 5) + 8 * q;   // range within the tile, 0..63
+        const int il = w + 8 * q;   // range within the tile, 0..63
         const int i = i0 + il;
         double vr = 0.0, vi = 0.0;
         if (live && i < nr) {
@@ -580,23 +585,51 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
             zi = zr * mi + zi * mr;
             zr = nz;
         }
+#pragma unroll
         for (int s = 0; s < S; ++s) {
             double br, bi;
-            T[s][2 * il][mloc] = split_part<F16>(vr, br);
-            T[s][2 * il + 1][mloc] = split_part<F16>(vi, bi);
+            const unsigned lo = split_part<F16>(vr, br), hi = split_part<F16>(vi, bi);
+            wd[s][q] = lo | (hi << 16);
             vr -= br;
             vi -= bi;
         }
     }
-    __syncthreads();
-    // write 128 columns x 32 modes per split as 16-byte vectors (4 per column)
-    for (int s = 0; s < S; ++s)
-        for (int v = threadIdx.x; v < 128 * 4; v += 256) {
-            const int col = v >> 2, part = v & 3;
-            const size_t n = 2 * (size_t)i0 + col;
-            const uint4 val = *(const uint4 *)&T[s][col][part * 8];
-            *(uint4 *)&Bhat[s * split_stride + P.b_off + tiled_index((int)n, t.k0 + part * 8, P.Kp)] = val;
+    // 8 x 8 transpose among the 8 lanes j = mloc & 7 of an octet of modes (three butterfly
+    // exchanges): afterwards wd[s][m] of lane j holds mode 8 (mloc >> 3) + m at range w + 8j
+    const int j = mloc & 7;
+#pragma unroll
+    for (int b = 2; b >= 0; --b) {
+        const int d = 1 << b;
+        const bool up = (j & d) != 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if ((q & d) == 0) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    // both words read and written unconditionally: a select between two
+                    // addresses of the array would keep it out of registers
+                    const unsigned a = wd[s][q], c = wd[s][q | d];
+                    const unsigned r = (unsigned)__shfl_xor((int)(up ? a : c), d);
+                    wd[s][q] = up ? r : a;
+                    wd[s][q | d] = up ? c : r;
+                }
+            }
+    }
+    // columns 2 (i0 + w + 8j) (real) and + 1 (imaginary), modes k0 + 8 (mloc >> 3) .. + 7:
+    // one 16-byte vector each
+    const int n = 2 * (i0 + w + 8 * j), kk = t.k0 + 8 * (mloc >> 3);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        u4 re, im;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            re[x] = (wd[s][2 * x] & 0xffffu) | (wd[s][2 * x + 1] << 16);
+            im[x] = (wd[s][2 * x] >> 16) | (wd[s][2 * x + 1] & 0xffff0000u);
         }
+        unsigned short *b = Bhat + s * split_stride + P.b_off;
+        *(u4 *)&b[tiled_index(n, kk, P.Kp)] = re;
+        *(u4 *)&b[tiled_index(n + 1, kk, P.Kp)] = im;
+    }
 }
 
 template <bool F16>
