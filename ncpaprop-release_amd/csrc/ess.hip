@@ -574,10 +574,10 @@ __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, co
                                              double *__restrict__ Q0, int *__restrict__ QEX) {
     __shared__ double s_ring[NCPA_RING];
     __shared__ double rec[NCPA_CHUNK][64];
-    const ModeWork wk = work[blockIdx.x];
+    const ModeWork wk = work[blockIdx.x >> 1];   // block 2w: forward sweep of item w, 2w + 1: backward
     if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
-    rows_item(P, EssPencil{P, P.n}, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec);
+    rows_item(P, EssPencil{P, P.n}, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec, (blockIdx.x & 1) != 0);
 }
 __device__ __forceinline__ double log2abs_s(double v, int e) { return log2(fabs(v)) + (double)e; }
 
@@ -1042,10 +1042,10 @@ int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, do
     {
         TimedLaunch tl(ctx, "rows");
         if (m->wide)
-            launch_k_wrows(dim3((unsigned)m->n_work), ctx->stream, m->d_mp, m->d_work, m->d_BC,
+            launch_k_wrows(dim3(2 * (unsigned)m->n_work), ctx->stream, m->d_mp, m->d_work, m->d_BC,
                            (size_t)m->P * m->grid.Nz, m->d_PQ, m->d_beta, m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
         else
-            hipLaunchKernelGGL(k_rows, dim3((unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
+            hipLaunchKernelGGL(k_rows, dim3(2 * (unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
                                m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
         const unsigned mb = (unsigned)((ld + 63) / 64);
         hipLaunchKernelGGL(k_rows_ref, dim3(mb), dim3(64 * NCPA_REF_WAVES), 0, ctx->stream, TH, THE, PH, PHE, Q0, QEX,

@@ -387,12 +387,15 @@ struct RowsArgs {
 // The sweeps keep their row loops free of branches: every row's continuant goes to a
 // 16-row LDS buffer (one column per lane) and the recorded rows of a chunk are copied out
 // once the chunk is done, before its rescale.
+// The two sweeps are independent and run in separate blocks (`backward` false: theta, true:
+// phi and q): each wave is one dependent chain over the rows, so two blocks per work item
+// halve the kernel's length and fill twice as many SIMDs.
 template <class Pen>
 __device__ __forceinline__ void rows_item(const ModeProb &P, const Pen &pen, const ModeWork wk,
                                           const double *__restrict__ k2, RowsArgs ra, double *__restrict__ TH,
                                           int *__restrict__ THE, double *__restrict__ PH, int *__restrict__ PHE,
                                           double *__restrict__ Q0, int *__restrict__ QEX, double *s_ring,
-                                          double (*rec)[64]) {
+                                          double (*rec)[64], bool backward) {
     const int lane = (int)threadIdx.x;
     const int j = wk.j0 + lane;
     const bool active = j < P.n_hi;
@@ -405,7 +408,7 @@ __device__ __forceinline__ void rows_item(const ModeProb &P, const Pen &pen, con
     const size_t src = (size_t)ra.nrec * ra.ld + o;   // slot of the source row
     const int last_rec = ra.row0 + (ra.nrec - 1) * ra.stride;
     // forward: theta_k for k = 0 .. n-2, recorded at row k+1
-    {
+    if (!backward) {
         double t1 = 1.0, t2 = 0.0;
         int ex = 0;
         int slot = 0, next = ra.row0;   // next recorded row and its slot
@@ -446,7 +449,7 @@ __device__ __forceinline__ void rows_item(const ModeProb &P, const Pen &pen, con
             });
     }
     // backward: phi_i for i = n-1 .. 0, recorded at row i-1; q for the normalisation
-    {
+    else {
         double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
         int ex = 0;
         int slot = ra.nrec - 1, next = last_rec;

@@ -338,11 +338,11 @@ __global__ void __launch_bounds__(64) k_wrows(const ModeProb *__restrict__ mp, c
                                               double *__restrict__ Q0, int *__restrict__ QEX) {
     __shared__ double s_ring[3 * 2 * WT_ONE];
     __shared__ double rec[NCPA_CHUNK][64];
-    const ModeWork wk = work[blockIdx.x];
+    const ModeWork wk = work[blockIdx.x >> 1];   // block 2w: forward sweep of item w, 2w + 1: backward
     if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
     const WidePencil<WT_ONE, PQ> pen{wide_sweeps<WT_ONE, PQ>(P, wk.prob, BC, plane, PQp, beta)};
-    rows_item(P, pen, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec);
+    rows_item(P, pen, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec, (blockIdx.x & 1) != 0);
 }
 
 // Batches of more work items than the GPU has SIMDs run many waves per SIMD, where the
