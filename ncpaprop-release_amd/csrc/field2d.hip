@@ -133,12 +133,16 @@ typedef __attribute__((address_space(3))) void lds_void;
 // tiles fall into row_groups groups (1, 2, 4 or 8) and the XCDs into as many classes: XCD x
 // takes group x % row_groups and every (8 / row_groups)-th N tile from x / row_groups, its
 // slots running over the group's M tiles first, so consecutive blocks on an XCD share a B
-// tile and the XCD's L2 holds 1 / row_groups of A plus the B tiles in flight.  Grid:
-// 8 x ceil(tiles_m / row_groups) x ceil(n_tiles / (8 / row_groups)) blocks.
+// tile and the XCD's L2 holds 1 / row_groups of A plus the B tiles in flight.  The groups
+// deal the M tiles out in passes of row_groups, alternating direction (groups 0 1 1 0 0 1 ...
+// for two): the top row tiles, whose all-zero k-steps the folded contraction skips, are
+// shared out evenly, and so is any work that grows or shrinks along the rows.
+// Grid: 8 x ceil(tiles_m / row_groups) x ceil(n_tiles / (8 / row_groups)) blocks.
 __device__ __forceinline__ void xcd_tile(int bid, int tiles_m, int row_groups, int &tm, int &tn) {
     const int per = (tiles_m + row_groups - 1) / row_groups;
     const int xcd = bid & 7, slot = bid >> 3;
-    tm = (xcd % row_groups) * per + slot % per;
+    const int pass = slot % per, g = xcd % row_groups;
+    tm = pass * row_groups + ((pass & 1) ? row_groups - 1 - g : g);
     tn = (slot / per) * (8 / row_groups) + xcd / row_groups;
 }
 
@@ -434,6 +438,32 @@ __global__ void k_asplit(const double *__restrict__ psi, size_t ldpsi, const Spl
             v -= back;
         }
     }
+}
+
+// Per chunk of the split Psi operand (128 rows x 32 modes: one k-step of one row tile, the
+// tiled layout's unit): 1 if any of its parts holds a nonzero, else 0.  Modes above their work
+// item's evanescent cut have exactly zero rows there (+0 or -0, after the mode's sign flip),
+// so whole chunks of the top row tiles are zero; their products are zeros that sum to +0,
+// which the folded contraction skips.  Sign bits are masked: -0 counts as zero.  One wave
+// per chunk.
+template <int S>
+__global__ void __launch_bounds__(256) k_aflags(const unsigned short *__restrict__ Ahat, size_t split_stride,
+                                                size_t n_chunks, int *__restrict__ flag) {
+    const size_t c = blockIdx.x * (size_t)4 + (threadIdx.x >> 6);
+    const int lane = (int)threadIdx.x & 63;
+    if (c >= n_chunks) return;
+    unsigned any = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const u4 *q = (const u4 *)(Ahat + s * split_stride + c * (TBM * TKS));
+#pragma unroll
+        for (int u = 0; u < TBM * TKS / 8 / 64; ++u) {
+            const u4 v = q[lane + 64 * u];
+            any |= (v[0] | v[1] | v[2] | v[3]) & 0x7fff7fffu;
+        }
+    }
+    const bool nz = __ballot(any != 0u) != 0ull;
+    if (lane == 0) flag[c] = nz ? 1 : 0;
 }
 
 // Per mode: k_pert = kr + i ki and g = ev psi(zs) / sqrt(k_pert), the mode's factor in
@@ -789,7 +819,8 @@ __global__ void __launch_bounds__(256, 2) k_gemm_fold(const SplitProb *__restric
                                                       size_t split_stride_a, size_t split_stride_b,
                                                       float *__restrict__ C, size_t ldc, int M, int N,
                                                       int tiles_m, int n_tiles_n, const int *__restrict__ SA, int Mp,
-                                                      const int *__restrict__ SB, int nr_pad, int row_groups) {
+                                                      const int *__restrict__ SB, int nr_pad, int row_groups,
+                                                      const int *__restrict__ aflag) {
     constexpr int BN = 64;
     typedef unsigned short ATile[TBM][TKS];
     typedef unsigned short BTile[BN][TKS];
@@ -845,12 +876,12 @@ __global__ void __launch_bounds__(256, 2) k_gemm_fold(const SplitProb *__restric
         }
     };
     const int kq = lds_group(lane & 15, lane >> 4);
-    auto step = [&](ATile *Ac, BTile *Bc, ATile *An, BTile *Bn, int k0) {
+    auto step = [&](ATile *Ac, BTile *Bc, ATile *An, BTile *Bn, int k_next) {
         // this wave's loads of the current stage (the only ones in flight), then every wave's
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         // the other stage was last read in the previous k-step, which every wave has left
-        if (k0 + TKS < Kp) issue(An, Bn, k0 + TKS);
+        if (k_next < Kp) issue(An, Bn, k_next);
         // fragments in the order the products first use them
         u4 af[4][S], bfr[2][S];
 #pragma unroll
@@ -883,10 +914,42 @@ __global__ void __launch_bounds__(256, 2) k_gemm_fold(const SplitProb *__restric
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc64[a][b][v] += (double)acc[a][b][v];
     };
-    if (Kp > 0) issue(As0, Bs0, 0);   // a problem without modes has no operand block to read
-    for (int k0 = 0; k0 < Kp; k0 += 2 * TKS) {
-        step(As0, Bs0, As1, Bs1, k0);
-        if (k0 + TKS < Kp) step(As1, Bs1, As0, Bs0, k0 + TKS);
+    // The k-steps whose Psi chunk (this row tile) is not all zero, one bit each (nkt <= 128;
+    // without flags every k-step): a skipped k-step's products are exact zeros, and adding
+    // +0 to an FP64 accumulator that started at +0 changes no bit, so the field is the same.
+    const int nkt = Kp / TKS;
+    unsigned long long km0 = 0, km1 = 0;
+    if (aflag) {
+        const int *fl = aflag + (P.a_off >> 12) + (size_t)tm * nkt;
+        km0 = __ballot(lane < nkt && fl[lane] != 0);
+        km1 = __ballot(lane + 64 < nkt && fl[lane + 64] != 0);
+    } else {
+        km0 = nkt >= 64 ? ~0ull : (1ull << nkt) - 1ull;
+        km1 = nkt >= 128 ? ~0ull : (nkt > 64 ? (1ull << (nkt - 64)) - 1ull : 0ull);
+    }
+    // first flagged k-step at or after kt, as a mode offset (Kp if none)
+    auto next_k = [&](int kt) -> int {
+        if (kt < 64) {
+            const unsigned long long m = km0 & (~0ull << kt);
+            if (m) return (__ffsll((long long)m) - 1) * TKS;
+            kt = 64;
+        }
+        if (kt < 128) {
+            const unsigned long long m = km1 & (~0ull << (kt - 64));
+            if (m) return (64 + __ffsll((long long)m) - 1) * TKS;
+        }
+        return Kp;
+    };
+    int k0 = next_k(0);   // a problem without modes has no operand block to read
+    if (k0 < Kp) issue(As0, Bs0, k0);
+    while (k0 < Kp) {
+        int kn = next_k(k0 / TKS + 1);
+        step(As0, Bs0, As1, Bs1, kn);
+        k0 = kn;
+        if (k0 >= Kp) break;
+        kn = next_k(k0 / TKS + 1);
+        step(As1, Bs1, As0, Bs0, kn);
+        k0 = kn;
     }
     store_tile<F16, 2>(C + P.c_off, ldc, M, N, m0 + wm, n0 + wn, lane, acc64,
                        F16 ? SA + (size_t)blockIdx.z * Mp : nullptr, F16 ? SB + (size_t)blockIdx.z * nr_pad : nullptr);
@@ -1079,6 +1142,7 @@ struct ncpa_field_stage {
     SplitProb *d_sp = nullptr;
     unsigned short *Ahat = nullptr, *Bhat = nullptr;
     int *SA = nullptr, *SB = nullptr;
+    int *aflag = nullptr;   // per 128 x 32 chunk of Ahat: any nonzero (k_aflags); null: none skipped
     // FP64 / FP32
     const double *psi_rows = nullptr;
     GemmProb *d_gp = nullptr;
@@ -1108,6 +1172,7 @@ void stage_release(ncpa_field_stage *st) {
     c->pool.put(st->Bhat);
     c->pool.put(st->SA);
     c->pool.put(st->SB);
+    c->pool.put(st->aflag);
     c->pool.put(st->d_gp);
     c->pool.put(st->bh);
     c->pool.put(st->ah);
@@ -1143,6 +1208,12 @@ int stage_split(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
     st->Bhat = pool_alloc<unsigned short>(ctx, std::max<size_t>(S * sb, 8));
     st->SA = F16 ? pool_alloc<int>(ctx, (size_t)P * Mp) : nullptr;
     st->SB = F16 ? pool_alloc<int>(ctx, (size_t)P * nr_pad) : nullptr;
+    // chunk flags: the folded contraction holds one bit per k-step of a row tile, 128 at most
+    int max_kp = 0;
+    for (int p = 0; p < P; ++p) max_kp = std::max(max_kp, sp[p].Kp);
+    const size_t n_chunks = sa / (TBM * TKS);
+    const bool flags = max_kp <= 128 * TKS && n_chunks > 0;
+    st->aflag = flags ? pool_alloc<int>(ctx, n_chunks) : nullptr;
     // needed only while staging
     ETile *d_tiles = pool_alloc<ETile>(ctx, std::max<size_t>(tiles.size(), 1));
     double *LG = F16 ? pool_alloc<double>(ctx, std::max(total, 1)) : nullptr;
@@ -1152,7 +1223,8 @@ int stage_split(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
         ctx->pool.put(LG);
         ctx->pool.put(KI);
     };
-    if (!st->d_sp || !st->Ahat || !st->Bhat || !d_tiles || (F16 && (!st->SA || !st->SB || !LG || !KI))) {
+    if (!st->d_sp || !st->Ahat || !st->Bhat || !d_tiles || (F16 && (!st->SA || !st->SB || !LG || !KI)) ||
+        (flags && !st->aflag)) {
         put_temps();
         return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_field2d: allocation failed");
     }
@@ -1177,6 +1249,9 @@ int stage_split(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
         }
         hipLaunchKernelGGL((k_asplit<S, F16>), dim3(64, P), dim3(256), 0, ctx->stream, psi, (size_t)total, st->d_sp,
                            d_off, M, Mp, sa, st->SA, st->Ahat);
+        if (flags)
+            hipLaunchKernelGGL((k_aflags<S>), dim3((unsigned)((n_chunks + 3) / 4)), dim3(256), 0, ctx->stream,
+                               st->Ahat, sa, n_chunks, st->aflag);
         hipLaunchKernelGGL((k_esplit<S, F16>), dim3((n_r + 63) / 64, (unsigned)tiles.size()), dim3(256), 0,
                            ctx->stream, d_tiles, st->d_sp, d_off, d_k2, d_abs, d_psis, dr, n_r, lossless, ev.real(),
                            ev.imag(), sb, st->SB, nr_pad, st->Bhat);
@@ -1269,7 +1344,7 @@ void launch_split(ncpa_ctx *ctx, const ncpa_field_stage *st, float *out, int var
         else
             hipLaunchKernelGGL((k_gemm_fold<S, NP, F16>), g, dim3(256), 0, ctx->stream, st->d_sp, st->Ahat,
                                st->Bhat, st->sa, st->sb, out, (size_t)st->N, st->M, st->N, tm, tn, st->SA, st->Mp,
-                               st->SB, st->nr_pad, v.row_groups);
+                               st->SB, st->nr_pad, v.row_groups, st->aflag);
     } else {
         const ContractVariant v = SPLIT_VARIANTS[variant];
         const int tn = st->Np / TBN;
