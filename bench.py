@@ -315,15 +315,35 @@ def field_gate(ctx, grid, problems):
     return errs, chosen
 
 
-def issued_mfma_flops(prec, counts, kernel=""):
+def nonzero_chunks(psi, counts):
+    """Chunks (problem, 128-row tile, 32-mode k-step) of the field's Psi operand [FIELD_ROWS x
+    modes] that hold a nonzero (-0 counts as zero): k_gemm_fold skips the others."""
+    import numpy as np
+    a = psi.reshape(FIELD_ROWS, -1)
+    mp = -(-FIELD_ROWS // 128) * 128
+    n, off = 0, 0
+    for k in counts:
+        k = int(k)
+        kp = -(-k // 32) * 32
+        nz = np.zeros((mp, kp), dtype=bool)
+        nz[:FIELD_ROWS, :k] = a[:, off:off + k] != 0.0
+        off += k
+        n += int(nz.reshape(mp // 128, 128, kp // 32, 32).any(axis=(1, 3)).sum())
+    return n
+
+
+def issued_mfma_flops(prec, counts, kernel="", psi=None):
     """MFMA flops one split contraction issues: 2 x (rows padded to 128) x (2 ranges padded to
-    the block width: 64 for k_gemm_fold, else 128) x (sum of modes padded to 32) x products;
+    the block width: 64 for k_gemm_fold, else 128) x (sum of modes padded to 32) x products,
+    and for k_gemm_fold only over the chunks of Psi that hold a nonzero (psi: its host copy);
     None for the FP64 / FP32 forms."""
     if prec not in PREC_KERNEL:
         return None
     mp = -(-FIELD_ROWS // 128) * 128
     bw = 64 if kernel.startswith("k_gemm_fold<") else 128
     np_ = -(-2 * NR // bw) * bw
+    if bw == 64 and psi is not None:
+        return 2.0 * 128 * np_ * 32 * nonzero_chunks(psi, counts) * PRODUCTS[prec]
     kp = sum(-(-int(k) // 32) * 32 for k in counts)
     return 2.0 * mp * np_ * kp * PRODUCTS[prec]
 
@@ -400,7 +420,8 @@ def run_field(lanes, args, prec):
            "sturm_steps": float(last.sweep_rows().astype(np.float64).sum()),
            "field_flops": 4.0 * FIELD_ROWS * last.total * NR, "streams": L,
            "field_mfma_flops": issued_mfma_flops(prec, last.counts,
-                                                 lanes[0][0].field_variant_info(prec, lanes[0][0].field_variant(prec))[0])}
+                                                 lanes[0][0].field_variant_info(prec, lanes[0][0].field_variant(prec))[0],
+                                                 lane_state[0]["rows"].numpy() if prec in PREC_KERNEL else None)}
     last.free()
     for st in lane_state:
         for b in st["tl"] + [st.get("rows"), st.get("out")]:
@@ -592,8 +613,8 @@ def field_main(args, cpu):
             roof["issued_mfma"] = {
                 "tflops": issued, "frac": issued / peak,
                 "note": "MFMA work the kernel issues: %d part products per algorithmic product, "
-                        "padded shapes; SQ_INSTS_VALU_MFMA_MOPS x 512 agrees (profiles/"
-                        "field_mfma_counters.md)" % PRODUCTS[chosen]}
+                        "padded shapes, k-steps whose Psi chunk is all zero skipped by k_gemm_fold; "
+                        "SQ_INSTS_MFMA x 16384 agrees (profiles/field_mfma_counters.md)" % PRODUCTS[chosen]}
     else:
         avg_s = fams["modes"][0] / max(fams["modes"][1], 1) / 1e3
         achieved = FLOPS_PER_STURM_STEP * res["sturm_steps"] / avg_s / 1e12
