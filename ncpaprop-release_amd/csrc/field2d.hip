@@ -942,14 +942,15 @@ __global__ void __launch_bounds__(256, 2) k_gemm_fold(const SplitProb *__restric
     };
     int k0 = next_k(0);   // a problem without modes has no operand block to read
     if (k0 < Kp) issue(As0, Bs0, k0);
-    while (k0 < Kp) {
-        int kn = next_k(k0 / TKS + 1);
-        step(As0, Bs0, As1, Bs1, kn);
-        k0 = kn;
-        if (k0 >= Kp) break;
-        kn = next_k(k0 / TKS + 1);
-        step(As1, Bs1, As0, Bs0, kn);
-        k0 = kn;
+    while (k0 < Kp) {   // two k-steps per pass, one per LDS stage
+        const int k1 = next_k(k0 / TKS + 1);
+        step(As0, Bs0, As1, Bs1, k1);
+        int k2 = Kp;
+        if (k1 < Kp) {
+            k2 = next_k(k1 / TKS + 1);
+            step(As1, Bs1, As0, Bs0, k2);
+        }
+        k0 = k2;
     }
     store_tile<F16, 2>(C + P.c_off, ldc, M, N, m0 + wm, n0 + wn, lane, acc64,
                        F16 ? SA + (size_t)blockIdx.z * Mp : nullptr, F16 ? SB + (size_t)blockIdx.z * nr_pad : nullptr);
