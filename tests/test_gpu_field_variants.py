@@ -5,8 +5,11 @@ Within a precision every variant sums each output element with the same operatio
 same order, so each must return variant 0's field to the bit.  The batch has uneven mode
 counts (three problems at 0.3, 0.7 and 1 Hz), a row count (300) that is not a multiple of the
 128-row tiles and a column count (2 x 1100) that is not a multiple of 64 or 128, so the
-padded edges of every tile shape are crossed.  Variant 0 itself is checked against the oracle
-by test_gpu_field2d.py and test_gpu_field_2hz.py.
+padded edges of every tile shape are crossed.  Its top row tile (rows above ~117 km) holds
+all-zero 128 x 32 chunks of Psi (modes above their evanescent cut), which the folded form
+(k_gemm_fold, variants 0-2 of precisions 6 and 7) skips and the three-stage form (variants
+3-4) does not: their equality checks that the skip changes no bit.  Variant 0 itself is
+checked against the oracle by test_gpu_field2d.py and test_gpu_field_2hz.py.
 """
 import numpy as np
 import pytest
@@ -35,6 +38,27 @@ def batch(gpu_ctx, synth_profile, tmp_path_factory):
 
 def _bits(a):
     return a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+
+
+def test_batch_has_zero_chunks(batch):
+    """The batch crosses the zero-chunk skip: some (problem, 128-row tile, 32-mode k-step) chunk
+    of Psi is all zero (+0 or -0), and some chunk of the same row tile is not."""
+    m, rows = batch
+    psi = rows.numpy().reshape(NROWS, m.total)
+    mp = -(-NROWS // 128) * 128
+    zero = nonzero = 0
+    off = 0
+    for k in m.counts:
+        k = int(k)
+        kp = -(-k // 32) * 32
+        nz = np.zeros((mp, kp), dtype=bool)
+        nz[:NROWS, :k] = psi[:, off:off + k] != 0.0
+        off += k
+        chunk = nz.reshape(mp // 128, 128, kp // 32, 32).any(axis=(1, 3))
+        zero += int((~chunk).sum())
+        nonzero += int(chunk.sum())
+    print("case zero chunks: %d zero, %d nonzero" % (zero, nonzero), flush=True)
+    assert zero > 0 and nonzero > 0
 
 
 @pytest.mark.parametrize("prec", range(8))
