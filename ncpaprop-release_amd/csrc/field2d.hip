@@ -395,6 +395,25 @@ __device__ __forceinline__ unsigned short split_part(double v, double &back) {
     return __builtin_bit_cast(unsigned short, h);
 }
 
+// The parts of a pair (re, im) as one 32-bit word, real part in the low half, and the values
+// they stand for.  BF16: one packed conversion (round to nearest even, as the scalar one)
+// for both; FP16: two scalar conversions (the packed FP16 form rounds toward zero).
+template <bool F16>
+__device__ __forceinline__ unsigned split_pair(double vr, double vi, double &br, double &bi) {
+    if (F16) {
+        const unsigned lo = split_part<true>(vr, br), hi = split_part<true>(vi, bi);
+        return lo | (hi << 16);
+    }
+    typedef float fl2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bh2 __attribute__((ext_vector_type(2)));
+    float fr = (float)vr, fi = (float)vi;
+    asm volatile("" : "+v"(fr), "+v"(fi));
+    const unsigned w = __builtin_bit_cast(unsigned, __builtin_convertvector(fl2{fr, fi}, bh2));
+    br = (double)__builtin_bit_cast(float, w << 16);
+    bi = (double)__builtin_bit_cast(float, w & 0xffff0000u);
+    return w;
+}
+
 #define SPLIT_TOP 14   // scaled operands: max |value| in [2^14, 2^15), FP16 max 65504
 
 // exponent e with max * 2^e in [2^14, 2^15) (0 for an all-zero set)
@@ -530,6 +549,26 @@ __global__ void __launch_bounds__(256) k_colscale(const SplitProb *__restrict__ 
     SB[(size_t)p * nr_pad + i] = e;
 }
 
+// Per mode, for the E split: k_pert = kr + i ki, the factor g = gr + i gi (mode_factor) and the
+// exact per-range step e^{i k dr} = fr + i fi (FP64 phase reduced mod 2 pi, sincos, exp), each in
+// an array of n (structure of arrays, for coalesced reads by the E tiles of the mode).
+__global__ void __launch_bounds__(256) k_mode_fac(const double *__restrict__ k2, const double *__restrict__ absn,
+                                                  const double *__restrict__ psis, size_t n, int lossless,
+                                                  double ev_re, double ev_im, double dr, double *__restrict__ fac) {
+    const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    double kr, ki, gr, gi, sn, cs;
+    mode_factor(k2[o], absn[o], psis[o], lossless, ev_re, ev_im, kr, ki, gr, gi);
+    sincos(f2_reduce(kr * dr), &sn, &cs);
+    const double a1 = exp(-ki * dr);
+    fac[o] = kr;
+    fac[n + o] = ki;
+    fac[2 * n + o] = gr;
+    fac[3 * n + o] = gi;
+    fac[4 * n + o] = a1 * cs;
+    fac[5 * n + o] = a1 * sn;
+}
+
 // E split: block = 32 modes x 64 ranges of one problem; phasors formed in FP64, split to
 // 16-bit parts and written transposed ([column][mode]).  The transpose runs in registers
 // (lane shuffles inside each octet of modes): the block holds 5 KB of LDS, so blocks fit in
@@ -540,12 +579,12 @@ struct ETile {
 
 template <int S, bool F16>
 __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles, const SplitProb *__restrict__ sp,
-                                                const int *__restrict__ mode_off, const double *__restrict__ k2,
-                                                const double *__restrict__ absn, const double *__restrict__ psis,
-                                                double dr, int nr, int lossless, double ev_re, double ev_im,
-                                                size_t split_stride, const int *__restrict__ SB, int nr_pad,
+                                                const int *__restrict__ mode_off, const double *__restrict__ fac,
+                                                size_t n_modes, double dr, int nr, size_t split_stride,
+                                                const int *__restrict__ SB, int nr_pad,
                                                 unsigned short *__restrict__ Bhat) {
     __shared__ double s_sc[64];
+    __shared__ double s_z[8][32][2], s_m8[32][2];
     const ETile t = tiles[blockIdx.y];
     const SplitProb P = sp[t.p];
     const int i0 = blockIdx.x * 64;
@@ -553,9 +592,10 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
     const int k = t.k0 + mloc;               // mode within the problem
     double kr = 0, ki = 0, gr = 0, gi = 0;
     const bool live = k < P.K;
+    const size_t o = live ? (size_t)mode_off[t.p] + k : 0;
     if (live) {
-        const size_t o = (size_t)mode_off[t.p] + k;
-        mode_factor(k2[o], absn[o], psis[o], lossless, ev_re, ev_im, kr, ki, gr, gi);
+        gr = fac[2 * n_modes + o];
+        gi = fac[3 * n_modes + o];
     }
     // 1000 / sqrt(r) of the tile's 64 ranges (times 2^SB with F16)
     if (threadIdx.x < 64) {
@@ -565,23 +605,22 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
         s_sc[threadIdx.x] = sc;
     }
     // The thread's ranges are i0 + w + 8q (w = tid / 32).  Threads 0-31 form, for their mode,
-    // e^{i k r} exact at the tile's first range (FP64 phase reduced mod 2 pi, sincos, exp)
-    // and the exact factor e^{i k dr}; its powers give the 8 starting phasors (w = 0..7) and
-    // the step e^{i k 8 dr}, shared through LDS: one sincos/exp pair per mode and tile.
-    __shared__ double s_z[8][32][2], s_m8[32][2];
+    // e^{i k r} exact at the tile's first range (FP64 phase reduced mod 2 pi, sincos, exp);
+    // the powers of the exact factor e^{i k dr} (k_mode_fac) give the 8 starting phasors
+    // (w = 0..7) and the step e^{i k 8 dr}, shared through LDS: one sincos/exp per mode and tile.
     if (threadIdx.x < 32) {
         double ar = 0.0, ai = 0.0, fr = 0.0, fi = 0.0;
         if (live) {
+            kr = fac[o];
+            ki = fac[n_modes + o];
+            fr = fac[4 * n_modes + o];
+            fi = fac[5 * n_modes + o];
             const double r0 = (i0 + 1) * dr;
             double sn, cs;
             sincos(f2_reduce(kr * r0), &sn, &cs);
             const double a0 = exp(-ki * r0);
             ar = a0 * cs;
             ai = a0 * sn;
-            sincos(f2_reduce(kr * dr), &sn, &cs);
-            const double a1 = exp(-ki * dr);
-            fr = a1 * cs;
-            fi = a1 * sn;
         }
         double pr = 1.0, pi = 0.0;   // e^{i k w dr}
         for (int w = 0; w < 8; ++w) {
@@ -618,8 +657,7 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             double br, bi;
-            const unsigned lo = split_part<F16>(vr, br), hi = split_part<F16>(vi, bi);
-            wd[s][q] = lo | (hi << 16);
+            wd[s][q] = split_pair<F16>(vr, vi, br, bi);
             vr -= br;
             vi -= bi;
         }
@@ -1219,12 +1257,14 @@ int stage_split(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
     ETile *d_tiles = pool_alloc<ETile>(ctx, std::max<size_t>(tiles.size(), 1));
     double *LG = F16 ? pool_alloc<double>(ctx, std::max(total, 1)) : nullptr;
     double *KI = F16 ? pool_alloc<double>(ctx, std::max(total, 1)) : nullptr;
+    double *FAC = pool_alloc<double>(ctx, 6 * (size_t)std::max(total, 1));   // k_mode_fac
     auto put_temps = [&]() {
         ctx->pool.put(d_tiles);
         ctx->pool.put(LG);
         ctx->pool.put(KI);
+        ctx->pool.put(FAC);
     };
-    if (!st->d_sp || !st->Ahat || !st->Bhat || !d_tiles || (F16 && (!st->SA || !st->SB || !LG || !KI)) ||
+    if (!st->d_sp || !st->Ahat || !st->Bhat || !d_tiles || !FAC || (F16 && (!st->SA || !st->SB || !LG || !KI)) ||
         (flags && !st->aflag)) {
         put_temps();
         return set_error(ctx, NCPA_ERR_ALLOC, "ncpa_field2d: allocation failed");
@@ -1253,9 +1293,11 @@ int stage_split(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
         if (flags)
             hipLaunchKernelGGL((k_aflags<S>), dim3((unsigned)((n_chunks + 3) / 4)), dim3(256), 0, ctx->stream,
                                st->Ahat, sa, n_chunks, st->aflag);
+        hipLaunchKernelGGL(k_mode_fac, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis,
+                           (size_t)total, lossless, ev.real(), ev.imag(), dr, FAC);
         hipLaunchKernelGGL((k_esplit<S, F16>), dim3((n_r + 63) / 64, (unsigned)tiles.size()), dim3(256), 0,
-                           ctx->stream, d_tiles, st->d_sp, d_off, d_k2, d_abs, d_psis, dr, n_r, lossless, ev.real(),
-                           ev.imag(), sb, st->SB, nr_pad, st->Bhat);
+                           ctx->stream, d_tiles, st->d_sp, d_off, FAC, (size_t)total, dr, n_r, sb, st->SB, nr_pad,
+                           st->Bhat);
     }
     e = hipGetLastError();
     put_temps();   // stream-ordered: later users of these blocks follow the staging kernels
