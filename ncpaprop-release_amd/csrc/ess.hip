@@ -691,6 +691,12 @@ __global__ void __launch_bounds__(256) k_rows_combine(const double *__restrict__
     const size_t ro = (size_t)r * ra.ld + o;
     const double tr = TH[ro], pr = PH[ro];
     const int ter = THE[ro], per = PHE[ro];
+    // |u| = |theta phi| 2^(te + pe) 2^c with c = -log2|q_0| - log2|u_r|: c = ci + cf, ci an
+    // integer and 2^cf in [1, 2) formed once per mode, so each row takes a product and an
+    // exponent shift (the rescaled continuants keep |theta phi| far from over- and underflow)
+    const double c = -lq - 0.5 * l2r, ci = floor(c);
+    const double mc = exp2(c - ci);
+    const int ce = (int)ci;
     for (int k = k0; k < k1; ++k) {
         const size_t ko = (size_t)k * ra.ld + o;
         const int row = ra.row0 + k * ra.stride;
@@ -699,11 +705,9 @@ __global__ void __launch_bounds__(256) k_rows_combine(const double *__restrict__
         int tev, pev, d;
         if (row <= row_r) { tv = TH[ko]; tev = THE[ko]; pv = pr; pev = per; d = row_r - row; }
         else { tv = tr; tev = ter; pv = PH[ko]; pev = PHE[ko]; d = row - row_r; }
-        double u = 0.0;
-        if (tv != 0.0 && pv != 0.0) {
-            const double sg = ((d & 1) ? 1.0 : -1.0) * copysign(1.0, tv) * copysign(1.0, pv) * sq;
-            u = sg * exp2(log2abs_prod(tv, tev, pv, pev) - lq - 0.5 * l2r);
-        }
+        // sign -(-1)^d sgn(theta phi) sgn(q_0); zero factors give zero
+        const double sg = ((d & 1) ? 1.0 : -1.0) * sq;
+        const double u = sg * ldexp(tv * pv * mc, tev + pev + ce);
         psi[ko] = flip * u;
     }
 }
