@@ -1127,7 +1127,7 @@ def main():
                     help="field workload with --schedule streams: batches in flight on this many streams")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run the plain copy kernel over MIB MiB first (byte-counter calibration)")
-    ap.add_argument("--modes-form", type=int, choices=[0, 1], default=0,
+    ap.add_argument("--modes-form", type=int, choices=[0, 1], default=1,
                     help="field workload: eigenvalue sweeps read the diagonal through LDS (0) or scalar loads (1)")
     ap.add_argument("--headline-only", action="store_true",
                     help="field workload: skip the latency and broadband legs that follow the headline")

@@ -61,9 +61,8 @@ int ncpa_ctx_kernel_time(ncpa_ctx_t *ctx, const char *kernel, double *total_ms, 
 int ncpa_ctx_reset_timing(ncpa_ctx_t *ctx);
 
 /* How the eigenvalue sweeps of this context's eager solves (ncpa_ess_solve) read the matrix
- * diagonal: 0 (default) through an LDS ring, fastest beside a running 2-D field contraction;
- * 1 by scalar loads (no LDS, 6 waves per SIMD), fastest for batches that fill the GPU.  Both
- * give the same results to the bit.  A broadband plan created with tuning sets the form (and
+ * diagonal: 1 (default) by scalar loads (no LDS, 6 waves per SIMD); 0 through an LDS ring.
+ * Both give the same results to the bit.  A broadband plan created with tuning sets the form (and
  * the multisection depth) its autotuner picked.  No counterpart in the reference, which has
  * one CPU path. */
 int ncpa_ctx_set_modes_form(ncpa_ctx_t *ctx, int form);

@@ -62,7 +62,7 @@ struct ncpa_ctx {
     ncpa::DevicePool pool;
     std::map<void *, size_t> dev_allocs;   // ncpa_dev_alloc buffers and their sizes
     int ms_passes = 2;     // multisection passes of the eigenvalue kernel (autotuned by plans)
-    bool modes_smem = false;   // eager eigenvalue sweeps read the diagonal by scalar loads
+    bool modes_smem = true;    // eager eigenvalue sweeps read the diagonal by scalar loads
     int field_variant[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // contraction variant per precision (ncpa_field_tune)
 };
 

@@ -79,7 +79,7 @@ class Context:
         return ms.value, n.value
 
     def set_modes_form(self, form):
-        """Eigenvalue sweeps of eager solves: 0 diagonal through LDS, 1 through scalar loads."""
+        """Eigenvalue sweeps of eager solves: 1 (default) diagonal through scalar loads, 0 through LDS."""
         self.check(self.lib.ncpa_ctx_set_modes_form(self.h, int(form)))
 
     def reset_timing(self):
