@@ -395,8 +395,8 @@ __device__ __forceinline__ unsigned short split_part(double v, double &back) {
     return __builtin_bit_cast(unsigned short, h);
 }
 
-// The parts of a pair (re, im) as one 32-bit word, real part in the low half, and the values
-// they stand for.  BF16: one packed conversion (round to nearest even, as the scalar one)
+// The parts of a pair of values (re, im, or two consecutive modes) as one 32-bit word, the
+// first in the low half, and the values they stand for.  BF16: one packed conversion (round to nearest even, as the scalar one)
 // for both; FP16: two scalar conversions (the packed FP16 form rounds toward zero).
 template <bool F16>
 __device__ __forceinline__ unsigned split_pair(double vr, double vi, double &br, double &bi) {
@@ -438,51 +438,49 @@ __global__ void __launch_bounds__(256) k_rowscale(const double *__restrict__ psi
     if (lane == 0) SA[(size_t)p * Mp + row] = split_scale(mx);
 }
 
-// A split: one thread per (row, k) of the padded block of problem p (grid.y = p); with
-// F16 every row is first scaled by 2^SA[row].
+// A split, one wave per chunk of 128 rows x 32 modes of problem p (one k-step of one row tile,
+// the tiled layout's unit: 64 contiguous bytes per row and part), two rows per step: lanes
+// 0-31 take the modes of one row and lanes 32-63 those of the next, so each load reads two
+// 256-byte runs of psi and each store writes two 64-byte rows of a part.  With F16 every row
+// is first scaled by 2^SA[row].  With flag non-null the wave also writes the chunk's flag: 1
+// if any part holds a nonzero, else 0.  Modes above their work item's evanescent cut have
+// exactly zero rows there (+0 or -0, after the mode's sign flip), so whole chunks of the top
+// row tiles are zero; their products are zeros that sum to +0, which the folded contraction
+// skips (sign bits are masked: -0 counts as zero).  Grid (ceil(chunks of the largest problem
+// / 4), P), block 256.
 template <int S, bool F16>
-__global__ void k_asplit(const double *__restrict__ psi, size_t ldpsi, const SplitProb *__restrict__ sp,
-                         const int *__restrict__ col_off, int M, int Mp, size_t split_stride,
-                         const int *__restrict__ SA, unsigned short *__restrict__ Ahat) {
+__global__ void __launch_bounds__(256) k_asplit(const double *__restrict__ psi, size_t ldpsi,
+                                                const SplitProb *__restrict__ sp, const int *__restrict__ col_off,
+                                                int M, int Mp, size_t split_stride, const int *__restrict__ SA,
+                                                unsigned short *__restrict__ Ahat, int *__restrict__ flag) {
     const int p = blockIdx.y;
     const SplitProb P = sp[p];
-    const size_t n = (size_t)Mp * P.Kp;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const int row = (int)(e / P.Kp), k = (int)(e % P.Kp);
-        double v = (row < M && k < P.K) ? psi[(size_t)row * ldpsi + col_off[p] + k] : 0.0;
-        if (F16) v = ldexp(v, SA[(size_t)p * Mp + row]);
+    const int nkt = P.Kp / TKS;
+    const int c = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    if (c >= (Mp / TBM) * nkt) return;
+    const int tm = c / nkt, k0 = (c % nkt) * TKS;
+    const int k = lane & 31;
+    const bool live_k = k0 + k < P.K;
+    const double *src = psi + col_off[p] + k0 + k;
+    unsigned short *dst = Ahat + P.a_off + (size_t)c * (TBM * TKS) + k;
+    unsigned any = 0;
+    for (int rl = lane >> 5; rl < TBM; rl += 2) {
+        const int row = tm * TBM + rl;
+        double v = (row < M && live_k) ? src[(size_t)row * ldpsi] : 0.0;
+        if (F16) v = ldexp(v, row < M ? SA[(size_t)p * Mp + row] : 0);
+#pragma unroll
         for (int s = 0; s < S; ++s) {
             double back;
-            Ahat[s * split_stride + P.a_off + tiled_index(row, k, P.Kp)] = split_part<F16>(v, back);
+            const unsigned short h = split_part<F16>(v, back);
+            dst[s * split_stride + (size_t)rl * TKS] = h;
+            any |= h & 0x7fffu;
             v -= back;
         }
     }
-}
-
-// Per chunk of the split Psi operand (128 rows x 32 modes: one k-step of one row tile, the
-// tiled layout's unit): 1 if any of its parts holds a nonzero, else 0.  Modes above their work
-// item's evanescent cut have exactly zero rows there (+0 or -0, after the mode's sign flip),
-// so whole chunks of the top row tiles are zero; their products are zeros that sum to +0,
-// which the folded contraction skips.  Sign bits are masked: -0 counts as zero.  One wave
-// per chunk.
-template <int S>
-__global__ void __launch_bounds__(256) k_aflags(const unsigned short *__restrict__ Ahat, size_t split_stride,
-                                                size_t n_chunks, int *__restrict__ flag) {
-    const size_t c = blockIdx.x * (size_t)4 + (threadIdx.x >> 6);
-    const int lane = (int)threadIdx.x & 63;
-    if (c >= n_chunks) return;
-    unsigned any = 0;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const u4 *q = (const u4 *)(Ahat + s * split_stride + c * (TBM * TKS));
-#pragma unroll
-        for (int u = 0; u < TBM * TKS / 8 / 64; ++u) {
-            const u4 v = q[lane + 64 * u];
-            any |= (v[0] | v[1] | v[2] | v[3]) & 0x7fff7fffu;
-        }
+    if (flag) {
+        const bool nz = __ballot(any != 0u) != 0ull;
+        if (lane == 0) flag[(P.a_off >> 12) + c] = nz ? 1 : 0;
     }
-    const bool nz = __ballot(any != 0u) != 0ull;
-    if (lane == 0) flag[c] = nz ? 1 : 0;
 }
 
 // Per mode: k_pert = kr + i ki and g = ev psi(zs) / sqrt(k_pert), the mode's factor in
@@ -1181,7 +1179,7 @@ struct ncpa_field_stage {
     SplitProb *d_sp = nullptr;
     unsigned short *Ahat = nullptr, *Bhat = nullptr;
     int *SA = nullptr, *SB = nullptr;
-    int *aflag = nullptr;   // per 128 x 32 chunk of Ahat: any nonzero (k_aflags); null: none skipped
+    int *aflag = nullptr;   // per 128 x 32 chunk of Ahat: any nonzero (k_asplit); null: none skipped
     // FP64 / FP32
     const double *psi_rows = nullptr;
     GemmProb *d_gp = nullptr;
@@ -1251,6 +1249,7 @@ int stage_split(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
     int max_kp = 0;
     for (int p = 0; p < P; ++p) max_kp = std::max(max_kp, sp[p].Kp);
     const size_t n_chunks = sa / (TBM * TKS);
+    const int max_chunks = (Mp / TBM) * (max_kp / TKS);   // of one problem
     const bool flags = max_kp <= 128 * TKS && n_chunks > 0;
     st->aflag = flags ? pool_alloc<int>(ctx, n_chunks) : nullptr;
     // needed only while staging
@@ -1288,11 +1287,8 @@ int stage_split(ncpa_ctx *ctx, ncpa_field_stage *st, const int *hoff, const int 
             hipLaunchKernelGGL(k_colscale, dim3((nr_pad + 255) / 256, P), dim3(256), 0, ctx->stream, st->d_sp, d_off,
                                LG, KI, dr, n_r, nr_pad, st->SB);
         }
-        hipLaunchKernelGGL((k_asplit<S, F16>), dim3(64, P), dim3(256), 0, ctx->stream, psi, (size_t)total, st->d_sp,
-                           d_off, M, Mp, sa, st->SA, st->Ahat);
-        if (flags)
-            hipLaunchKernelGGL((k_aflags<S>), dim3((unsigned)((n_chunks + 3) / 4)), dim3(256), 0, ctx->stream,
-                               st->Ahat, sa, n_chunks, st->aflag);
+        hipLaunchKernelGGL((k_asplit<S, F16>), dim3((unsigned)((max_chunks + 3) / 4), P), dim3(256), 0, ctx->stream,
+                           psi, (size_t)total, st->d_sp, d_off, M, Mp, sa, st->SA, st->Ahat, st->aflag);
         hipLaunchKernelGGL(k_mode_fac, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, d_k2, d_abs, d_psis,
                            (size_t)total, lossless, ev.real(), ev.imag(), dr, FAC);
         hipLaunchKernelGGL((k_esplit<S, F16>), dim3((n_r + 63) / 64, (unsigned)tiles.size()), dim3(256), 0,

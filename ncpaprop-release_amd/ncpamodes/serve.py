@@ -93,7 +93,7 @@ def _field_bytes(nms, nrows, nr, precision):
         sa, sb = sum(Mp * k for k in Kp), sum(Np * k for k in Kp)
         held = _r256(SIZEOF["SplitProb"] * P) + _r256(2 * max(S * sa, 8)) + _r256(2 * max(S * sb, 8))
         if max(Kp) <= 128 * 32 and sa > 0:
-            held += _r256(4 * (sa // (128 * 32)))            # per-chunk flags of Ahat (k_aflags)
+            held += _r256(4 * (sa // (128 * 32)))            # per-chunk flags of Ahat (k_asplit)
         temps = _r256(SIZEOF["ETile"] * max(sum(Kp) // 32, 1)) + _r256(6 * 8 * tot)   # + per-mode factors
         if f16:
             held += _r256(4 * P * Mp) + _r256(4 * P * (Np // 2))
