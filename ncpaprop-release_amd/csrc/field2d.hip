@@ -620,10 +620,10 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
             ar = a0 * cs;
             ai = a0 * sn;
         }
-        double pr = 1.0, pi = 0.0;   // e^{i k w dr}
-        for (int w = 0; w < 8; ++w) {
-            s_z[w][mloc][0] = ar * pr - ai * pi;
-            s_z[w][mloc][1] = ar * pi + ai * pr;
+        double pr = 1.0, pi = 0.0;   // e^{i k v dr}
+        for (int v = 0; v < 8; ++v) {
+            s_z[v][mloc][0] = ar * pr - ai * pi;
+            s_z[v][mloc][1] = ar * pi + ai * pr;
             const double t = pr * fr - pi * fi;
             pi = pr * fi + pi * fr;
             pr = t;
