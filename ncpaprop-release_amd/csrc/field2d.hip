@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -567,9 +568,22 @@ __global__ void __launch_bounds__(256) k_mode_fac(const double *__restrict__ k2,
     fac[5 * n + o] = a1 * sn;
 }
 
+// x of the lane j ^ d (d = 1, 2 or 4, inside each group of 8 lanes) by DPP lane moves, with no
+// LDS traffic: quad permutations [1 0 3 2] and [2 3 0 1] for d = 1 and 2; for d = 4 the row
+// shifts by 4 (lane i reads lane i + 4, or lane i - 4), chosen by bit 2 of the lane.
+template <int D>
+__device__ __forceinline__ unsigned xor_lanes(unsigned x, int lane) {
+    const int v = (int)x;
+    if constexpr (D == 1) return (unsigned)__builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    if constexpr (D == 2) return (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    const int up = __builtin_amdgcn_update_dpp(0, v, 0x104, 0xF, 0xF, false);   // row_shl:4
+    const int dn = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);   // row_shr:4
+    return (unsigned)((lane & 4) ? dn : up);
+}
+
 // E split: block = 32 modes x 64 ranges of one problem; phasors formed in FP64, split to
 // 16-bit parts and written transposed ([column][mode]).  The transpose runs in registers
-// (lane shuffles inside each octet of modes): the block holds 5 KB of LDS, so blocks fit in
+// (DPP lane moves inside each octet of modes): the block holds 5 KB of LDS, so blocks fit in
 // the 16 KB that two 72 KB contraction blocks leave on a CU.
 struct ETile {
     int p, k0;
@@ -663,9 +677,8 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
     // 8 x 8 transpose among the 8 lanes j = mloc & 7 of an octet of modes (three butterfly
     // exchanges): afterwards wd[s][m] of lane j holds mode 8 (mloc >> 3) + m at range w + 8j
     const int j = mloc & 7;
-#pragma unroll
-    for (int b = 2; b >= 0; --b) {
-        const int d = 1 << b;
+    auto exchange = [&](auto dc) {
+        constexpr int d = decltype(dc)::value;
         const bool up = (j & d) != 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q)
@@ -675,12 +688,15 @@ __global__ void __launch_bounds__(256) k_esplit(const ETile *__restrict__ tiles,
                     // both words read and written unconditionally: a select between two
                     // addresses of the array would keep it out of registers
                     const unsigned a = wd[s][q], c = wd[s][q | d];
-                    const unsigned r = (unsigned)__shfl_xor((int)(up ? a : c), d);
+                    const unsigned r = xor_lanes<d>(up ? a : c, (int)threadIdx.x);
                     wd[s][q] = up ? r : a;
                     wd[s][q | d] = up ? c : r;
                 }
             }
-    }
+    };
+    exchange(std::integral_constant<int, 4>());
+    exchange(std::integral_constant<int, 2>());
+    exchange(std::integral_constant<int, 1>());
     // columns 2 (i0 + w + 8j) (real) and + 1 (imaginary), modes k0 + 8 (mloc >> 3) .. + 7:
     // one 16-byte vector each
     const int n = 2 * (i0 + w + 8 * j), kk = t.k0 + 8 * (mloc >> 3);
