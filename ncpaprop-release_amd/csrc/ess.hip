@@ -361,11 +361,17 @@ struct EssPencilT {
     __device__ __forceinline__ double qscale(double) const { return 1.0; }
     template <class Step, class ChunkEnd>
     __device__ __forceinline__ void up(double *ring, int m, double s, Step step, ChunkEnd ce) const {
-        sweep_up(p.a, ring, m, [&](double ak, int k) { step(ak - s, k); }, ce);
+        if constexpr (SMEM)
+            sweep_up_smem(p.a, m, [&](double ak, int k) { step(ak - s, k); }, ce);
+        else
+            sweep_up(p.a, ring, m, [&](double ak, int k) { step(ak - s, k); }, ce);
     }
     template <class Step, class ChunkEnd>
     __device__ __forceinline__ void down(double *ring, int m, double s, Step step, ChunkEnd ce) const {
-        sweep_down(p.a, ring, m, [&](double ai, int i) { step(ai - s, i); }, ce);
+        if constexpr (SMEM)
+            sweep_down_smem(p.a, m, [&](double ai, int i) { step(ai - s, i); }, ce);
+        else
+            sweep_down(p.a, ring, m, [&](double ai, int i) { step(ai - s, i); }, ce);
     }
     // phase 3: sensitivities at the converged eigenvalue (every lane, one sweep)
     __device__ __forceinline__ void finish(const ModeProb &P, double *s_ring, double sig, bool collapsed, int nclu,
@@ -566,18 +572,26 @@ __global__ void __launch_bounds__(1024) k_plan_build(const ProbDev *__restrict__
 // ------------------------------------------------------------------------------------
 // eigenvectors at a strided set of rows (plus the source row)
 // ------------------------------------------------------------------------------------
-// Eigenvector rows of the effective-sound-speed modes (rows_item in modes.h).
+// Eigenvector rows of the effective-sound-speed modes (rows_item in modes.h), in the sweep form
+// of the context's eigenvalue kernel: the diagonal through the LDS ring with the LDS record
+// buffer (SMEM false), or by scalar loads with the recorded rows stored as they are swept
+// (rows_item_direct; no LDS at all).
+template <bool SMEM>
 __global__ void __launch_bounds__(64) k_rows(const ModeProb *__restrict__ mp, const ModeWork *__restrict__ work,
                                              const double *__restrict__ k2, RowsArgs ra,
                                              double *__restrict__ TH, int *__restrict__ THE,
                                              double *__restrict__ PH, int *__restrict__ PHE,
                                              double *__restrict__ Q0, int *__restrict__ QEX) {
-    __shared__ double s_ring[NCPA_RING];
-    __shared__ double rec[NCPA_CHUNK][64];
     const ModeWork wk = work[blockIdx.x >> 1];   // block 2w: forward sweep of item w, 2w + 1: backward
     if (wk.prob < 0) return;
     const ModeProb P = mp[wk.prob];
-    rows_item(P, EssPencil{P, P.n}, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec, (blockIdx.x & 1) != 0);
+    if constexpr (SMEM) {
+        rows_item_direct(P, EssPencilT<true>{P, P.n}, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, (blockIdx.x & 1) != 0);
+    } else {
+        __shared__ double s_ring[NCPA_RING];
+        __shared__ double rec[NCPA_CHUNK][64];
+        rows_item(P, EssPencil{P, P.n}, wk, k2, ra, TH, THE, PH, PHE, Q0, QEX, s_ring, rec, (blockIdx.x & 1) != 0);
+    }
 }
 __device__ __forceinline__ double log2abs_s(double v, int e) { return log2(fabs(v)) + (double)e; }
 
@@ -1048,9 +1062,12 @@ int launch_rows(ncpa_ctx *ctx, ncpa_modes *m, int row0, int stride, int nrec, do
         if (m->wide)
             launch_k_wrows(dim3(2 * (unsigned)m->n_work), ctx->stream, m->d_mp, m->d_work, m->d_BC,
                            (size_t)m->P * m->grid.Nz, m->d_PQ, m->d_beta, m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
+        else if (ctx->modes_smem)
+            hipLaunchKernelGGL(k_rows<true>, dim3(2 * (unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp,
+                               m->d_work, m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
         else
-            hipLaunchKernelGGL(k_rows, dim3(2 * (unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp, m->d_work,
-                               m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
+            hipLaunchKernelGGL(k_rows<false>, dim3(2 * (unsigned)m->n_work), dim3(64), 0, ctx->stream, m->d_mp,
+                               m->d_work, m->d_k2, ra, TH, THE, PH, PHE, Q0, QEX);
         const unsigned mb = (unsigned)((ld + 63) / 64);
         hipLaunchKernelGGL(k_rows_ref, dim3(mb), dim3(64 * NCPA_REF_WAVES), 0, ctx->stream, TH, THE, PH, PHE, Q0, QEX,
                            m->d_mode_prob, m->d_mp, ra, ld, RS, RL, FL, m->d_psis_rows);

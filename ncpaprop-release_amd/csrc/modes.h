@@ -501,4 +501,134 @@ __device__ __forceinline__ void rows_item(const ModeProb &P, const Pen &pen, con
     }
 }
 
+// rows_item with the recorded rows stored as the sweep passes them (no record buffer, no LDS):
+// the recorded rows and the source row are wave-uniform, so each row's test is a scalar
+// compare and a branch taken only at the rows recorded; with the diagonal by scalar loads the
+// kernel touches no LDS at all.  The values and exponents stored are those of rows_item.
+template <class Pen>
+__device__ __forceinline__ void rows_item_direct(const ModeProb &P, const Pen &pen, const ModeWork wk,
+                                                 const double *__restrict__ k2, RowsArgs ra,
+                                                 double *__restrict__ TH, int *__restrict__ THE,
+                                                 double *__restrict__ PH, int *__restrict__ PHE,
+                                                 double *__restrict__ Q0, int *__restrict__ QEX, bool backward) {
+    const int lane = (int)threadIdx.x;
+    const int j = wk.j0 + lane;
+    const bool active = j < P.n_hi;
+    const size_t o = (size_t)P.out_off + (size_t)(active ? (j - P.n_lo) : 0);
+    const double s = active ? pen.shift(k2[o]) : 0.5 * (P.lo + P.hi);
+    const int n = pen.rows_kept(__shfl(s, 0));
+    const size_t src = (size_t)ra.nrec * ra.ld + o;
+    const int last_rec = ra.row0 + (ra.nrec - 1) * ra.stride;
+    if (!backward) {
+        // forward: theta_k for k = 0 .. n-2; row k+1 records theta_k
+        double t1 = 1.0, t2 = 0.0;
+        int ex = 0;
+        int slot = 0, next = ra.row0;
+        if (next == 0) {
+            if (active) { TH[o] = 1.0; THE[o] = 0; }
+            slot = 1;
+            next = ra.nrec > 1 ? ra.row0 + ra.stride : INT_MAX;
+        }
+        if (P.srow == 0 && active) { TH[src] = 1.0; THE[src] = 0; }
+        for (int k = slot, row = next; k < ra.nrec; ++k, row += ra.stride)   // rows left out
+            if (row >= n && active) {
+                TH[(size_t)k * ra.ld + o] = 0.0;
+                THE[(size_t)k * ra.ld + o] = 0;
+            }
+        if (next >= n) next = INT_MAX;
+        int srow = P.srow >= 1 && P.srow <= n - 1 ? P.srow : INT_MAX;
+        int ev = min(next, srow);   // the next row that stores anything: one test per row
+        pen.up(
+            nullptr, n - 1, s,
+            [&](double am, int k) {
+                const double t0 = fma(am, t1, -t2);
+                t2 = t1;
+                t1 = t0;
+                if (k + 1 == ev) {
+                    if (ev == next) {
+                        if (active) {
+                            TH[(size_t)slot * ra.ld + o] = t1;
+                            THE[(size_t)slot * ra.ld + o] = ex;
+                        }
+                        ++slot;
+                        next = (slot < ra.nrec && next + ra.stride < n) ? next + ra.stride : INT_MAX;
+                    }
+                    if (ev == srow) {
+                        if (active) {
+                            TH[src] = t1;
+                            THE[src] = ex;
+                        }
+                        srow = INT_MAX;
+                    }
+                    ev = min(next, srow);
+                }
+            },
+            [&](int, int) {
+                const int kk = pow2_exponent(t1);
+                t1 = ldexp(t1, -kk);
+                t2 = ldexp(t2, -kk);
+                ex += kk;
+            });
+    } else {
+        // backward: phi_i for i = n-1 .. 0; row i-1 records phi_i; q for the normalisation
+        double p1 = 1.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
+        int ex = 0;
+        int slot = ra.nrec - 1, next = last_rec;
+        for (; slot >= 0 && next >= n; --slot, next -= ra.stride)   // rows left out
+            if (active) {
+                PH[(size_t)slot * ra.ld + o] = 0.0;
+                PHE[(size_t)slot * ra.ld + o] = 0;
+            }
+        if (slot < 0) next = INT_MIN;
+        if (next == n - 1) {
+            if (active) { PH[(size_t)slot * ra.ld + o] = 1.0; PHE[(size_t)slot * ra.ld + o] = 0; }
+            --slot;
+            next = slot >= 0 ? next - ra.stride : INT_MIN;
+        }
+        if (P.srow == n - 1 && active) { PH[src] = 1.0; PHE[src] = 0; }
+        int srow = P.srow >= 0 && P.srow <= n - 2 ? P.srow : INT_MIN;
+        int ev = max(next, srow);   // the next row (going down) that stores anything
+        pen.down(
+            nullptr, n, s,
+            [&](double am, int i) {
+                const double p0 = fma(am, p1, -p2);
+                const double q0 = fma(am, q1, -(q2 + p1));
+                p2 = p1;
+                p1 = p0;
+                q2 = q1;
+                q1 = q0;
+                if (i - 1 == ev) {
+                    if (ev == next) {
+                        if (active) {
+                            PH[(size_t)slot * ra.ld + o] = p1;
+                            PHE[(size_t)slot * ra.ld + o] = ex;
+                        }
+                        --slot;
+                        next = slot >= 0 ? next - ra.stride : INT_MIN;
+                    }
+                    if (ev == srow) {
+                        if (active) {
+                            PH[src] = p1;
+                            PHE[src] = ex;
+                        }
+                        srow = INT_MIN;
+                    }
+                    ev = max(next, srow);
+                }
+            },
+            [&](int, int) {
+                const int k = pow2_exponent(p1);
+                p1 = ldexp(p1, -k);
+                p2 = ldexp(p2, -k);
+                q1 = ldexp(q1, -k);
+                q2 = ldexp(q2, -k);
+                ex += k;
+            });
+        if (active) {
+            Q0[o] = q1 * pen.qscale(s);
+            QEX[o] = ex;
+        }
+    }
+}
+
 }  // namespace ncpa
