@@ -60,9 +60,12 @@ int ncpa_ctx_set_timing(ncpa_ctx_t *ctx, int enable);
 int ncpa_ctx_kernel_time(ncpa_ctx_t *ctx, const char *kernel, double *total_ms, int *launches);
 int ncpa_ctx_reset_timing(ncpa_ctx_t *ctx);
 
-/* How the eigenvalue sweeps of this context's eager solves (ncpa_ess_solve) read the matrix
- * diagonal: 1 (default) by scalar loads (no LDS, 6 waves per SIMD); 0 through an LDS ring.
- * Both give the same results to the bit.  A broadband plan created with tuning sets the form (and
+/* How the eigenvalue sweeps of this context's eager solves (ncpa_ess_solve) and the sweeps of
+ * its eigenvector rows (ncpa_modes_rows) read the matrix diagonal: 1 (default) by scalar loads
+ * (no LDS, 6 waves per SIMD; the rows store their recorded values as the sweep passes them);
+ * 0 through an LDS ring (the rows keep their records in LDS).  The wide-angle solver
+ * (ncpa_wmod_solve) reads its rows through LDS in both forms.  Both forms give the same results
+ * to the bit.  A broadband plan created with tuning sets the form (and
  * the multisection depth) its autotuner picked.  No counterpart in the reference, which has
  * one CPU path. */
 int ncpa_ctx_set_modes_form(ncpa_ctx_t *ctx, int form);

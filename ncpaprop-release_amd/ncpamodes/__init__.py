@@ -79,7 +79,8 @@ class Context:
         return ms.value, n.value
 
     def set_modes_form(self, form):
-        """Eigenvalue sweeps of eager solves: 1 (default) diagonal through scalar loads, 0 through LDS."""
+        """Eigenvalue sweeps of eager solves and eigenvector-row sweeps (rows_device): 1 (default)
+        diagonal through scalar loads, 0 through LDS; the same results to the bit."""
         self.check(self.lib.ncpa_ctx_set_modes_form(self.h, int(form)))
 
     def reset_timing(self):
